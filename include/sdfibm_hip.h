@@ -1,0 +1,162 @@
+/*
+ * sdfibm_hip.h -- C ABI of the MI355X (gfx950) solid-fluid coupling step.
+ *
+ * One call of sdfibm_interact() is the reference's SolidCloud::interact(time, dt)
+ * (src/solidcloud.cpp:435-464): zero the solid fields, for every solid in index
+ * order enumerate the cells it touches (src/cellenumerator.cpp:6-78), compute
+ * each cell's solid volume fraction (src/geometrictools.cpp:6-116), add the
+ * direct-forcing terms and reduce the fluid force and torque on the solid
+ * (src/solidcloud.cpp:361-433), then clip As to at most 1
+ * (src/solidcloud.cpp:564-570).
+ *
+ * The mesh, U, As, Fs, Ts and Ct live on the device.  Plain C: pointers, sizes
+ * and status codes; no exceptions cross this boundary.  Every entry point
+ * returns SDFIBM_OK (0) or a negative sdfibm_status; sdfibm_last_error() gives
+ * the message of the last failure on a context.
+ *
+ * Threading: one context per host thread; calls on one context are not
+ * reentrant.  Ownership: the caller owns every host array it passes; the
+ * context owns its device memory.
+ */
+#ifndef SDFIBM_HIP_H
+#define SDFIBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDFIBM_ABI_VERSION 1
+
+typedef enum {
+    SDFIBM_OK = 0,
+    SDFIBM_ERR_ARG = -1,       /* bad argument (null pointer, size, dt <= 0, NaN state) */
+    SDFIBM_ERR_HIP = -2,       /* a HIP runtime call failed (message has the HIP error) */
+    SDFIBM_ERR_NOMEM = -3,     /* device allocation failed */
+    SDFIBM_ERR_SHAPE = -4,     /* unknown shape id or invalid shape parameters */
+    SDFIBM_ERR_TILE = -5,      /* a solid's cell tile exceeds the kernel's limits */
+    SDFIBM_ERR_STATE = -6,     /* call out of order (e.g. interact before set_solids) */
+    SDFIBM_ERR_MESH = -7,      /* unsupported or inconsistent mesh description */
+    SDFIBM_ERR_CAPACITY = -8,  /* an internal buffer was too small for this step */
+    SDFIBM_ERR_NODEVICE = -9   /* no HIP device available */
+} sdfibm_status;
+
+/* Shape ids.  Names and parameters follow the reference's shape plugins
+ * (src/libshape/ headers, registered by type name in shapefactory.h:11-39). */
+typedef enum {
+    SDFIBM_SHAPE_SPHERE = 0,         /* sphere.h:10-47        prm: radius, com(3)             */
+    SDFIBM_SHAPE_ELLIPSOID = 1,      /* ellipsoid.h:10-55     prm: radiusa, radiusb, radiusc  */
+    SDFIBM_SHAPE_CIRCLE = 2,         /* circle.h:10-47        prm: radius, com(3)             */
+    SDFIBM_SHAPE_ELLIPSE = 3,        /* ellipse.h:10-56       prm: radiusa, radiusb, -, com(3) */
+    SDFIBM_SHAPE_BOX = 4,            /* box.h:10-59           prm: radiusa,b,c, com(3)        */
+    SDFIBM_SHAPE_RECTANGLE = 5,      /* rectangle.h:10-60     prm: radiusa, radiusb, -, com(3) */
+    SDFIBM_SHAPE_PLANE = 6,          /* plane.h:10-30         no parameters; inside: local y<0 */
+    SDFIBM_SHAPE_CIRCLE_TAIL = 7,    /* circle_tail.h:10-63   prm: radius, ratio, thickness, com(3) */
+    SDFIBM_SHAPE_CIRCLE_TWOTAIL = 8  /* circle_twotail.h:10-68 prm: radius, ratio, thickness, com(3) */
+} sdfibm_shape_id;
+
+/* One solid: shape + state.  Replaces Solid (src/solid.h:14-50) as far as the
+ * coupling step reads it: centre, orientation, velocity, omega
+ * (solid.h:19-26) and the shape it points at (solid.h:48).  160 bytes. */
+typedef struct {
+    int32_t shape;     /* sdfibm_shape_id */
+    int32_t flags;     /* reserved, 0 */
+    double prm[6];     /* shape parameters, solidDict units */
+    double center[3];
+    double q[4];       /* orientation quaternion (w, x, y, z); world2local = conj(q) (p - center) q */
+    double vel[3];
+    double omega[3];
+} sdfibm_solid;
+
+typedef enum {
+    SDFIBM_MESH_RECTILINEAR = 0, /* points = xs x ys x zs tensor product; geometry in closed form */
+    SDFIBM_MESH_STRUCTURED = 1   /* one hex block with arbitrary point positions; geometry given */
+} sdfibm_mesh_kind;
+
+/* A logically structured hex mesh (one blockMesh block): points numbered
+ * i-fastest then j then k, cells likewise, cell points and faces in OpenFOAM's
+ * hex-model order.  Replaces MeshInfo (src/meshinfo.h:11-29): cellCells and
+ * cellPoints are implicit in (i, j, k); points, cell centres, volumes, face
+ * centres and face area vectors are the arrays below. */
+typedef struct {
+    int32_t kind;             /* sdfibm_mesh_kind */
+    int32_t nx, ny, nz;       /* cells per direction */
+    /* RECTILINEAR */
+    const double* xs;         /* [nx+1] */
+    const double* ys;         /* [ny+1] */
+    const double* zs;         /* [nz+1] */
+    /* STRUCTURED (all AoS xyz, FP64, as OpenFOAM stores them) */
+    const double* points;         /* [(nx+1)(ny+1)(nz+1) * 3] */
+    const double* cell_centres;   /* [nx*ny*nz * 3]  (mesh.C())  */
+    const double* cell_volumes;   /* [nx*ny*nz]      (mesh.V())  */
+    const double* face_centres[3];/* x-faces [(nx+1)*ny*nz*3], y-faces [nx*(ny+1)*nz*3], z-faces [nx*ny*(nz+1)*3] */
+    const double* face_areas[3];  /* same layout; area vectors (mesh.Sf()), sign ignored */
+} sdfibm_mesh_desc;
+
+typedef enum { SDFIBM_FP32 = 0, SDFIBM_FP64 = 1 } sdfibm_precision;
+
+typedef struct {
+    int32_t two_d;        /* meta.on_twod: cell apex z set to 0 (geometrictools.cpp:57-58) */
+    int32_t precision;    /* sdfibm_precision of the resident fields and the fraction arithmetic */
+    double rho_f;         /* fluid density multiplying force/torque; 0 when on_fluid = 0
+                             (solidcloud.cpp:247-251, 424-425) */
+    int64_t reserved[4];
+} sdfibm_config;
+
+typedef struct sdfibm_ctx sdfibm_ctx;
+
+/* Create a context on the current HIP device: copies the mesh to the device
+ * once and allocates U, As, Fs, Ts, Ct.  Replaces the SolidCloud constructor's
+ * field binding (solidcloud.cpp:209-217) and MeshInfo (meshinfo.h:22-29). */
+int sdfibm_create(const sdfibm_mesh_desc* mesh, const sdfibm_config* cfg, sdfibm_ctx** out);
+int sdfibm_destroy(sdfibm_ctx* ctx);
+
+/* Upload the solids (index order = the reference's m_solids order,
+ * solidcloud.h:49).  Validates shape ids and parameters. */
+int sdfibm_set_solids(sdfibm_ctx* ctx, const sdfibm_solid* solids, int64_t n);
+
+/* Fluid velocity U: host AoS FP64 [N*3] (a volVectorField's internal field),
+ * converted to the resident SoA layout in the context's precision. */
+int sdfibm_set_velocity(sdfibm_ctx* ctx, const double* U_aos);
+
+/* SolidCloud::interact(time, dt) (solidcloud.cpp:435-464).  force_torque, if
+ * not NULL, receives [n_solids * 6] FP64 (fx fy fz tx ty tz) per solid, the
+ * values the reference stores with setFluidForceAndTorque (solid.h:136-140).
+ * Fields stay on the device. */
+int sdfibm_interact(sdfibm_ctx* ctx, double time, double dt, double* force_torque);
+
+/* Device -> host copies of the solid fields as FP64 (any pointer may be NULL):
+ * As [N], Fs [N*3] AoS, Ts [N], Ct [N] (Ct stored as the reference's double field). */
+int sdfibm_get_fields(sdfibm_ctx* ctx, double* As, double* Fs_aos, double* Ts, double* Ct);
+
+/* Device pointers of the resident fields (element type per precision: float or
+ * double; Ct is int32).  For zero-copy use by a device-resident caller. */
+int sdfibm_field_pointers(sdfibm_ctx* ctx, void** U3 /* [3]: Ux,Uy,Uz */, void** As,
+                          void** Fs3 /* [3] */, void** Ts, int32_t** Ct);
+
+/* Candidate cells of one solid from the last interact, in the reference's
+ * summation order: ALL_INSIDE ascending, CENTER_INSIDE ascending,
+ * CENTER_OUTSIDE ascending (solidcloud.cpp:367-374).  counts[0..2] get the
+ * three set sizes, counts[3] the seed cell (-1: none); cells/alpha (may be
+ * NULL) get up to `capacity` entries.  For parity tests. */
+int sdfibm_candidates(sdfibm_ctx* ctx, int64_t solid, int64_t* counts, int64_t* cells,
+                      double* alpha, int64_t capacity);
+
+/* Timing of the phases of the last interact (milliseconds, HIP events on the
+ * context's stream): [0] setup+binning, [1] per-solid tiles, [2] field pass,
+ * [3] total.  Returns the number of values written (<= n). */
+int sdfibm_last_timing(sdfibm_ctx* ctx, double* ms, int n);
+
+/* Launch the work of interact on a caller-provided hipStream_t (NULL = the
+ * context's own stream). */
+int sdfibm_set_stream(sdfibm_ctx* ctx, void* hip_stream);
+
+const char* sdfibm_last_error(const sdfibm_ctx* ctx);
+const char* sdfibm_status_string(int status);
+int sdfibm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDFIBM_HIP_H */

@@ -1,0 +1,87 @@
+"""ctypes binding of ``libsdfibm_hip.so`` (the C ABI in include/sdfibm_hip.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for the coupling step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsdfibm_hip.so")
+
+# Every symbol include/sdfibm_hip.h declares.
+EXPORTS = (
+    "sdfibm_create", "sdfibm_destroy", "sdfibm_set_solids", "sdfibm_set_velocity",
+    "sdfibm_interact", "sdfibm_get_fields", "sdfibm_field_pointers", "sdfibm_candidates",
+    "sdfibm_last_timing", "sdfibm_set_stream", "sdfibm_last_error", "sdfibm_status_string",
+    "sdfibm_abi_version",
+)
+
+SDFIBM_OK = 0
+STATUS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_SHAPE", -5: "ERR_TILE",
+          -6: "ERR_STATE", -7: "ERR_MESH", -8: "ERR_CAPACITY", -9: "ERR_NODEVICE"}
+MESH_RECTILINEAR, MESH_STRUCTURED = 0, 1
+FP32, FP64 = 0, 1
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("xs", C.c_void_p), ("ys", C.c_void_p), ("zs", C.c_void_p),
+                ("points", C.c_void_p), ("cell_centres", C.c_void_p), ("cell_volumes", C.c_void_p),
+                ("face_centres", C.c_void_p * 3), ("face_areas", C.c_void_p * 3)]
+
+
+class Config(C.Structure):
+    _fields_ = [("two_d", C.c_int32), ("precision", C.c_int32), ("rho_f", C.c_double),
+                ("reserved", C.c_int64 * 4)]
+
+
+class SdfibmError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m sdfibm_amd.build` "
+                          "(the coupling step has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P, i64, i32, d = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+    sig = {
+        "sdfibm_create": (i32, [P, P, P]),
+        "sdfibm_destroy": (i32, [P]),
+        "sdfibm_set_solids": (i32, [P, P, i64]),
+        "sdfibm_set_velocity": (i32, [P, P]),
+        "sdfibm_interact": (i32, [P, d, d, P]),
+        "sdfibm_get_fields": (i32, [P, P, P, P, P]),
+        "sdfibm_field_pointers": (i32, [P, P, P, P, P, P]),
+        "sdfibm_candidates": (i32, [P, i64, P, P, P, i64]),
+        "sdfibm_last_timing": (i32, [P, P, i32]),
+        "sdfibm_set_stream": (i32, [P, P]),
+        "sdfibm_last_error": (C.c_char_p, [P]),
+        "sdfibm_status_string": (C.c_char_p, [i32]),
+        "sdfibm_abi_version": (i32, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status: int, ctx=None):
+    if status != SDFIBM_OK:
+        L = load()
+        msg = L.sdfibm_last_error(ctx)
+        raise SdfibmError(status, (msg or b"").decode())
+    return status

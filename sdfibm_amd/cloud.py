@@ -1,0 +1,149 @@
+"""Host-side mirror of the reference's ``SolidCloud`` coupling interface.
+
+``SolidCloud.interact(time, dt)`` is ``SolidCloud::interact`` (src/solidcloud.cpp:435-464):
+it zero-fills As/Fs/Ts/Ct, runs every solid's enumeration + fraction + forcing +
+force/torque reduction, clips As, and stores each solid's fluid force and torque
+(``Solid::setFluidForceAndTorque``, src/solid.h:136-140).  Here it calls the HIP
+kernels through the C ABI; the fields stay on the device and are copied back only
+on request.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .mesh import StructuredHexMesh
+from .solids import SOLID_DTYPE
+
+__all__ = ["SolidCloud"]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class SolidCloud:
+    """Coupling-step context on the current GPU.
+
+    mesh       StructuredHexMesh (rectilinear → closed-form geometry on the device; otherwise
+               the host geometry arrays are uploaded once)
+    two_d      meta.on_twod
+    rho_f      fluid density (0 reproduces on_fluid = 0)
+    precision  "fp64" (parity mode) or "fp32" (resident fields and fractions in FP32; the
+               inside tests and the force/torque reduction stay FP64)
+    """
+
+    def __init__(self, mesh: StructuredHexMesh, *, two_d: bool = False, rho_f: float = 1.0,
+                 precision: str = "fp64", force_structured: bool = False):
+        L = _lib.load()
+        self.mesh = mesh
+        self.n_cells = mesh.n_cells
+        self._keep = []
+        d = _lib.MeshDesc()
+        d.nx, d.ny, d.nz = mesh.nx, mesh.ny, mesh.nz
+        if mesh.rectilinear and not force_structured:
+            d.kind = _lib.MESH_RECTILINEAR
+            xs, ys, zs = (np.ascontiguousarray(a, dtype=np.float64) for a in (mesh.xs, mesh.ys, mesh.zs))
+            self._keep += [xs, ys, zs]
+            d.xs, d.ys, d.zs = xs.ctypes.data, ys.ctypes.data, zs.ctypes.data
+        else:
+            d.kind = _lib.MESH_STRUCTURED
+            g = mesh.geometry()
+            arrs = dict(points=np.ascontiguousarray(mesh.points, dtype=np.float64),
+                        cc=np.ascontiguousarray(g["cc"]), V=np.ascontiguousarray(g["V"]),
+                        fxc=np.ascontiguousarray(g["fx_c"]), fyc=np.ascontiguousarray(g["fy_c"]),
+                        fzc=np.ascontiguousarray(g["fz_c"]), fxs=np.ascontiguousarray(g["fx_s"]),
+                        fys=np.ascontiguousarray(g["fy_s"]), fzs=np.ascontiguousarray(g["fz_s"]))
+            self._keep += list(arrs.values())
+            d.points = arrs["points"].ctypes.data
+            d.cell_centres = arrs["cc"].ctypes.data
+            d.cell_volumes = arrs["V"].ctypes.data
+            for f, key in enumerate("xyz"):
+                d.face_centres[f] = arrs[f"f{key}c"].ctypes.data
+                d.face_areas[f] = arrs[f"f{key}s"].ctypes.data
+        cfg = _lib.Config()
+        cfg.two_d = 1 if two_d else 0
+        cfg.precision = _lib.FP64 if precision == "fp64" else _lib.FP32
+        cfg.rho_f = float(rho_f)
+        self.precision = precision
+        h = C.c_void_p()
+        st = L.sdfibm_create(C.byref(d), C.byref(cfg), C.byref(h))
+        _lib.check(st, None)
+        self._h = h
+        self._keep = []  # the library copied the mesh
+        self.n_solids = 0
+
+    # -- lifecycle ---------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().sdfibm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state -------------------------------------------------------------------------------
+    def set_solids(self, solids: np.ndarray):
+        solids = np.ascontiguousarray(solids, dtype=SOLID_DTYPE)
+        _lib.check(_lib.load().sdfibm_set_solids(self._h, _p(solids), solids.size), self._h)
+        self.n_solids = solids.size
+
+    def set_velocity(self, U: np.ndarray):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.n_cells, 3)
+        _lib.check(_lib.load().sdfibm_set_velocity(self._h, _p(U)), self._h)
+
+    # -- the coupling step -------------------------------------------------------------------
+    def interact(self, time: float, dt: float) -> np.ndarray:
+        ft = np.zeros((self.n_solids, 6))
+        _lib.check(_lib.load().sdfibm_interact(self._h, float(time), float(dt), _p(ft)), self._h)
+        return ft
+
+    def interact_device(self, time: float, dt: float) -> None:
+        """interact without copying force/torque back (it still waits for the step)."""
+        _lib.check(_lib.load().sdfibm_interact(self._h, float(time), float(dt), None), self._h)
+
+    def fields(self) -> dict:
+        n = self.n_cells
+        out = dict(As=np.zeros(n), Fs=np.zeros((n, 3)), Ts=np.zeros(n), Ct=np.zeros(n))
+        _lib.check(_lib.load().sdfibm_get_fields(self._h, _p(out["As"]), _p(out["Fs"]),
+                                                 _p(out["Ts"]), _p(out["Ct"])), self._h)
+        return out
+
+    def candidates(self, solid: int):
+        """(counts[4], cells, alpha) of one solid in the reference's summation order."""
+        L = _lib.load()
+        counts = np.zeros(4, dtype=np.int64)
+        _lib.check(L.sdfibm_candidates(self._h, int(solid), _p(counts), None, None, 0), self._h)
+        n = int(counts[:3].sum())
+        cells = np.zeros(n, dtype=np.int64)
+        alpha = np.zeros(n)
+        _lib.check(L.sdfibm_candidates(self._h, int(solid), _p(counts), _p(cells), _p(alpha), n),
+                   self._h)
+        return counts, cells, alpha
+
+    def timing(self) -> np.ndarray:
+        ms = np.zeros(4)
+        _lib.load().sdfibm_last_timing(self._h, _p(ms), 4)
+        return ms
+
+    def device_pointers(self) -> dict:
+        L = _lib.load()
+        U3 = (C.c_void_p * 3)()
+        F3 = (C.c_void_p * 3)()
+        As, Ts = C.c_void_p(), C.c_void_p()
+        Ct = C.c_void_p()
+        _lib.check(L.sdfibm_field_pointers(self._h, U3, C.byref(As), F3, C.byref(Ts), C.byref(Ct)),
+                   self._h)
+        return dict(U=[U3[i] for i in range(3)], As=As.value, Fs=[F3[i] for i in range(3)],
+                    Ts=Ts.value, Ct=Ct.value)

@@ -1,0 +1,1555 @@
+// interact.hip -- MI355X (gfx950) kernels of the solid-fluid coupling step and
+// the C ABI of include/sdfibm_hip.h.
+//
+// One interact (SolidCloud::interact, src/solidcloud.cpp:435-464) runs as:
+//
+//   k_setup        one thread per solid: shape constants, the solid's cell tile
+//                  (index box holding every cell with a vertex strictly inside),
+//                  the seed cell (nearest cell centre, solidcloud.cpp:363).
+//   k_scan         tile sizes -> offsets into the per-tile code buffer.
+//   k_solid        one workgroup per finite solid: inside-vertex bits of the
+//                  tile in LDS (64-bit rows along x), cells with >= 1 inside
+//                  vertex, the connected component grown from the seed (or the
+//                  first such cell when the seed has none; cellenumerator.cpp:
+//                  6-78) by bit-parallel flood fill, classification, volume
+//                  fractions of the cut cells, and the solid's force and torque
+//                  (FP64 block reduction).  Writes one code per tile cell.
+//   k_bin_*        per-brick lists of the solids whose tiles overlap the brick.
+//   k_fields       one workgroup per brick of cells: every cell of the mesh is
+//                  written exactly once -- As (clipped to 1), Fs, Ts, Ct summed
+//                  over the brick's solids in index order, planes evaluated in
+//                  place.  This fuses the zero-fill, the per-solid accumulation
+//                  and checkAlpha (solidcloud.cpp:438-441, 403-422, 564-570).
+//   k_plane_reduce force/torque of plane solids from per-brick partials.
+//
+// Tile codes (element type R): NaN = not a candidate; +inf = ALL_INSIDE
+// (alpha 1); finite >= 0 with sign bit clear = CENTER_INSIDE with that alpha;
+// sign bit set = CENTER_OUTSIDE with alpha = |code|.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sdfibm_hip.h"
+#include "mesh_dev.h"
+
+using namespace sdfibm_dev;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxPerBrick = 256;  // solids per brick handled by k_fields
+constexpr int kMaxPlanes = 8;
+
+enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_PAIRCAP = 4, EF_BRICK = 8 };
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ inline double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
+
+template <int NV>
+__device__ inline void wave_reduce(double* v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+        for (int t = 0; t < NV; ++t) v[t] += shfl_xor_d(v[t], off);
+}
+
+// Block-wide sum of NV doubles (blockDim.x multiple of 64); result valid in thread 0.
+template <int NV>
+__device__ inline void block_reduce(double* v, double* lds /* [NV * nwaves] */) {
+    wave_reduce<NV>(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int t = 0; t < NV; ++t) lds[t * 16 + wid] = v[t];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int t = 0; t < NV; ++t) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += lds[t * 16 + w];
+            v[t] = s;
+        }
+    }
+    __syncthreads();
+}
+
+template <class R>
+__device__ inline R code_none() {
+    return R(__builtin_nan(""));
+}
+
+// Run fill inside a row: bits of `a` reachable from `s` (s subset of a) along x.
+__device__ inline unsigned long long run_fill(unsigned long long a, unsigned long long s) {
+    const unsigned long long up = (((a + s) ^ a) & a) | s;
+    const unsigned long long ra = __builtin_bitreverse64(a), rs = __builtin_bitreverse64(s);
+    const unsigned long long dn = __builtin_bitreverse64((((ra + rs) ^ ra) & ra) | rs);
+    return up | dn;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_i32(int* p, long long n, int v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int MK>
+__global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, SolidK* __restrict__ K,
+                        TileInfo* __restrict__ T, long long* __restrict__ tile_cells, double* __restrict__ ft,
+                        int* __restrict__ err) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const SolidK k = make_solid_k(in[s]);
+    K[s] = k;
+    TileInfo t = make_tile<MK>(m, k);
+    t.seed = t.cls == TC_EMPTY ? -1 : nearest_cell<MK>(m, k.c);
+    if (t.cls == TC_TOOBIG) atomicOr(err, EF_TILE);
+    tile_cells[s] = (t.cls == TC_SMALL || t.cls == TC_LARGE) ? (long long)t.n[0] * t.n[1] * t.n[2] : 0;
+    T[s] = t;
+    for (int q = 0; q < 6; ++q) ft[6 * s + q] = 0.0;  // solids that touch no cell exert nothing
+}
+
+// Exclusive scan of n values (single block of 1024 threads).  out[n] = total.
+template <class T>
+__global__ void k_scan(const T* __restrict__ in, long long* __restrict__ out, long long n) {
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    const long long chunk = (n + 1023) / 1024;
+    const long long b = t * chunk, e = b + chunk < n ? b + chunk : n;
+    long long s = 0;
+    for (long long i = b; i < e; ++i) s += (long long)in[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = t > 0 ? part[t - 1] : 0;
+    for (long long i = b; i < e; ++i) {
+        const long long x = (long long)in[i];
+        out[i] = run;
+        run += x;
+    }
+    if (t == 1023) out[n] = part[1023];
+}
+
+__global__ void k_tile_offsets(TileInfo* __restrict__ T, const long long* __restrict__ off, int ns,
+                               long long cap, int* __restrict__ err) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0 && off[ns] > cap) atomicOr(err, EF_CODECAP);
+    if (s >= ns) return;
+    T[s].off = off[s];
+}
+
+// Per-solid tile pass.  One workgroup per solid of class CLS.
+template <class R, int MK, int MAXR, int CLS>
+__global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const SolidK* __restrict__ Ks,
+                                                  const TileInfo* __restrict__ Ts, const R* __restrict__ Ux,
+                                                  const R* __restrict__ Uy, const R* __restrict__ Uz,
+                                                  R* __restrict__ codes, long long code_cap, double dt,
+                                                  double rho_f, double* __restrict__ ft) {
+    const int s = blockIdx.x;
+    if (s >= ns) return;
+    const TileInfo T = Ts[s];
+    if (T.cls != CLS) return;
+    if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) return;
+
+    __shared__ unsigned long long vin[MAXR];
+    __shared__ unsigned long long act[MAXR];
+    __shared__ unsigned long long all8[MAXR];
+    __shared__ unsigned long long reach[MAXR];
+    __shared__ int rowpre[MAXR + 1];
+    __shared__ double red[6 * 16];
+    __shared__ int sh_first, sh_changed;
+
+    const SolidK K = Ks[s];
+    const int tid = threadIdx.x;
+    const int nx = T.n[0], ny = T.n[1], nz = T.n[2];
+    const int VX = nx + 1, VY = ny + 1;
+    const int VR = VY * (nz + 1), CR = ny * nz;
+    const unsigned long long cmask = (nx >= 64) ? ~0ull : ((1ull << nx) - 1ull);
+
+    for (int r = tid; r < VR; r += kBlock) vin[r] = 0ull;
+    if (tid == 0) sh_first = 0x7fffffff;
+    __syncthreads();
+
+    // 1. inside test at every tile vertex (FP64, reference operation order)
+    const long long NV = (long long)VX * VR;
+    for (long long v = tid; v < NV; v += kBlock) {
+        const int i = (int)(v % VX);
+        const int r = (int)(v / VX);
+        const int j = r % VY, k = r / VY;
+        const d3 p = vertex_pos<MK>(m, T.lo[0] + i, T.lo[1] + j, T.lo[2] + k);
+        if (inside_world(K, p)) atomicOr(&vin[r], 1ull << i);
+    }
+    __syncthreads();
+
+    // 2. per cell row: >= 1 inside vertex / all 8 inside
+    for (int r = tid; r < CR; r += kBlock) {
+        const int j = r % ny, k = r / ny;
+        const unsigned long long a = vin[k * VY + j], b = vin[k * VY + j + 1], c = vin[(k + 1) * VY + j],
+                                 d = vin[(k + 1) * VY + j + 1];
+        const unsigned long long o = a | b | c | d, n = a & b & c & d;
+        act[r] = (o | (o >> 1)) & cmask;
+        all8[r] = (n & (n >> 1)) & cmask;
+        reach[r] = 0ull;
+        if (act[r]) atomicMin(&sh_first, r * 64 + __builtin_ctzll(act[r]));
+    }
+    __syncthreads();
+
+    // 3. seed: nearest cell if it has an inside vertex, else the first cell (index
+    //    order) that has one.  Tile raster order is ascending cell index.
+    if (tid == 0) {
+        int seed_r = -1, seed_b = -1;
+        if (T.seed >= 0) {
+            const int si = T.seed % m.nx, sj = (T.seed / m.nx) % m.ny, sk = T.seed / (m.nx * m.ny);
+            const int li = si - T.lo[0], lj = sj - T.lo[1], lk = sk - T.lo[2];
+            if (li >= 0 && li < nx && lj >= 0 && lj < ny && lk >= 0 && lk < nz) {
+                const int r = lk * ny + lj;
+                if ((act[r] >> li) & 1ull) {
+                    seed_r = r;
+                    seed_b = li;
+                }
+            }
+        }
+        if (seed_r < 0 && sh_first != 0x7fffffff) {
+            seed_r = sh_first / 64;
+            seed_b = sh_first % 64;
+        }
+        if (seed_r >= 0) reach[seed_r] = run_fill(act[seed_r], 1ull << seed_b);
+        sh_changed = seed_r >= 0 ? 1 : 0;
+    }
+    __syncthreads();
+
+    // 4. flood fill over face neighbours within cells that have an inside vertex
+    while (sh_changed) {
+        __syncthreads();
+        if (tid == 0) sh_changed = 0;
+        __syncthreads();
+        int ch = 0;
+        for (int r = tid; r < CR; r += kBlock) {
+            const int j = r % ny, k = r / ny;
+            unsigned long long nb = reach[r];
+            if (j > 0) nb |= reach[r - 1];
+            if (j < ny - 1) nb |= reach[r + 1];
+            if (k > 0) nb |= reach[r - ny];
+            if (k < nz - 1) nb |= reach[r + ny];
+            const unsigned long long a = act[r];
+            const unsigned long long sd = nb & a;
+            if (sd & ~reach[r]) {
+                reach[r] = run_fill(a, sd);
+                ch = 1;
+            }
+        }
+        if (ch) sh_changed = 1;
+        __syncthreads();
+    }
+
+    // 5. cut cells (reached, not all inside): exclusive row prefix counts
+    {
+        __shared__ int part[kBlock];
+        const int chunk = (CR + kBlock - 1) / kBlock;
+        const int r0 = tid * chunk, r1 = min(r0 + chunk, CR);
+        int sum = 0;
+        for (int r = r0; r < r1; ++r) sum += __builtin_popcountll(reach[r] & ~all8[r]);
+        part[tid] = sum;
+        __syncthreads();
+        for (int off = 1; off < kBlock; off <<= 1) {
+            const int v = tid >= off ? part[tid - off] : 0;
+            __syncthreads();
+            part[tid] += v;
+            __syncthreads();
+        }
+        int run = tid > 0 ? part[tid - 1] : 0;
+        for (int r = r0; r < r1; ++r) {
+            rowpre[r] = run;
+            run += __builtin_popcountll(reach[r] & ~all8[r]);
+        }
+        if (tid == kBlock - 1) rowpre[CR] = part[kBlock - 1];
+        __syncthreads();
+    }
+    const int ncut = rowpre[CR];
+
+    const double dtINV = 1.0 / dt;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    auto force = [&](int gi, int gj, int gk, double alpha) {
+        const long long c = cell_id(m, gi, gj, gk);
+        const d3 cc = cell_centre<MK>(m, gi, gj, gk);
+        const double V = cell_volume<MK>(m, gi, gj, gk);
+        const d3 r = {cc.x - K.c[0], cc.y - K.c[1], cc.z - K.c[2]};
+        const d3 us = d3{K.v[0], K.v[1], K.v[2]} + cross(d3{K.w[0], K.w[1], K.w[2]}, r);
+        const d3 uf{(double)Ux[c], (double)Uy[c], (double)Uz[c]};
+        const d3 f = alpha * (uf - us);
+        const d3 tq = cross(r, f);
+        acc[0] += (f.x * V) * dtINV;
+        acc[1] += (f.y * V) * dtINV;
+        acc[2] += (f.z * V) * dtINV;
+        acc[3] += (tq.x * V) * dtINV;
+        acc[4] += (tq.y * V) * dtINV;
+        acc[5] += (tq.z * V) * dtINV;
+    };
+
+    R* tcodes = codes + T.off;
+    const long long NC = (long long)nx * CR;
+    // 6a. raster pass: codes of non-candidates and ALL_INSIDE cells, their forcing
+    for (long long c = tid; c < NC; c += kBlock) {
+        const int i = (int)(c % nx);
+        const int r = (int)(c / nx);
+        const unsigned long long bit = 1ull << i;
+        if (!(reach[r] & bit)) {
+            tcodes[c] = code_none<R>();
+        } else if (all8[r] & bit) {
+            tcodes[c] = R(__builtin_huge_val());
+            force(T.lo[0] + i, T.lo[1] + r % ny, T.lo[2] + r / ny, 1.0);
+        }
+    }
+    // 6b. compacted pass over the cut cells: fraction, centre test, forcing
+    for (int q = tid; q < ncut; q += kBlock) {
+        int lo = 0, hi = CR;  // row r with rowpre[r] <= q < rowpre[r+1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (rowpre[mid] <= q)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const int r = lo;
+        unsigned long long bits = reach[r] & ~all8[r];
+        for (int t = q - rowpre[r]; t > 0; --t) bits &= bits - 1;
+        const int i = __builtin_ctzll(bits);
+        const int gi = T.lo[0] + i, gj = T.lo[1] + r % ny, gk = T.lo[2] + r / ny;
+        const R alpha = fraction_of_cell<R, MK>(m, K, gi, gj, gk);
+        const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
+        tcodes[(long long)r * nx + i] = cin ? alpha : -alpha;
+        force(gi, gj, gk, (double)alpha);
+    }
+    block_reduce<6>(acc, red);
+    if (tid == 0)
+        for (int t = 0; t < 6; ++t) ft[6 * s + t] = acc[t] * rho_f;
+}
+
+template <int BX, int BY, int BZ>
+__global__ void k_bin(int ns, MeshDev m, const TileInfo* __restrict__ T, int nbx, int nby, int* __restrict__ count,
+                      const long long* __restrict__ boff, int* __restrict__ blist, long long pair_cap,
+                      int* __restrict__ err, int fill) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const TileInfo t = T[s];
+    if (t.cls != TC_SMALL && t.cls != TC_LARGE) return;
+    const int b0x = t.lo[0] / BX, b1x = (t.lo[0] + t.n[0] - 1) / BX;
+    const int b0y = t.lo[1] / BY, b1y = (t.lo[1] + t.n[1] - 1) / BY;
+    const int b0z = t.lo[2] / BZ, b1z = (t.lo[2] + t.n[2] - 1) / BZ;
+    for (int bz = b0z; bz <= b1z; ++bz)
+        for (int by = b0y; by <= b1y; ++by)
+            for (int bx = b0x; bx <= b1x; ++bx) {
+                const long long b = bx + (long long)nbx * (by + (long long)nby * bz);
+                const int slot = atomicAdd(&count[b], 1);
+                if (fill) {
+                    const long long at = boff[b] + slot;
+                    if (at < pair_cap)
+                        blist[at] = s;
+                    else
+                        atomicOr(err, EF_PAIRCAP);
+                }
+            }
+}
+
+// Brick culling for a plane (inside: local y < 0, local y = n . (p - c) with
+// n = R(q) e_y): 0 = no vertex of the box can be inside, 1 = every vertex is
+// inside, 2 = undecided.  The margin covers the rounding of the exact test.
+__device__ inline int plane_box_state(const SolidK& K, const double* bb) {
+    double Rm[3][3];
+    rot_of(K.q, Rm);
+    const double n0 = Rm[0][1], n1 = Rm[1][1], n2 = Rm[2][1];
+    const double cx = 0.5 * (bb[0] + bb[1]) - K.c[0], cy = 0.5 * (bb[2] + bb[3]) - K.c[1],
+                 cz = 0.5 * (bb[4] + bb[5]) - K.c[2];
+    const double ext = fabs(n0) * 0.5 * (bb[1] - bb[0]) + fabs(n1) * 0.5 * (bb[3] - bb[2]) +
+                       fabs(n2) * 0.5 * (bb[5] - bb[4]);
+    const double mid = n0 * cx + n1 * cy + n2 * cz;
+    const double tol = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + ext + fabs(K.c[0]) + fabs(K.c[1]) + fabs(K.c[2])) + 1e-300;
+    if (mid - ext > tol) return 0;
+    if (mid + ext < -tol) return 1;
+    return 2;
+}
+
+template <int MK, int BX, int BY, int BZ>
+__device__ inline void brick_vertex_box(const MeshDev& m, const double* brick_box, long long b, int bx, int by, int bz,
+                                        double* bb) {
+    if constexpr (MK == MK_RECT) {
+        const int i0 = bx * BX, j0 = by * BY, k0 = bz * BZ;
+        const int i1 = min(i0 + BX, m.nx), j1 = min(j0 + BY, m.ny), k1 = min(k0 + BZ, m.nz);
+        bb[0] = m.xs[i0];
+        bb[1] = m.xs[i1];
+        bb[2] = m.ys[j0];
+        bb[3] = m.ys[j1];
+        bb[4] = m.zs[k0];
+        bb[5] = m.zs[k1];
+    } else {
+        for (int t = 0; t < 6; ++t) bb[t] = brick_box[6 * b + t];
+    }
+}
+
+__device__ inline bool bit_get(const unsigned* bits, long long c) { return (bits[c >> 5] >> (c & 31)) & 1u; }
+
+// Cells of the mesh with >= 1 inside vertex of a plane: bitmap, list, lowest index.
+template <int MK, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX* BY* BZ)
+    k_plane_active(MeshDev m, int nbx, int nby, const double* __restrict__ brick_box, SolidK K,
+                   unsigned* __restrict__ act, int* __restrict__ list, int* __restrict__ nlist, long long cap,
+                   int* __restrict__ minidx) {
+    const long long b = blockIdx.x;
+    const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long long)nbx * nby));
+    double bb[6];
+    brick_vertex_box<MK, BX, BY, BZ>(m, brick_box, b, bx, by, bz, bb);
+    const int st = plane_box_state(K, bb);
+    if (st == 0) return;
+    const int tid = threadIdx.x;
+    const int i = bx * BX + tid % BX, j = by * BY + (tid / BX) % BY, k = bz * BZ + tid / (BX * BY);
+    if (i >= m.nx || j >= m.ny || k >= m.nz) return;
+    int nin = 8;
+    if (st == 2) {
+        d3 P[8];
+        cell_vertices<MK>(m, i, j, k, P);
+        nin = 0;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) nin += inside_world(K, P[v]) ? 1 : 0;
+    }
+    if (nin == 0) return;
+    const long long c = cell_id(m, i, j, k);
+    atomicOr(&act[c >> 5], 1u << (c & 31));
+    const int slot = atomicAdd(nlist, 1);
+    if (slot < cap) list[slot] = (int)c;
+    atomicMin(minidx, (int)c);
+}
+
+__device__ inline int uf_find(const int* L, int x) {
+    int p = L[x];
+    while (p != x) {
+        x = p;
+        p = L[x];
+    }
+    return x;
+}
+
+__global__ void k_ccl_init(const int* __restrict__ list, const int* __restrict__ nlist, int* __restrict__ L) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < *nlist) L[list[t]] = list[t];
+}
+
+// Union over face neighbours (+x, +y, +z) of active cells: hook the larger root
+// under the smaller one.
+__global__ void k_ccl_hook(MeshDev m, const int* __restrict__ list, const int* __restrict__ nlist,
+                           const unsigned* __restrict__ act, int* L, int* changed) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *nlist) return;
+    const int c = list[t];
+    const int i = c % m.nx, j = (c / m.nx) % m.ny, k = c / (m.nx * m.ny);
+    int nb[3];
+    int nn = 0;
+    if (i + 1 < m.nx) nb[nn++] = c + 1;
+    if (j + 1 < m.ny) nb[nn++] = c + m.nx;
+    if (k + 1 < m.nz) nb[nn++] = c + m.nx * m.ny;
+    for (int q = 0; q < nn; ++q) {
+        if (!bit_get(act, nb[q])) continue;
+        int a = uf_find(L, c), b2 = uf_find(L, nb[q]);
+        while (a != b2) {
+            const int hi = a > b2 ? a : b2, lo = a > b2 ? b2 : a;
+            const int old = atomicMin(&L[hi], lo);
+            *changed = 1;
+            if (old == hi) break;  // hi was a root and now points at lo
+            a = uf_find(L, old);
+            b2 = uf_find(L, lo);
+        }
+    }
+}
+
+__global__ void k_ccl_compress(const int* __restrict__ list, const int* __restrict__ nlist, int* L) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *nlist) return;
+    const int c = list[t];
+    L[c] = uf_find(L, c);
+}
+
+// Cells of the seed's component (seed: the nearest cell if active, else the
+// lowest-index active cell) -> member bitmap of the plane.
+__global__ void k_plane_member(const int* __restrict__ list, const int* __restrict__ nlist, const int* __restrict__ L,
+                               const unsigned* __restrict__ act, const TileInfo* __restrict__ T, int solid,
+                               const int* __restrict__ minidx, unsigned* __restrict__ member,
+                               int* __restrict__ seed_used) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *nlist) return;
+    const int seed = T[solid].seed;
+    const int s = (seed >= 0 && bit_get(act, seed)) ? seed : *minidx;
+    const int root = uf_find(L, s);
+    const int c = list[t];
+    if (uf_find(L, c) == root) atomicOr(&member[c >> 5], 1u << (c & 31));
+    if (t == 0) *seed_used = s;
+}
+
+// Field pass: one workgroup per brick of BX*BY*BZ cells.
+template <class R, int MK, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX* BY* BZ)
+    k_fields(MeshDev m, int nbx, int nby, const long long* __restrict__ boff, const int* __restrict__ blist,
+             long long pair_cap, const SolidK* __restrict__ Ks, const TileInfo* __restrict__ Ts,
+             const R* __restrict__ codes, long long code_cap, const int* __restrict__ planes, int nplanes,
+             const unsigned* __restrict__ plane_member, long long member_words,
+             const double* __restrict__ brick_box, const R* __restrict__ Ux, const R* __restrict__ Uy,
+             const R* __restrict__ Uz, R* __restrict__ As, R* __restrict__ Fx, R* __restrict__ Fy,
+             R* __restrict__ Fz, R* __restrict__ Tsf, int* __restrict__ Ct, double dt,
+             double* __restrict__ plane_partial, long long nbricks, int* __restrict__ err) {
+    constexpr int NT = BX * BY * BZ;
+    __shared__ int raw[kMaxPerBrick];
+    __shared__ int lst[kMaxPerBrick];
+    __shared__ double red[6 * 16];
+    __shared__ int pl_any[kMaxPlanes];
+
+    const long long b = blockIdx.x;
+    const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long long)nbx * nby));
+    const int tid = threadIdx.x;
+    const int ti = tid % BX, tj = (tid / BX) % BY, tk = tid / (BX * BY);
+    const int i = bx * BX + ti, j = by * BY + tj, k = bz * BZ + tk;
+    const bool valid = i < m.nx && j < m.ny && k < m.nz;
+
+    long long p0 = boff[b], p1 = boff[b + 1];
+    if (p1 > pair_cap) p1 = pair_cap;
+    int cnt = (int)(p1 - p0);
+    if (cnt > kMaxPerBrick) {
+        if (tid == 0) atomicOr(err, EF_BRICK);
+        cnt = kMaxPerBrick;
+    }
+    for (int t = tid; t < cnt; t += NT) raw[t] = blist[p0 + t];
+    if (tid < kMaxPlanes) pl_any[tid] = 0;
+    __syncthreads();
+    for (int t = tid; t < cnt; t += NT) {  // rank sort (solid ids are distinct)
+        const int v = raw[t];
+        int rank = 0;
+        for (int u = 0; u < cnt; ++u) rank += raw[u] < v;
+        lst[rank] = v;
+    }
+    __syncthreads();
+
+    // brick vertex box, for plane culling
+    double bb[6];
+    brick_vertex_box<MK, BX, BY, BZ>(m, brick_box, b, bx, by, bz, bb);
+
+    const double dtINV = 1.0 / dt;
+    double sA = 0.0, sT = 0.0, sF[3] = {0.0, 0.0, 0.0};
+    int ct = 0;
+    bool have_u = false;
+    d3 uf{0, 0, 0}, cc{0, 0, 0};
+    double V = 0.0;
+    double pacc[kMaxPlanes][6];
+#pragma unroll
+    for (int p = 0; p < kMaxPlanes; ++p)
+#pragma unroll
+        for (int t = 0; t < 6; ++t) pacc[p][t] = 0.0;
+
+    auto contribute = [&](const SolidK& K, double alpha, int ctype, int plane_slot) {
+        if (!have_u) {
+            const long long c = cell_id(m, i, j, k);
+            uf = {(double)Ux[c], (double)Uy[c], (double)Uz[c]};
+            cc = cell_centre<MK>(m, i, j, k);
+            V = cell_volume<MK>(m, i, j, k);
+            have_u = true;
+        }
+        const d3 r = {cc.x - K.c[0], cc.y - K.c[1], cc.z - K.c[2]};
+        const d3 us = d3{K.v[0], K.v[1], K.v[2]} + cross(d3{K.w[0], K.w[1], K.w[2]}, r);
+        const d3 f = alpha * (uf - us);
+        sA += alpha;
+        sT += alpha;
+        sF[0] += f.x * dtINV;
+        sF[1] += f.y * dtINV;
+        sF[2] += f.z * dtINV;
+        ct = ctype;
+        if (plane_slot >= 0) {
+            const d3 tq = cross(r, f);
+            double* a = pacc[plane_slot];
+            a[0] += (f.x * V) * dtINV;
+            a[1] += (f.y * V) * dtINV;
+            a[2] += (f.z * V) * dtINV;
+            a[3] += (tq.x * V) * dtINV;
+            a[4] += (tq.y * V) * dtINV;
+            a[5] += (tq.z * V) * dtINV;
+        }
+    };
+
+    int ia = 0, ip = 0;
+    while (ia < cnt || ip < nplanes) {
+        const int sa = ia < cnt ? lst[ia] : 0x7fffffff;
+        const int sp = ip < nplanes ? planes[ip] : 0x7fffffff;
+        if (sa < sp) {
+            ++ia;
+            if (!valid) continue;
+            const TileInfo T = Ts[sa];
+            if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) continue;  // skipped by k_solid
+            const int li = i - T.lo[0], lj = j - T.lo[1], lk = k - T.lo[2];
+            if (li < 0 || li >= T.n[0] || lj < 0 || lj >= T.n[1] || lk < 0 || lk >= T.n[2]) continue;
+            const R code = codes[T.off + (long long)li + (long long)T.n[0] * (lj + (long long)T.n[1] * lk)];
+            if (code != code) continue;  // NaN: not a candidate
+            const SolidK K = Ks[sa];
+            if (isinf(code))
+                contribute(K, 1.0, sa + 4, -1);
+            else
+                contribute(K, (double)fabs(code), signbit(code) ? 3 : 2, -1);
+        } else {
+            const int slot = ip;
+            ++ip;
+            const SolidK K = Ks[sp];
+            // plane: only cells of the seed's component (member bitmap); cull the brick first
+            const int st = plane_box_state(K, bb);
+            if (st == 0) continue;  // no vertex of the brick can be inside
+            if (!valid) continue;
+            const long long c = cell_id(m, i, j, k);
+            if (!bit_get(plane_member + (long long)slot * member_words, c)) continue;
+            int nin = 8;
+            if (st == 2) {
+                d3 P[8];
+                cell_vertices<MK>(m, i, j, k, P);
+                nin = 0;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) nin += inside_world(K, P[v]) ? 1 : 0;
+            }
+            if (nin == 0) continue;
+            pl_any[slot] = 1;
+            if (nin == 8) {
+                contribute(K, 1.0, sp + 4, slot);
+            } else {
+                const R alpha = fraction_of_cell<R, MK>(m, K, i, j, k);
+                const bool cin = inside_world(K, cell_centre<MK>(m, i, j, k));
+                contribute(K, (double)alpha, cin ? 2 : 3, slot);
+            }
+        }
+    }
+
+    if (valid) {
+        const long long c = cell_id(m, i, j, k);
+        As[c] = R(sA < 1.0 ? sA : 1.0);
+        Fx[c] = R(sF[0]);
+        Fy[c] = R(sF[1]);
+        Fz[c] = R(sF[2]);
+        Tsf[c] = R(sT);
+        Ct[c] = ct;
+    }
+    if (nplanes > 0) {
+        __syncthreads();
+        for (int p = 0; p < nplanes; ++p) {
+            if (!pl_any[p]) {
+                if (tid == 0)
+                    for (int t = 0; t < 6; ++t) plane_partial[((long long)p * nbricks + b) * 6 + t] = 0.0;
+                continue;
+            }
+            double v6[6];
+            for (int t = 0; t < 6; ++t) v6[t] = pacc[p][t];
+            block_reduce<6>(v6, red);
+            if (tid == 0)
+                for (int t = 0; t < 6; ++t) plane_partial[((long long)p * nbricks + b) * 6 + t] = v6[t];
+        }
+    }
+}
+
+__global__ void k_plane_reduce(const double* __restrict__ partial, long long nbricks, const int* __restrict__ planes,
+                               double rho_f, double* __restrict__ ft) {
+    __shared__ double red[6 * 16];
+    const int p = blockIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const long long chunk = (nbricks + blockDim.x - 1) / blockDim.x;
+    const long long b0 = threadIdx.x * chunk, b1 = b0 + chunk < nbricks ? b0 + chunk : nbricks;
+    for (long long b = b0; b < b1; ++b)
+        for (int t = 0; t < 6; ++t) acc[t] += partial[((long long)p * nbricks + b) * 6 + t];
+    block_reduce<6>(acc, red);
+    if (threadIdx.x == 0)
+        for (int t = 0; t < 6; ++t) ft[6 * planes[p] + t] = acc[t] * rho_f;
+}
+
+// Candidate codes of a plane over the whole mesh (parity / diagnostics only).
+template <class R, int MK>
+__global__ void k_plane_codes(MeshDev m, SolidK K, const unsigned* __restrict__ member, R* __restrict__ out) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long N = (long long)m.nx * m.ny * m.nz;
+    if (c >= N) return;
+    if (!bit_get(member, c)) {
+        out[c] = code_none<R>();
+        return;
+    }
+    const int i = (int)(c % m.nx), j = (int)((c / m.nx) % m.ny), k = (int)(c / ((long long)m.nx * m.ny));
+    d3 P[8];
+    cell_vertices<MK>(m, i, j, k, P);
+    int nin = 0;
+    for (int v = 0; v < 8; ++v) nin += inside_world(K, P[v]) ? 1 : 0;
+    if (nin == 0) {
+        out[c] = code_none<R>();
+    } else if (nin == 8) {
+        out[c] = R(__builtin_huge_val());
+    } else {
+        const R a = fraction_of_cell<R, MK>(m, K, i, j, k);
+        out[c] = inside_world(K, cell_centre<MK>(m, i, j, k)) ? a : -a;
+    }
+}
+
+template <class R>
+__global__ void k_aos_to_soa(const double* __restrict__ in, long long n, R* __restrict__ x, R* __restrict__ y,
+                             R* __restrict__ z) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    x[c] = R(in[3 * c]);
+    y[c] = R(in[3 * c + 1]);
+    z[c] = R(in[3 * c + 2]);
+}
+
+template <class R>
+__global__ void k_fields_to_f64(const R* __restrict__ As, const R* __restrict__ Fx, const R* __restrict__ Fy,
+                                const R* __restrict__ Fz, const R* __restrict__ Tsf, const int* __restrict__ Ct,
+                                long long n, double* __restrict__ oA, double* __restrict__ oF,
+                                double* __restrict__ oT, double* __restrict__ oC) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    oA[c] = (double)As[c];
+    oF[3 * c] = (double)Fx[c];
+    oF[3 * c + 1] = (double)Fy[c];
+    oF[3 * c + 2] = (double)Fz[c];
+    oT[c] = (double)Tsf[c];
+    oC[c] = (double)Ct[c];
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+constexpr int kBX3 = 32, kBY3 = 4, kBZ3 = 2;  // brick of a 3-D mesh
+constexpr int kBX2 = 32, kBY2 = 8, kBZ2 = 1;  // brick of a one-cell-thick (2-D) mesh
+
+struct sdfibm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int kind = 0, prec = 0, two_d = 0;
+    int nx = 0, ny = 0, nz = 0;
+    long long N = 0;
+    double rho_f = 1.0;
+    MeshDev m{};   // device pointers
+    MeshDev hm{};  // host pointers (tile estimates)
+    std::vector<double> h_xs, h_ys, h_zs;
+    // device buffers
+    std::vector<void*> allocs;
+    void* U[3] = {nullptr, nullptr, nullptr};
+    void* As = nullptr;
+    void* Fs[3] = {nullptr, nullptr, nullptr};
+    void* Tsf = nullptr;
+    int* Ct = nullptr;
+    double* brick_box = nullptr;
+    int bx = 0, by = 0, bz = 0;
+    int nbx = 0, nby = 0, nbz = 0;
+    long long nbricks = 0;
+    int* bcount = nullptr;
+    long long* boff = nullptr;
+    int* err = nullptr;
+    // solids
+    long long ns = 0;
+    bool have_solids = false, have_step = false;
+    std::vector<sdfibm_solid> h_solids;
+    sdfibm_solid* d_solids = nullptr;
+    SolidK* d_K = nullptr;
+    TileInfo* d_T = nullptr;
+    long long* d_tile_cells = nullptr;
+    long long* d_tile_off = nullptr;
+    long long solid_cap = 0;
+    void* d_codes = nullptr;
+    long long code_cap = 0;
+    int* d_blist = nullptr;
+    long long pair_cap = 0;
+    double* d_ft = nullptr;
+    std::vector<int> h_planes;
+    int* d_planes = nullptr;
+    double* d_plane_partial = nullptr;
+    // plane components: active bitmap, active list, union-find labels, member bitmaps
+    unsigned* d_pact = nullptr;
+    int* d_plist = nullptr;
+    int* d_plabel = nullptr;
+    int* d_pscalars = nullptr;  // [0] list size, [1] lowest active index, [2] changed flag
+    unsigned* d_pmember = nullptr;
+    int* d_pseed = nullptr;
+    long long member_words = 0;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double timing[4] = {0, 0, 0, 0};
+    std::string msg;
+};
+
+namespace {
+
+thread_local std::string g_noctx_msg;
+
+int fail(sdfibm_ctx* c, int code, const std::string& m) {
+    if (c)
+        c->msg = m;
+    else
+        g_noctx_msg = m;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                                     \
+    do {                                                                                                      \
+        hipError_t e_ = (call);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return fail((ctx), SDFIBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+int dalloc(sdfibm_ctx* c, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(c, SDFIBM_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+    }
+    c->allocs.push_back(*p);
+    return SDFIBM_OK;
+}
+
+void dfree(sdfibm_ctx* c, void* p) {
+    if (!p) return;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+size_t rsize(const sdfibm_ctx* c) { return c->prec == SDFIBM_FP64 ? sizeof(double) : sizeof(float); }
+
+unsigned grid_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+bool finite3(const double* v, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+int validate_solid(const sdfibm_solid& s, long long idx, std::string& why) {
+    if (s.shape < 0 || s.shape > SDFIBM_SHAPE_CIRCLE_TWOTAIL) {
+        why = "solid " + std::to_string(idx) + ": unknown shape id " + std::to_string(s.shape);
+        return SDFIBM_ERR_SHAPE;
+    }
+    if (!finite3(s.prm, 6) || !finite3(s.center, 3) || !finite3(s.q, 4) || !finite3(s.vel, 3) ||
+        !finite3(s.omega, 3)) {
+        why = "solid " + std::to_string(idx) + ": non-finite parameter or state";
+        return SDFIBM_ERR_ARG;
+    }
+    const double qn = s.q[0] * s.q[0] + s.q[1] * s.q[1] + s.q[2] * s.q[2] + s.q[3] * s.q[3];
+    if (!(qn > 0.5 && qn < 2.0)) {
+        why = "solid " + std::to_string(idx) + ": quaternion is not (close to) unit length";
+        return SDFIBM_ERR_ARG;
+    }
+    int npos = 0;
+    switch (s.shape) {
+    case SDFIBM_SHAPE_SPHERE:
+    case SDFIBM_SHAPE_CIRCLE: npos = 1; break;
+    case SDFIBM_SHAPE_ELLIPSOID:
+    case SDFIBM_SHAPE_BOX: npos = 3; break;
+    case SDFIBM_SHAPE_ELLIPSE:
+    case SDFIBM_SHAPE_RECTANGLE: npos = 2; break;
+    case SDFIBM_SHAPE_CIRCLE_TAIL:
+    case SDFIBM_SHAPE_CIRCLE_TWOTAIL: npos = 1; break;
+    default: npos = 0;
+    }
+    for (int t = 0; t < npos; ++t)
+        if (!(s.prm[t] > 0)) {
+            why = "solid " + std::to_string(idx) + ": shape size parameter " + std::to_string(t) + " must be > 0";
+            return SDFIBM_ERR_SHAPE;
+        }
+    if ((s.shape == SDFIBM_SHAPE_CIRCLE_TAIL || s.shape == SDFIBM_SHAPE_CIRCLE_TWOTAIL) &&
+        !(s.prm[1] >= 0 && s.prm[2] > 0)) {
+        why = "solid " + std::to_string(idx) + ": tail ratio must be >= 0 and thickness > 0";
+        return SDFIBM_ERR_SHAPE;
+    }
+    return SDFIBM_OK;
+}
+
+template <class R, int MK>
+int launch_step(sdfibm_ctx* c, double dt) {
+    const int ns = (int)c->ns;
+    hipStream_t st = c->stream;
+    (void)hipEventRecord(c->ev[0], st);
+    k_fill_i32<<<1, 64, 0, st>>>(c->err, 1, 0);
+    if (ns > 0) {
+        k_setup<MK><<<grid_for(ns, 128), 128, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_cells,
+                                                       c->d_ft, c->err);
+        k_scan<long long><<<1, 1024, 0, st>>>(c->d_tile_cells, c->d_tile_off, ns);
+        k_tile_offsets<<<grid_for(ns, 128), 128, 0, st>>>(c->d_T, c->d_tile_off, ns, c->code_cap, c->err);
+    }
+    k_fill_i32<<<grid_for(c->nbricks, 256), 256, 0, st>>>(c->bcount, c->nbricks, 0);
+    if (ns > 0) {
+        for (int pass = 0; pass < 2; ++pass) {
+            if (c->nz == 1)
+                k_bin<kBX2, kBY2, kBZ2><<<grid_for(ns, 128), 128, 0, st>>>(
+                    ns, c->m, c->d_T, c->nbx, c->nby, c->bcount, c->boff, c->d_blist, c->pair_cap, c->err, pass);
+            else
+                k_bin<kBX3, kBY3, kBZ3><<<grid_for(ns, 128), 128, 0, st>>>(
+                    ns, c->m, c->d_T, c->nbx, c->nby, c->bcount, c->boff, c->d_blist, c->pair_cap, c->err, pass);
+            if (pass == 0) {
+                k_scan<int><<<1, 1024, 0, st>>>(c->bcount, c->boff, c->nbricks);
+                k_fill_i32<<<grid_for(c->nbricks, 256), 256, 0, st>>>(c->bcount, c->nbricks, 0);
+            }
+        }
+    } else {
+        k_scan<int><<<1, 1024, 0, st>>>(c->bcount, c->boff, c->nbricks);
+    }
+    (void)hipEventRecord(c->ev[1], st);
+    R* codes = (R*)c->d_codes;
+    if (ns > 0) {
+        k_solid<R, MK, kRowsSmall, TC_SMALL><<<ns, kBlock, 0, st>>>(
+            c->m, ns, c->d_K, c->d_T, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap,
+            dt, c->rho_f, c->d_ft);
+        k_solid<R, MK, kRowsLarge, TC_LARGE><<<ns, kBlock, 0, st>>>(
+            c->m, ns, c->d_K, c->d_T, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap,
+            dt, c->rho_f, c->d_ft);
+    }
+    const int np = (int)c->h_planes.size();
+    for (int p = 0; p < np; ++p) {
+        const int sp = c->h_planes[p];
+        const SolidK K = make_solid_k(c->h_solids[sp]);
+        unsigned* member = c->d_pmember + (long long)p * c->member_words;
+        k_fill_i32<<<grid_for(c->member_words, 256), 256, 0, st>>>((int*)c->d_pact, c->member_words, 0);
+        k_fill_i32<<<grid_for(c->member_words, 256), 256, 0, st>>>((int*)member, c->member_words, 0);
+        k_fill_i32<<<1, 64, 0, st>>>(c->d_pscalars, 1, 0);
+        k_fill_i32<<<1, 64, 0, st>>>(c->d_pscalars + 1, 1, 0x7fffffff);
+        if (c->nz == 1)
+            k_plane_active<MK, kBX2, kBY2, kBZ2><<<(unsigned)c->nbricks, kBX2 * kBY2 * kBZ2, 0, st>>>(
+                c->m, c->nbx, c->nby, c->brick_box, K, c->d_pact, c->d_plist, c->d_pscalars, c->N, c->d_pscalars + 1);
+        else
+            k_plane_active<MK, kBX3, kBY3, kBZ3><<<(unsigned)c->nbricks, kBX3 * kBY3 * kBZ3, 0, st>>>(
+                c->m, c->nbx, c->nby, c->brick_box, K, c->d_pact, c->d_plist, c->d_pscalars, c->N, c->d_pscalars + 1);
+        int nact = 0;
+        if (hipMemcpyAsync(&nact, c->d_pscalars, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return -1;
+        if (nact == 0) continue;
+        const unsigned g = grid_for(nact, 256);
+        k_ccl_init<<<g, 256, 0, st>>>(c->d_plist, c->d_pscalars, c->d_plabel);
+        int changed = 1, it = 0;
+        for (; changed && it < 64; ++it) {
+            k_fill_i32<<<1, 64, 0, st>>>(c->d_pscalars + 2, 1, 0);
+            k_ccl_hook<<<g, 256, 0, st>>>(c->m, c->d_plist, c->d_pscalars, c->d_pact, c->d_plabel, c->d_pscalars + 2);
+            k_ccl_compress<<<g, 256, 0, st>>>(c->d_plist, c->d_pscalars, c->d_plabel);
+            if (hipMemcpyAsync(&changed, c->d_pscalars + 2, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return -1;
+        }
+        if (changed) return -2;
+        k_plane_member<<<g, 256, 0, st>>>(c->d_plist, c->d_pscalars, c->d_plabel, c->d_pact, c->d_T, sp,
+                                          c->d_pscalars + 1, member, c->d_pseed + p);
+    }
+    (void)hipEventRecord(c->ev[2], st);
+    if (c->nz == 1) {
+        k_fields<R, MK, kBX2, kBY2, kBZ2><<<(unsigned)c->nbricks, kBX2 * kBY2 * kBZ2, 0, st>>>(
+            c->m, c->nbx, c->nby, c->boff, c->d_blist, c->pair_cap, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np,
+            c->d_pmember, c->member_words,
+            c->brick_box, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], (R*)c->As, (R*)c->Fs[0],
+            (R*)c->Fs[1], (R*)c->Fs[2], (R*)c->Tsf, c->Ct, dt, c->d_plane_partial, c->nbricks, c->err);
+    } else {
+        k_fields<R, MK, kBX3, kBY3, kBZ3><<<(unsigned)c->nbricks, kBX3 * kBY3 * kBZ3, 0, st>>>(
+            c->m, c->nbx, c->nby, c->boff, c->d_blist, c->pair_cap, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np,
+            c->d_pmember, c->member_words,
+            c->brick_box, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], (R*)c->As, (R*)c->Fs[0],
+            (R*)c->Fs[1], (R*)c->Fs[2], (R*)c->Tsf, c->Ct, dt, c->d_plane_partial, c->nbricks, c->err);
+    }
+    if (np > 0) k_plane_reduce<<<np, 256, 0, st>>>(c->d_plane_partial, c->nbricks, c->d_planes, c->rho_f, c->d_ft);
+    (void)hipEventRecord(c->ev[3], st);
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+namespace {
+
+int setup_mesh(sdfibm_ctx* c, const sdfibm_mesh_desc* d) {
+    const long long nx = d->nx, ny = d->ny, nz = d->nz;
+    const long long N = nx * ny * nz;
+    const long long NP = (nx + 1) * (ny + 1) * (nz + 1);
+    MeshDev m{};
+    m.nx = (int)nx;
+    m.ny = (int)ny;
+    m.nz = (int)nz;
+    m.two_d = c->two_d;
+    int rc;
+    if (d->kind == SDFIBM_MESH_RECTILINEAR) {
+        if (!d->xs || !d->ys || !d->zs) return fail(c, SDFIBM_ERR_ARG, "rectilinear mesh needs xs, ys, zs");
+        c->h_xs.assign(d->xs, d->xs + nx + 1);
+        c->h_ys.assign(d->ys, d->ys + ny + 1);
+        c->h_zs.assign(d->zs, d->zs + nz + 1);
+        for (auto* v : {&c->h_xs, &c->h_ys, &c->h_zs}) {
+            for (size_t t = 0; t < v->size(); ++t) {
+                if (!std::isfinite((*v)[t])) return fail(c, SDFIBM_ERR_MESH, "non-finite mesh coordinate");
+                if (t > 0 && !((*v)[t] > (*v)[t - 1]))
+                    return fail(c, SDFIBM_ERR_MESH, "mesh coordinates must be strictly increasing");
+            }
+        }
+        double* p[3];
+        const std::vector<double>* src[3] = {&c->h_xs, &c->h_ys, &c->h_zs};
+        for (int a = 0; a < 3; ++a) {
+            if ((rc = dalloc(c, (void**)&p[a], src[a]->size() * 8))) return rc;
+            HIPCHK(c, hipMemcpy(p[a], src[a]->data(), src[a]->size() * 8, hipMemcpyHostToDevice));
+        }
+        m.xs = p[0];
+        m.ys = p[1];
+        m.zs = p[2];
+        c->hm = m;
+        c->hm.xs = c->h_xs.data();
+        c->hm.ys = c->h_ys.data();
+        c->hm.zs = c->h_zs.data();
+    } else if (d->kind == SDFIBM_MESH_STRUCTURED) {
+        if (!d->points || !d->cell_centres || !d->cell_volumes)
+            return fail(c, SDFIBM_ERR_ARG, "structured mesh needs points, cell_centres, cell_volumes");
+        for (int f = 0; f < 3; ++f)
+            if (!d->face_centres[f] || !d->face_areas[f])
+                return fail(c, SDFIBM_ERR_ARG, "structured mesh needs the three face families");
+        // nominal uniform grid: the box spanned by the corner points
+        const double* P = d->points;
+        auto pidx = [&](long long i, long long j, long long k) { return i + (nx + 1) * (j + (ny + 1) * k); };
+        const long long pe = pidx(nx, ny, nz);
+        double o[3], h[3];
+        const long long nn[3] = {nx, ny, nz};
+        for (int a = 0; a < 3; ++a) {
+            o[a] = P[a];
+            h[a] = (P[3 * pe + a] - P[a]) / (double)nn[a];
+            if (!(h[a] > 0)) return fail(c, SDFIBM_ERR_MESH, "structured mesh: corner points do not span a box");
+        }
+        double jit = 0.0;
+        std::vector<double> soa(NP);
+        double* dp[3];
+        for (int a = 0; a < 3; ++a) {
+            for (long long k = 0; k <= nz; ++k)
+                for (long long j = 0; j <= ny; ++j)
+                    for (long long i = 0; i <= nx; ++i) {
+                        const long long q = pidx(i, j, k);
+                        const double v = P[3 * q + a];
+                        if (!std::isfinite(v)) return fail(c, SDFIBM_ERR_MESH, "non-finite point");
+                        const long long idx = a == 0 ? i : (a == 1 ? j : k);
+                        jit = std::max(jit, std::fabs((v - o[a]) / h[a] - (double)idx));
+                        soa[q] = v;
+                    }
+            if ((rc = dalloc(c, (void**)&dp[a], NP * 8))) return rc;
+            HIPCHK(c, hipMemcpy(dp[a], soa.data(), NP * 8, hipMemcpyHostToDevice));
+        }
+        if (jit > 0.45)
+            return fail(c, SDFIBM_ERR_MESH,
+                        "structured mesh: points deviate more than 0.45 cells from a uniform grid (" +
+                            std::to_string(jit) + ")");
+        m.px = dp[0];
+        m.py = dp[1];
+        m.pz = dp[2];
+        std::vector<double> cs(N);
+        double* dc[3];
+        for (int a = 0; a < 3; ++a) {
+            for (long long q = 0; q < N; ++q) cs[q] = d->cell_centres[3 * q + a];
+            if ((rc = dalloc(c, (void**)&dc[a], N * 8))) return rc;
+            HIPCHK(c, hipMemcpy(dc[a], cs.data(), N * 8, hipMemcpyHostToDevice));
+        }
+        m.ccx = dc[0];
+        m.ccy = dc[1];
+        m.ccz = dc[2];
+        double* dv;
+        if ((rc = dalloc(c, (void**)&dv, N * 8))) return rc;
+        HIPCHK(c, hipMemcpy(dv, d->cell_volumes, N * 8, hipMemcpyHostToDevice));
+        m.vol = dv;
+        const long long nf[3] = {(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)};
+        for (int f = 0; f < 3; ++f) {
+            double *a, *b;
+            if ((rc = dalloc(c, (void**)&a, nf[f] * 24))) return rc;
+            if ((rc = dalloc(c, (void**)&b, nf[f] * 24))) return rc;
+            HIPCHK(c, hipMemcpy(a, d->face_centres[f], nf[f] * 24, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(b, d->face_areas[f], nf[f] * 24, hipMemcpyHostToDevice));
+            m.fc[f] = a;
+            m.sf[f] = b;
+        }
+        for (int a = 0; a < 3; ++a) {
+            m.o[a] = o[a];
+            m.h[a] = h[a];
+        }
+        m.jit = jit;
+        c->hm = m;  // host copy: only the nominal grid is used on the host
+        // brick vertex boxes (plane culling)
+        const int BX = nz == 1 ? kBX2 : kBX3, BY = nz == 1 ? kBY2 : kBY3, BZ = nz == 1 ? kBZ2 : kBZ3;
+        const long long nbx = (nx + BX - 1) / BX, nby = (ny + BY - 1) / BY, nbz = (nz + BZ - 1) / BZ;
+        std::vector<double> box(nbx * nby * nbz * 6);
+        for (long long bz = 0; bz < nbz; ++bz)
+            for (long long by = 0; by < nby; ++by)
+                for (long long bx = 0; bx < nbx; ++bx) {
+                    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+                    for (long long k = bz * BZ; k <= std::min(bz * BZ + BZ, nz); ++k)
+                        for (long long j = by * BY; j <= std::min(by * BY + BY, ny); ++j)
+                            for (long long i = bx * BX; i <= std::min(bx * BX + BX, nx); ++i) {
+                                const long long q = pidx(i, j, k);
+                                for (int a = 0; a < 3; ++a) {
+                                    lo[a] = std::min(lo[a], P[3 * q + a]);
+                                    hi[a] = std::max(hi[a], P[3 * q + a]);
+                                }
+                            }
+                    double* bb = &box[6 * (bx + nbx * (by + nby * bz))];
+                    bb[0] = lo[0];
+                    bb[1] = hi[0];
+                    bb[2] = lo[1];
+                    bb[3] = hi[1];
+                    bb[4] = lo[2];
+                    bb[5] = hi[2];
+                }
+        if ((rc = dalloc(c, (void**)&c->brick_box, box.size() * 8))) return rc;
+        HIPCHK(c, hipMemcpy(c->brick_box, box.data(), box.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        return fail(c, SDFIBM_ERR_MESH, "unknown mesh kind " + std::to_string(d->kind));
+    }
+    c->m = m;
+    return SDFIBM_OK;
+}
+
+template <class R, int MK>
+int interact_impl(sdfibm_ctx* c, double dt) {
+    const int r = launch_step<R, MK>(c, dt);
+    HIPCHK(c, hipGetLastError());
+    if (r == -1) return fail(c, SDFIBM_ERR_HIP, "plane component pass: HIP copy/synchronize failed");
+    if (r == -2) return fail(c, SDFIBM_ERR_CAPACITY, "plane component labelling did not converge");
+    return SDFIBM_OK;
+}
+
+int dispatch_interact(sdfibm_ctx* c, double dt) {
+    if (c->prec == SDFIBM_FP64)
+        return c->kind == SDFIBM_MESH_RECTILINEAR ? interact_impl<double, MK_RECT>(c, dt)
+                                                  : interact_impl<double, MK_STRUCT>(c, dt);
+    return c->kind == SDFIBM_MESH_RECTILINEAR ? interact_impl<float, MK_RECT>(c, dt)
+                                              : interact_impl<float, MK_STRUCT>(c, dt);
+}
+
+std::string err_text(int flags) {
+    std::string s;
+    if (flags & EF_TILE) s += "a solid's cell tile exceeds 63 cells along x or 4096 vertex rows; ";
+    if (flags & EF_CODECAP) s += "tile code buffer too small; ";
+    if (flags & EF_PAIRCAP) s += "brick pair buffer too small; ";
+    if (flags & EF_BRICK) s += "more than 256 solids overlap one brick; ";
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sdfibm_abi_version(void) { return SDFIBM_ABI_VERSION; }
+
+const char* sdfibm_status_string(int status) {
+    switch (status) {
+    case SDFIBM_OK: return "ok";
+    case SDFIBM_ERR_ARG: return "invalid argument";
+    case SDFIBM_ERR_HIP: return "HIP runtime error";
+    case SDFIBM_ERR_NOMEM: return "device allocation failed";
+    case SDFIBM_ERR_SHAPE: return "invalid shape";
+    case SDFIBM_ERR_TILE: return "solid tile too large";
+    case SDFIBM_ERR_STATE: return "call out of order";
+    case SDFIBM_ERR_MESH: return "unsupported mesh";
+    case SDFIBM_ERR_CAPACITY: return "buffer capacity exceeded";
+    case SDFIBM_ERR_NODEVICE: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+const char* sdfibm_last_error(const sdfibm_ctx* c) { return c ? c->msg.c_str() : g_noctx_msg.c_str(); }
+
+int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ctx** out) {
+    if (!out) return fail(nullptr, SDFIBM_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!d || !cfg) return fail(nullptr, SDFIBM_ERR_ARG, "mesh or config is NULL");
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(nullptr, SDFIBM_ERR_ARG, "mesh dimensions must be >= 1");
+    if ((long long)d->nx * d->ny * d->nz > (1LL << 31) - 1)
+        return fail(nullptr, SDFIBM_ERR_MESH, "more than 2^31-1 cells");
+    if (cfg->precision != SDFIBM_FP32 && cfg->precision != SDFIBM_FP64)
+        return fail(nullptr, SDFIBM_ERR_ARG, "precision must be SDFIBM_FP32 or SDFIBM_FP64");
+    if (!std::isfinite(cfg->rho_f)) return fail(nullptr, SDFIBM_ERR_ARG, "rho_f must be finite");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return fail(nullptr, SDFIBM_ERR_NODEVICE, "no HIP device available");
+    }
+    sdfibm_ctx* c = new sdfibm_ctx;
+    (void)hipGetDevice(&c->device);
+    c->kind = d->kind;
+    c->prec = cfg->precision;
+    c->two_d = cfg->two_d ? 1 : 0;
+    c->rho_f = cfg->rho_f;
+    c->nx = d->nx;
+    c->ny = d->ny;
+    c->nz = d->nz;
+    c->N = (long long)d->nx * d->ny * d->nz;
+    int rc = setup_mesh(c, d);
+    if (rc) {
+        g_noctx_msg = c->msg;
+        sdfibm_destroy(c);
+        return rc;
+    }
+    auto bail = [&](int code) {
+        g_noctx_msg = c->msg;
+        sdfibm_destroy(c);
+        return code;
+    };
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        c->msg = "hipStreamCreate failed";
+        return bail(SDFIBM_ERR_HIP);
+    }
+    c->own_stream = true;
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            c->msg = "hipEventCreate failed";
+            return bail(SDFIBM_ERR_HIP);
+        }
+    const size_t rs = rsize(c);
+    for (int a = 0; a < 3; ++a) {
+        if ((rc = dalloc(c, &c->U[a], c->N * rs))) return bail(rc);
+        if ((rc = dalloc(c, &c->Fs[a], c->N * rs))) return bail(rc);
+        if (hipMemset(c->U[a], 0, c->N * rs) != hipSuccess) return bail(fail(c, SDFIBM_ERR_HIP, "memset U"));
+    }
+    if ((rc = dalloc(c, &c->As, c->N * rs))) return bail(rc);
+    if ((rc = dalloc(c, &c->Tsf, c->N * rs))) return bail(rc);
+    if ((rc = dalloc(c, (void**)&c->Ct, c->N * 4))) return bail(rc);
+    const bool flat = c->nz == 1;
+    c->bx = flat ? kBX2 : kBX3;
+    c->by = flat ? kBY2 : kBY3;
+    c->bz = flat ? kBZ2 : kBZ3;
+    c->nbx = (c->nx + c->bx - 1) / c->bx;
+    c->nby = (c->ny + c->by - 1) / c->by;
+    c->nbz = (c->nz + c->bz - 1) / c->bz;
+    c->nbricks = (long long)c->nbx * c->nby * c->nbz;
+    if ((rc = dalloc(c, (void**)&c->bcount, c->nbricks * 4))) return bail(rc);
+    if ((rc = dalloc(c, (void**)&c->boff, (c->nbricks + 1) * 8))) return bail(rc);
+    if ((rc = dalloc(c, (void**)&c->err, 64))) return bail(rc);
+    *out = c;
+    return SDFIBM_OK;
+}
+
+int sdfibm_destroy(sdfibm_ctx* c) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SDFIBM_OK;
+}
+
+int sdfibm_set_stream(sdfibm_ctx* c, void* s) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (c->own_stream && c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+    }
+    c->own_stream = false;
+    c->stream = (hipStream_t)s;
+    if (!s) {
+        if (hipStreamCreate(&c->stream) != hipSuccess) return fail(c, SDFIBM_ERR_HIP, "hipStreamCreate failed");
+        c->own_stream = true;
+    }
+    return SDFIBM_OK;
+}
+
+int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (n < 0 || (n > 0 && !solids)) return fail(c, SDFIBM_ERR_ARG, "bad solid array");
+    if (n > (1LL << 30)) return fail(c, SDFIBM_ERR_ARG, "too many solids");
+    std::vector<int> planes;
+    long long cells = 0, pairs = 0;
+    for (int64_t s = 0; s < n; ++s) {
+        std::string why;
+        const int v = validate_solid(solids[s], s, why);
+        if (v) return fail(c, v, why);
+        const SolidK K = make_solid_k(solids[s]);
+        if (solids[s].shape == SDFIBM_SHAPE_PLANE) {
+            planes.push_back((int)s);
+            continue;
+        }
+        TileInfo t = c->kind == SDFIBM_MESH_RECTILINEAR ? make_tile<MK_RECT>(c->hm, K) : make_tile<MK_STRUCT>(c->hm, K);
+        if (t.cls == TC_TOOBIG)
+            return fail(c, SDFIBM_ERR_TILE, "solid " + std::to_string(s) + ": " + err_text(EF_TILE));
+        if (t.cls == TC_EMPTY) continue;
+        // allow one extra cell layer per axis for motion between set_solids calls
+        const long long ex = std::min(t.n[0] + 1, c->nx), ey = std::min(t.n[1] + 1, c->ny),
+                        ez = std::min(t.n[2] + 1, c->nz);
+        cells += ex * ey * ez;
+        pairs += (long long)(ex / c->bx + 2) * (ey / c->by + 2) * (ez / c->bz + 2);
+    }
+    if ((int)planes.size() > kMaxPlanes)
+        return fail(c, SDFIBM_ERR_ARG, "at most " + std::to_string(kMaxPlanes) + " plane solids are supported");
+    int rc;
+    if (n > c->solid_cap) {
+        for (void* p : {(void*)c->d_solids, (void*)c->d_K, (void*)c->d_T, (void*)c->d_tile_cells,
+                        (void*)c->d_tile_off, (void*)c->d_ft})
+            dfree(c, p);
+        const long long cap = std::max<long long>(n, 16);
+        if ((rc = dalloc(c, (void**)&c->d_solids, cap * sizeof(sdfibm_solid)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_K, cap * sizeof(SolidK)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_T, cap * sizeof(TileInfo)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_tile_cells, cap * 8))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_tile_off, (cap + 1) * 8))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_ft, cap * 6 * 8))) return rc;
+        c->solid_cap = cap;
+    }
+    const long long want_codes = cells + cells / 20 + 1024;
+    if (want_codes > c->code_cap) {
+        dfree(c, c->d_codes);
+        c->d_codes = nullptr;
+        c->code_cap = 0;
+        if ((rc = dalloc(c, &c->d_codes, want_codes * rsize(c)))) return rc;
+        c->code_cap = want_codes;
+    }
+    const long long want_pairs = pairs + pairs / 20 + 1024;
+    if (want_pairs > c->pair_cap) {
+        dfree(c, c->d_blist);
+        c->d_blist = nullptr;
+        c->pair_cap = 0;
+        if ((rc = dalloc(c, (void**)&c->d_blist, want_pairs * 4))) return rc;
+        c->pair_cap = want_pairs;
+    }
+    dfree(c, c->d_planes);
+    dfree(c, c->d_plane_partial);
+    c->d_planes = nullptr;
+    c->d_plane_partial = nullptr;
+    if (!planes.empty()) {
+        if ((rc = dalloc(c, (void**)&c->d_planes, planes.size() * 4))) return rc;
+        HIPCHK(c, hipMemcpy(c->d_planes, planes.data(), planes.size() * 4, hipMemcpyHostToDevice));
+        if ((rc = dalloc(c, (void**)&c->d_plane_partial, planes.size() * c->nbricks * 6 * 8))) return rc;
+    }
+    if (!planes.empty() && !c->d_pact) {
+        c->member_words = (c->N + 31) / 32;
+        if ((rc = dalloc(c, (void**)&c->d_pact, c->member_words * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_plist, c->N * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_plabel, c->N * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_pscalars, 64))) return rc;
+    }
+    dfree(c, c->d_pmember);
+    dfree(c, c->d_pseed);
+    c->d_pmember = nullptr;
+    c->d_pseed = nullptr;
+    if (!planes.empty()) {
+        if ((rc = dalloc(c, (void**)&c->d_pmember, planes.size() * c->member_words * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_pseed, planes.size() * 4))) return rc;
+    }
+    c->h_planes = planes;
+    c->h_solids.assign(solids, solids + n);
+    if (n > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_solids, solids, n * sizeof(sdfibm_solid), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ns = n;
+    c->have_solids = true;
+    c->have_step = false;
+    return SDFIBM_OK;
+}
+
+int sdfibm_set_velocity(sdfibm_ctx* c, const double* U) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!U) return fail(c, SDFIBM_ERR_ARG, "U is NULL");
+    void* tmp = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &tmp, c->N * 24))) return rc;
+    hipError_t e = hipMemcpyAsync(tmp, U, c->N * 24, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        if (c->prec == SDFIBM_FP64)
+            k_aos_to_soa<double><<<grid_for(c->N, 256), 256, 0, c->stream>>>(
+                (const double*)tmp, c->N, (double*)c->U[0], (double*)c->U[1], (double*)c->U[2]);
+        else
+            k_aos_to_soa<float><<<grid_for(c->N, 256), 256, 0, c->stream>>>(
+                (const double*)tmp, c->N, (float*)c->U[0], (float*)c->U[1], (float*)c->U[2]);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(c, tmp);
+    if (e != hipSuccess) return fail(c, SDFIBM_ERR_HIP, std::string("set_velocity: ") + hipGetErrorString(e));
+    return SDFIBM_OK;
+}
+
+int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
+    (void)time;
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "interact before set_solids");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
+    int rc = dispatch_interact(c, dt);
+    if (rc) return rc;
+    int flags = 0;
+    HIPCHK(c, hipMemcpyAsync(&flags, c->err, 4, hipMemcpyDeviceToHost, c->stream));
+    if (ft && c->ns > 0) HIPCHK(c, hipMemcpyAsync(ft, c->d_ft, c->ns * 48, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms[3] = {0, 0, 0};
+    (void)hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]);
+    c->timing[0] = ms[0];
+    c->timing[1] = ms[1];
+    c->timing[2] = ms[2];
+    c->timing[3] = (double)ms[0] + ms[1] + ms[2];
+    if (flags) {
+        c->have_step = false;
+        return fail(c, (flags & EF_TILE) ? SDFIBM_ERR_TILE : SDFIBM_ERR_CAPACITY, err_text(flags));
+    }
+    c->have_step = true;
+    return SDFIBM_OK;
+}
+
+int sdfibm_last_timing(sdfibm_ctx* c, double* ms, int n) {
+    if (!c || !ms || n < 0) return SDFIBM_ERR_ARG;
+    const int k = n < 4 ? n : 4;
+    for (int t = 0; t < k; ++t) ms[t] = c->timing[t];
+    return k;
+}
+
+int sdfibm_get_fields(sdfibm_ctx* c, double* As, double* Fs, double* Ts, double* Ct) {
+    if (!c) return SDFIBM_ERR_ARG;
+    const long long N = c->N;
+    const size_t rs = rsize(c);
+    std::vector<unsigned char> buf(N * rs);
+    auto fetch = [&](void* src, double* dst, int stride, int comp) -> int {
+        HIPCHK(c, hipMemcpy(buf.data(), src, N * rs, hipMemcpyDeviceToHost));
+        if (rs == 8) {
+            const double* b = (const double*)buf.data();
+            for (long long q = 0; q < N; ++q) dst[stride * q + comp] = b[q];
+        } else {
+            const float* b = (const float*)buf.data();
+            for (long long q = 0; q < N; ++q) dst[stride * q + comp] = (double)b[q];
+        }
+        return SDFIBM_OK;
+    };
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc;
+    if (As && (rc = fetch(c->As, As, 1, 0))) return rc;
+    if (Ts && (rc = fetch(c->Tsf, Ts, 1, 0))) return rc;
+    if (Fs)
+        for (int a = 0; a < 3; ++a)
+            if ((rc = fetch(c->Fs[a], Fs, 3, a))) return rc;
+    if (Ct) {
+        std::vector<int> ci(N);
+        HIPCHK(c, hipMemcpy(ci.data(), c->Ct, N * 4, hipMemcpyDeviceToHost));
+        for (long long q = 0; q < N; ++q) Ct[q] = (double)ci[q];
+    }
+    return SDFIBM_OK;
+}
+
+int sdfibm_field_pointers(sdfibm_ctx* c, void** U3, void** As, void** Fs3, void** Ts, int32_t** Ct) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (U3)
+        for (int a = 0; a < 3; ++a) U3[a] = c->U[a];
+    if (Fs3)
+        for (int a = 0; a < 3; ++a) Fs3[a] = c->Fs[a];
+    if (As) *As = c->As;
+    if (Ts) *Ts = c->Tsf;
+    if (Ct) *Ct = c->Ct;
+    return SDFIBM_OK;
+}
+
+int sdfibm_candidates(sdfibm_ctx* c, int64_t solid, int64_t* counts, int64_t* cells, double* alpha,
+                      int64_t capacity) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!c->have_step) return fail(c, SDFIBM_ERR_STATE, "candidates before a successful interact");
+    if (solid < 0 || solid >= c->ns || !counts) return fail(c, SDFIBM_ERR_ARG, "bad solid index or counts");
+    const size_t rs = rsize(c);
+    TileInfo T;
+    HIPCHK(c, hipMemcpy(&T, c->d_T + solid, sizeof(TileInfo), hipMemcpyDeviceToHost));
+    std::vector<unsigned char> raw;
+    long long nt = 0;
+    int lo[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+    if (T.cls == TC_PLANE) {
+        nt = c->N;
+        raw.resize(nt * rs);
+        void* tmp = nullptr;
+        int rc;
+        if ((rc = dalloc(c, &tmp, nt * rs))) return rc;
+        const SolidK K = make_solid_k(c->h_solids[solid]);
+        const int p = (int)(std::find(c->h_planes.begin(), c->h_planes.end(), (int)solid) - c->h_planes.begin());
+        const unsigned* mem = c->d_pmember + (long long)p * c->member_words;
+        if (c->prec == SDFIBM_FP64) {
+            if (c->kind == SDFIBM_MESH_RECTILINEAR)
+                k_plane_codes<double, MK_RECT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, K, mem, (double*)tmp);
+            else
+                k_plane_codes<double, MK_STRUCT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, K, mem, (double*)tmp);
+        } else {
+            if (c->kind == SDFIBM_MESH_RECTILINEAR)
+                k_plane_codes<float, MK_RECT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, K, mem, (float*)tmp);
+            else
+                k_plane_codes<float, MK_STRUCT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, K, mem, (float*)tmp);
+        }
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(raw.data(), tmp, nt * rs, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dfree(c, tmp);
+        if (e != hipSuccess) return fail(c, SDFIBM_ERR_HIP, std::string("plane codes: ") + hipGetErrorString(e));
+        n[0] = c->nx;
+        n[1] = c->ny;
+        n[2] = c->nz;
+    } else if (T.cls == TC_SMALL || T.cls == TC_LARGE) {
+        nt = (long long)T.n[0] * T.n[1] * T.n[2];
+        raw.resize(nt * rs);
+        HIPCHK(c, hipMemcpy(raw.data(), (unsigned char*)c->d_codes + T.off * rs, nt * rs, hipMemcpyDeviceToHost));
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = T.lo[a];
+            n[a] = T.n[a];
+        }
+    }
+    std::vector<long long> lists[3];
+    std::vector<double> al[3];
+    for (long long q = 0; q < nt; ++q) {
+        const double v = rs == 8 ? ((const double*)raw.data())[q] : (double)((const float*)raw.data())[q];
+        if (std::isnan(v)) continue;
+        const long long li = q % n[0], lj = (q / n[0]) % n[1], lk = q / ((long long)n[0] * n[1]);
+        const long long cell = (lo[0] + li) + (long long)c->nx * ((lo[1] + lj) + (long long)c->ny * (lo[2] + lk));
+        int t;
+        double a;
+        if (std::isinf(v)) {
+            t = 0;
+            a = 1.0;
+        } else if (std::signbit(v)) {
+            t = 2;
+            a = -v;
+        } else {
+            t = 1;
+            a = v;
+        }
+        lists[t].push_back(cell);
+        al[t].push_back(a);
+    }
+    counts[0] = (int64_t)lists[0].size();
+    counts[1] = (int64_t)lists[1].size();
+    counts[2] = (int64_t)lists[2].size();
+    counts[3] = -1;
+    if (counts[0] + counts[1] + counts[2] > 0) {
+        // the seed actually used: the nearest cell if it is a candidate, else the first candidate
+        long long near = T.seed;
+        long long first = -1;
+        bool near_in = false;
+        for (int t = 0; t < 3; ++t)
+            for (long long v : lists[t]) {
+                if (first < 0 || v < first) first = v;
+                if (v == near) near_in = true;
+            }
+        counts[3] = near_in ? near : first;
+    }
+    long long k = 0;
+    for (int t = 0; t < 3; ++t)
+        for (size_t q = 0; q < lists[t].size(); ++q) {
+            if (k < capacity) {
+                if (cells) cells[k] = lists[t][q];
+                if (alpha) alpha[k] = al[t][q];
+            }
+            ++k;
+        }
+    if (k > capacity && (cells || alpha)) return fail(c, SDFIBM_ERR_CAPACITY, "candidate buffer too small");
+    return SDFIBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,402 @@
+// mesh_dev.h -- device view of a logically structured hex mesh and the
+// per-solid cell tiles (shared by host set-up code and kernels).
+#pragma once
+
+#include "fraction.h"
+
+namespace sdfibm_dev {
+
+enum MeshKind { MK_RECT = 0, MK_STRUCT = 1 };
+
+struct MeshDev {
+    int nx, ny, nz, two_d;
+    // rectilinear coordinates
+    const double* xs;
+    const double* ys;
+    const double* zs;
+    // structured: SoA points, SoA cell centres, volumes, AoS face centres/areas per family
+    const double* px;
+    const double* py;
+    const double* pz;
+    const double* ccx;
+    const double* ccy;
+    const double* ccz;
+    const double* vol;
+    const double* fc[3];
+    const double* sf[3];
+    // structured: nominal uniform grid the points scatter around (origin, spacing) and
+    // the largest point displacement from it, in cells
+    double o[3], h[3], jit;
+};
+
+__host__ __device__ inline long long cell_id(const MeshDev& m, int i, int j, int k) {
+    return (long long)i + (long long)m.nx * ((long long)j + (long long)m.ny * k);
+}
+
+template <int MK>
+__device__ inline d3 vertex_pos(const MeshDev& m, int i, int j, int k) {
+    if constexpr (MK == MK_RECT) {
+        return {m.xs[i], m.ys[j], m.zs[k]};
+    } else {
+        const long long p = (long long)i + (long long)(m.nx + 1) * ((long long)j + (long long)(m.ny + 1) * k);
+        return {m.px[p], m.py[p], m.pz[p]};
+    }
+}
+
+template <int MK>
+__device__ inline d3 cell_centre(const MeshDev& m, int i, int j, int k) {
+    if constexpr (MK == MK_RECT) {
+        return {(m.xs[i] + m.xs[i + 1]) * 0.5, (m.ys[j] + m.ys[j + 1]) * 0.5, (m.zs[k] + m.zs[k + 1]) * 0.5};
+    } else {
+        const long long c = cell_id(m, i, j, k);
+        return {m.ccx[c], m.ccy[c], m.ccz[c]};
+    }
+}
+
+template <int MK>
+__device__ inline double cell_volume(const MeshDev& m, int i, int j, int k) {
+    if constexpr (MK == MK_RECT) {
+        return ((m.xs[i + 1] - m.xs[i]) * (m.ys[j + 1] - m.ys[j])) * (m.zs[k + 1] - m.zs[k]);
+    } else {
+        return m.vol[cell_id(m, i, j, k)];
+    }
+}
+
+// The 8 vertex positions of cell (i,j,k) in cellPoints order.
+template <int MK>
+__device__ inline void cell_vertices(const MeshDev& m, int i, int j, int k, d3* P) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) P[v] = vertex_pos<MK>(m, i + (v & 1), j + ((v >> 1) & 1), k + ((v >> 2) & 1));
+}
+
+// Face data of cell (i,j,k) in the cell's OpenFOAM face order: owned internal
+// faces (+x, +y, +z), boundary faces in patch order (xmin, xmax, ymin, ymax,
+// zmin, zmax), then neighbour internal faces (-z, -y, -x).
+template <class R, int MK>
+__device__ inline void cell_faces(const MeshDev& m, int i, int j, int k, d3 v0, CellFaces<R>& F) {
+    int n = 0;
+    auto put = [&](int fam, int a, int b, int c, const unsigned char* lst) {
+        d3 fc, sf;
+        if constexpr (MK == MK_RECT) {
+            if (fam == 0) {
+                fc = {m.xs[a], (m.ys[b] + m.ys[b + 1]) * 0.5, (m.zs[c] + m.zs[c + 1]) * 0.5};
+                sf = {(m.ys[b + 1] - m.ys[b]) * (m.zs[c + 1] - m.zs[c]), 0.0, 0.0};
+            } else if (fam == 1) {
+                fc = {(m.xs[a] + m.xs[a + 1]) * 0.5, m.ys[b], (m.zs[c] + m.zs[c + 1]) * 0.5};
+                sf = {0.0, (m.xs[a + 1] - m.xs[a]) * (m.zs[c + 1] - m.zs[c]), 0.0};
+            } else {
+                fc = {(m.xs[a] + m.xs[a + 1]) * 0.5, (m.ys[b] + m.ys[b + 1]) * 0.5, m.zs[c]};
+                sf = {0.0, 0.0, (m.xs[a + 1] - m.xs[a]) * (m.ys[b + 1] - m.ys[b])};
+            }
+        } else {
+            long long f;
+            if (fam == 0)
+                f = (long long)a + (long long)(m.nx + 1) * ((long long)b + (long long)m.ny * c);
+            else if (fam == 1)
+                f = (long long)a + (long long)m.nx * ((long long)b + (long long)(m.ny + 1) * c);
+            else
+                f = (long long)a + (long long)m.nx * ((long long)b + (long long)m.ny * c);
+            const double* pc = m.fc[fam] + 3 * f;
+            const double* ps = m.sf[fam] + 3 * f;
+            fc = {pc[0], pc[1], pc[2]};
+            sf = {ps[0], ps[1], ps[2]};
+        }
+        F.fc[n] = {R(fc.x - v0.x), R(fc.y - v0.y), R(fc.z - v0.z)};
+        F.sf[n] = {R(sf.x), R(sf.y), R(sf.z)};
+        F.lst[n] = lst;
+        ++n;
+    };
+    if (i < m.nx - 1) put(0, i + 1, j, k, kFacePX);
+    if (j < m.ny - 1) put(1, i, j + 1, k, kFacePY);
+    if (k < m.nz - 1) put(2, i, j, k + 1, kFacePZ);
+    if (i == 0) put(0, i, j, k, kFaceMXb);
+    if (i == m.nx - 1) put(0, i + 1, j, k, kFacePX);
+    if (j == 0) put(1, i, j, k, kFaceMYb);
+    if (j == m.ny - 1) put(1, i, j + 1, k, kFacePY);
+    if (k == 0) put(2, i, j, k, kFaceMZb);
+    if (k == m.nz - 1) put(2, i, j, k + 1, kFacePZ);
+    if (k > 0) put(2, i, j, k, kFaceMZi);
+    if (j > 0) put(1, i, j, k, kFaceMYi);
+    if (i > 0) put(0, i, j, k, kFaceMXi);
+    F.n = n;
+}
+
+// Volume fraction of cell (i,j,k) inside solid K (calcCellVolume / V), with the
+// 8 corner signed distances evaluated in R.
+template <class R, int MK>
+__device__ inline R fraction_of_cell(const MeshDev& m, const SolidK& K, int i, int j, int k) {
+    d3 Pw[8];
+    cell_vertices<MK>(m, i, j, k, Pw);
+    r3<R> P[8];
+    R phi[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        P[v] = {R(Pw[v].x - Pw[0].x), R(Pw[v].y - Pw[0].y), R(Pw[v].z - Pw[0].z)};
+        phi[v] = sdf_world<R>(K, Pw[v]);
+    }
+    CellFaces<R> F;
+    cell_faces<R, MK>(m, i, j, k, Pw[0], F);
+    const R V = R(cell_volume<MK>(m, i, j, k));
+    return cell_fraction<R>(P, phi, F, V, m.two_d != 0, R(0.0 - Pw[0].z));
+}
+
+// ---------------------------------------------------------------------------
+// per-solid tiles
+// ---------------------------------------------------------------------------
+enum TileClass { TC_EMPTY = 0, TC_SMALL = 1, TC_LARGE = 2, TC_PLANE = 3, TC_TOOBIG = 4 };
+
+struct TileInfo {
+    int lo[3];
+    int n[3];
+    long long off;  // offset of the tile's first code
+    int seed;       // nearest cell to the solid centre (global index), -1 if none
+    int cls;        // TileClass
+};
+
+constexpr int kMaxTileX = 63;      // cells along x per tile (vertex row fits one 64-bit word)
+constexpr int kRowsSmall = 1024;   // vertex rows (ny+1)(nz+1) of a small tile
+constexpr int kRowsLarge = 4096;   // vertex rows of a large tile
+
+// first index in sorted a[0..n) with a[idx] >= v (n if none)
+__host__ __device__ inline int lower_bound_d(const double* a, int n, double v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+// last index with a[idx] <= v (-1 if none)
+__host__ __device__ inline int last_le_d(const double* a, int n, double v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo - 1;
+}
+
+// Rotation matrix of q (local -> world).
+__host__ __device__ inline void rot_of(const double* q, double Rm[3][3]) {
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    Rm[0][0] = 1 - 2 * (y * y + z * z);
+    Rm[0][1] = 2 * (x * y - w * z);
+    Rm[0][2] = 2 * (x * z + w * y);
+    Rm[1][0] = 2 * (x * y + w * z);
+    Rm[1][1] = 1 - 2 * (x * x + z * z);
+    Rm[1][2] = 2 * (y * z - w * x);
+    Rm[2][0] = 2 * (x * z - w * y);
+    Rm[2][1] = 2 * (y * z + w * x);
+    Rm[2][2] = 1 - 2 * (x * x + y * y);
+}
+
+// World-space axis-aligned bounds (centre wc, half extents wh, padded) of the
+// region where a solid's inside test can hold; false for unbounded shapes.
+// Bodies are bounded through their local box; the tailed discs through the
+// union of the disc's box and the exact corners of the tail rectangle(s).
+__host__ __device__ inline bool world_bounds(const SolidK& K, double* wc, double* wh) {
+    const double INF = __builtin_huge_val();
+    double Rm[3][3];
+    rot_of(K.q, Rm);
+    double lo[3] = {INF, INF, INF}, hi[3] = {-INF, -INF, -INF};
+    auto add_point = [&](double px, double py, double pz) {  // local point
+        for (int d = 0; d < 3; ++d) {
+            const double w = K.c[d] + (Rm[d][0] * px + Rm[d][1] * py + Rm[d][2] * pz);
+            lo[d] = fmin(lo[d], w);
+            hi[d] = fmax(hi[d], w);
+        }
+    };
+    // local z extrusion of the 2-D shapes: unbounded along any world axis local z touches
+    auto extrude = [&]() {
+        for (int d = 0; d < 3; ++d)
+            if (fabs(Rm[d][2]) > 1e-12) {
+                lo[d] = -INF;
+                hi[d] = INF;
+            }
+    };
+    if (K.shape == SDFIBM_SHAPE_CIRCLE_TAIL || K.shape == SDFIBM_SHAPE_CIRCLE_TWOTAIL) {
+        const double r = K.k[0], ra = K.k[2], rb = K.k[3];
+        const double cx = -K.com[0], cy = -K.com[1];
+        for (int d = 0; d < 3; ++d) {  // the disc about -com in the local xy plane
+            const double w = K.c[d] + (Rm[d][0] * cx + Rm[d][1] * cy);
+            const double e = r * sqrt(Rm[d][0] * Rm[d][0] + Rm[d][1] * Rm[d][1]);
+            lo[d] = fmin(lo[d], w - e);
+            hi[d] = fmax(hi[d], w + e);
+        }
+        const double ux[4] = {0.0, 2 * ra, 2 * ra, 0.0}, uy[4] = {-rb, -rb, rb, rb};
+        if (K.shape == SDFIBM_SHAPE_CIRCLE_TAIL) {
+            for (int t = 0; t < 4; ++t) add_point(ux[t] + cx, uy[t] + cy, 0.0);
+        } else {
+            // tails: rot30(q) or rot30(flipy(q)) in [0, 2ra] x [-rb, rb]; invert rot30
+            const double c30 = 0.866025404, det = c30 * c30 + 0.25;
+            for (int t = 0; t < 4; ++t) {
+                const double qx = (c30 * ux[t] - 0.5 * uy[t]) / det, qy = (0.5 * ux[t] + c30 * uy[t]) / det;
+                add_point(qx + cx, qy + cy, 0.0);
+                add_point(qx + cx, -qy + cy, 0.0);
+            }
+        }
+        extrude();
+    } else {
+        d3 lc, lh;
+        if (!local_bounds(K, lc, lh)) return false;
+        const double lcv[3] = {lc.x, lc.y, lc.z}, lhv[3] = {lh.x, lh.y, lh.z};
+        for (int d = 0; d < 3; ++d) {
+            const double w = K.c[d] + (Rm[d][0] * lcv[0] + Rm[d][1] * lcv[1] + Rm[d][2] * lcv[2]);
+            double e = 0.0;
+            for (int f = 0; f < 3; ++f) {
+                const double rr = fabs(Rm[d][f]);
+                if (lhv[f] > 1e300) {
+                    if (rr > 1e-12) e = INF;
+                } else {
+                    e += rr * lhv[f];
+                }
+            }
+            lo[d] = w - e;
+            hi[d] = w + e;
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (!(hi[d] - lo[d] < 1e300)) {
+            wc[d] = 0.0;
+            wh[d] = INF;
+            continue;
+        }
+        wc[d] = 0.5 * (lo[d] + hi[d]);
+        wh[d] = 0.5 * (hi[d] - lo[d]) * (1.0 + 1e-6) + 1e-12 * (fabs(wc[d]) + 1.0);
+    }
+    return true;
+}
+
+// Cell tile of a solid: the index box containing every cell that has a vertex
+// strictly inside the solid, and the nearest cell to the solid centre.
+// `cx`,`cy`,`cz` are the 1-D coordinates (rectilinear) used for the seed.
+template <int MK>
+__host__ __device__ inline TileInfo make_tile(const MeshDev& m, const SolidK& K) {
+    TileInfo T{};
+    T.seed = -1;
+    T.off = 0;
+    double wc[3], wh[3];
+    if (!world_bounds(K, wc, wh)) {
+        T.cls = TC_PLANE;
+        return T;
+    }
+    const int nn[3] = {m.nx, m.ny, m.nz};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        const double a = wc[d] - wh[d], b = wc[d] + wh[d];
+        int vlo, vhi;  // vertex index range that may hold inside vertices
+        if (!(wh[d] < 1e300)) {
+            vlo = 0;
+            vhi = nn[d];
+        } else if constexpr (MK == MK_RECT) {
+            const double* xs = d == 0 ? m.xs : (d == 1 ? m.ys : m.zs);
+            vlo = lower_bound_d(xs, nn[d] + 1, a);
+            vhi = last_le_d(xs, nn[d] + 1, b);
+        } else {
+            const double fa = (a - m.o[d]) / m.h[d] - m.jit, fb = (b - m.o[d]) / m.h[d] + m.jit;
+            vlo = fa < -1.0 ? 0 : (int)ceil(fa);
+            vhi = fb > (double)nn[d] + 1.0 ? nn[d] : (int)floor(fb);
+            if (vlo < 0) vlo = 0;
+            if (vhi > nn[d]) vhi = nn[d];
+        }
+        if (vlo > vhi) {
+            T.cls = TC_EMPTY;
+            return T;
+        }
+        lo[d] = vlo - 1 < 0 ? 0 : vlo - 1;
+        hi[d] = vhi > nn[d] - 1 ? nn[d] - 1 : vhi;
+        if (lo[d] > hi[d]) {
+            T.cls = TC_EMPTY;
+            return T;
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        T.lo[d] = lo[d];
+        T.n[d] = hi[d] - lo[d] + 1;
+    }
+    const long long vrows = (long long)(T.n[1] + 1) * (T.n[2] + 1);
+    if (T.n[0] > kMaxTileX || vrows > kRowsLarge)
+        T.cls = TC_TOOBIG;
+    else
+        T.cls = vrows > kRowsSmall ? TC_LARGE : TC_SMALL;
+    return T;
+}
+
+// Nearest cell centre to p (meshSearch::findNearestCell).  Ties -- squared
+// distances within a relative 1e-10 of the minimum -- go to the lowest cell
+// index, so that rounding of the centres does not decide between equidistant
+// cells.
+constexpr double kNearTie = 1e-10;
+
+template <int MK>
+__host__ __device__ inline int nearest_cell(const MeshDev& m, const double* p) {
+    int c0[3], r;
+    const int nn[3] = {m.nx, m.ny, m.nz};
+    if constexpr (MK == MK_RECT) {
+        // per axis the nearest centre is one of two neighbours; search that 2x2x2 block
+        for (int d = 0; d < 3; ++d) {
+            const double* xs = d == 0 ? m.xs : (d == 1 ? m.ys : m.zs);
+            int lo = 0, hi = nn[d] - 1;
+            while (lo < hi) {  // first centre >= p
+                const int mid = (lo + hi) >> 1;
+                if ((xs[mid] + xs[mid + 1]) * 0.5 < p[d])
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            c0[d] = lo > 0 ? lo - 1 : 0;
+        }
+        r = 1;
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            double f = floor((p[d] - m.o[d]) / m.h[d]);
+            if (f < 0) f = 0;
+            if (f > nn[d] - 1) f = nn[d] - 1;
+            c0[d] = (int)f;
+        }
+        r = (int)ceil(2.0 * m.jit) + 1;
+    }
+    const int lo0 = MK == MK_RECT ? 0 : -r;
+    double best = __builtin_huge_val();
+    for (int pass = 0; pass < 2; ++pass) {
+        long long bc = -1;
+        const double lim = best * (1.0 + kNearTie) + 1e-300;
+        for (int k = c0[2] + lo0; k <= c0[2] + r; ++k) {
+            if (k < 0 || k >= m.nz) continue;
+            for (int j = c0[1] + lo0; j <= c0[1] + r; ++j) {
+                if (j < 0 || j >= m.ny) continue;
+                for (int i = c0[0] + lo0; i <= c0[0] + r; ++i) {
+                    if (i < 0 || i >= m.nx) continue;
+                    const long long c = cell_id(m, i, j, k);
+                    double cx, cy, cz;
+                    if constexpr (MK == MK_RECT) {
+                        cx = (m.xs[i] + m.xs[i + 1]) * 0.5;
+                        cy = (m.ys[j] + m.ys[j + 1]) * 0.5;
+                        cz = (m.zs[k] + m.zs[k + 1]) * 0.5;
+                    } else {
+                        cx = m.ccx[c];
+                        cy = m.ccy[c];
+                        cz = m.ccz[c];
+                    }
+                    const double dx = cx - p[0], dy = cy - p[1], dz = cz - p[2];
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    if (pass == 0) {
+                        if (d2 < best) best = d2;
+                    } else if (d2 <= lim && (bc < 0 || c < bc)) {
+                        bc = c;
+                    }
+                }
+            }
+        }
+        if (pass == 1) return (int)bc;
+    }
+    return -1;
+}
+
+}  // namespace sdfibm_dev

@@ -1,0 +1,349 @@
+// shapes.h -- device shape predicates and signed distances.
+//
+// Two predicates per shape, as in the reference (src/libshape/ishape.h:51-52):
+//   inside(p)  strict comparisons without square roots (isInside), FP64 always,
+//              so candidate-cell classification is bit-identical to an FP64 CPU
+//              evaluation in the same operation order;
+//   sdf<R>(p)  signed distance through sdf::filter (signedDistance), in R =
+//              float or double, used only for volume fractions.
+// Both act on local coordinates conj(q) (p - t) q (ishape.h:43-46).
+//
+// This file is compiled with FMA contraction off (see the build line), so
+// a*b + c stays two rounded operations, as on the CPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/sdfibm_hip.h"
+
+namespace sdfibm_dev {
+
+constexpr double kTOL = 1e-8;    // sdf.h:9
+constexpr double kSMALL = 1e-6;  // types.h:19
+
+struct d3 {
+    double x, y, z;
+};
+__host__ __device__ inline d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline d3 operator*(double s, d3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__host__ __device__ inline double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline d3 cross(d3 a, d3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// Per-solid constants derived once per step from the solid record, with the
+// same expressions the reference's shape constructors use (e.g. 1.0/(a*a)).
+struct SolidK {
+    int shape;
+    int pad;
+    double c[3];    // centre
+    double q[4];    // (w, x, y, z)
+    double com[3];  // shape com offset (zero where the shape has none)
+    double k[6];    // shape constants, see make_solid_k
+    double v[3];    // velocity
+    double w[3];    // angular velocity
+};
+
+__host__ __device__ inline SolidK make_solid_k(const sdfibm_solid& s) {
+    SolidK K{};
+    K.shape = s.shape;
+    for (int d = 0; d < 3; ++d) {
+        K.c[d] = s.center[d];
+        K.v[d] = s.vel[d];
+        K.w[d] = s.omega[d];
+    }
+    for (int d = 0; d < 4; ++d) K.q[d] = s.q[d];
+    const double* p = s.prm;
+    switch (s.shape) {
+    case SDFIBM_SHAPE_SPHERE:
+    case SDFIBM_SHAPE_CIRCLE:
+        K.k[0] = p[0];
+        K.k[1] = p[0] * p[0];
+        K.com[0] = p[1];
+        K.com[1] = p[2];
+        K.com[2] = p[3];
+        break;
+    case SDFIBM_SHAPE_ELLIPSOID:
+        K.k[0] = 1.0 / (p[0] * p[0]);
+        K.k[1] = 1.0 / (p[1] * p[1]);
+        K.k[2] = 1.0 / (p[2] * p[2]);
+        break;
+    case SDFIBM_SHAPE_ELLIPSE:
+        K.k[0] = 1.0 / (p[0] * p[0]);
+        K.k[1] = 1.0 / (p[1] * p[1]);
+        K.com[0] = p[3];
+        K.com[1] = p[4];
+        K.com[2] = p[5];
+        break;
+    case SDFIBM_SHAPE_BOX:
+        K.k[0] = p[0];
+        K.k[1] = p[1];
+        K.k[2] = p[2];
+        K.com[0] = p[3];
+        K.com[1] = p[4];
+        K.com[2] = p[5];
+        break;
+    case SDFIBM_SHAPE_RECTANGLE:
+        K.k[0] = p[0];
+        K.k[1] = p[1];
+        K.com[0] = p[3];
+        K.com[1] = p[4];
+        K.com[2] = p[5];
+        break;
+    case SDFIBM_SHAPE_CIRCLE_TAIL:
+    case SDFIBM_SHAPE_CIRCLE_TWOTAIL:
+        K.k[0] = p[0];
+        K.k[1] = p[0] * p[0];
+        K.k[2] = (p[1] + 1) * 0.5 * p[0];  // tail half-length radiusa
+        K.k[3] = (s.shape == SDFIBM_SHAPE_CIRCLE_TAIL) ? p[2] : p[2] * 0.5;  // half-thickness
+        K.com[0] = p[3];
+        K.com[1] = p[4];
+        K.com[2] = p[5];
+        break;
+    default: break;
+    }
+    return K;
+}
+
+// world2local = conjugate(q).transform(p - t), with OpenFOAM's quaternion
+// product order: m = a.mulq0v(u), result = (m * conjugate(a)).v, a = conj(q).
+__host__ __device__ inline d3 world_to_local(const SolidK& K, d3 p) {
+    const d3 u = p - d3{K.c[0], K.c[1], K.c[2]};
+    const double aw = K.q[0];
+    const d3 av{-K.q[1], -K.q[2], -K.q[3]};
+    const double mw = -dot(av, u);
+    const d3 mv = (aw * u) + cross(av, u);
+    const d3 cav{K.q[1], K.q[2], K.q[3]};
+    return (mw * cav + aw * mv) + cross(mv, cav);
+}
+
+__host__ __device__ inline bool rect_bool(double x, double y, double ra, double rb) {
+    return fabs(x) < ra && fabs(y) < rb;
+}
+
+// Strict inside test in local coordinates (each shape's isInside).
+__host__ __device__ inline bool inside_local(const SolidK& K, d3 p) {
+    switch (K.shape) {
+    case SDFIBM_SHAPE_SPHERE: {
+        const d3 q = d3{K.com[0], K.com[1], K.com[2]} + p;
+        return q.x * q.x + q.y * q.y + q.z * q.z < K.k[1];
+    }
+    case SDFIBM_SHAPE_CIRCLE: {
+        const d3 q = d3{K.com[0], K.com[1], K.com[2]} + d3{p.x, p.y, 0.0};
+        return q.x * q.x + q.y * q.y + q.z * q.z < K.k[1];
+    }
+    case SDFIBM_SHAPE_ELLIPSOID:
+        return p.x * p.x * K.k[0] + p.y * p.y * K.k[1] + p.z * p.z * K.k[2] < 1.0;
+    case SDFIBM_SHAPE_ELLIPSE: {
+        const d3 q = d3{K.com[0], K.com[1], K.com[2]} + p;
+        return q.x * q.x * K.k[0] + q.y * q.y * K.k[1] < 1.0;
+    }
+    case SDFIBM_SHAPE_BOX: {
+        const d3 q = d3{K.com[0], K.com[1], K.com[2]} + p;
+        return fabs(q.x) < K.k[0] && fabs(q.y) < K.k[1] && fabs(q.z) < K.k[2];
+    }
+    case SDFIBM_SHAPE_RECTANGLE: {
+        const d3 q = d3{K.com[0], K.com[1], K.com[2]} + p;
+        return rect_bool(q.x, q.y, K.k[0], K.k[1]);
+    }
+    case SDFIBM_SHAPE_PLANE: return p.y < 0;
+    case SDFIBM_SHAPE_CIRCLE_TAIL: {
+        const d3 q = d3{K.com[0], K.com[1], K.com[2]} + p;
+        const bool b1 = rect_bool(q.x - K.k[2], q.y - 0.0, K.k[2], K.k[3]);
+        const bool b2 = q.x * q.x + q.y * q.y + 0.0 * 0.0 < K.k[1];
+        return b1 || b2;
+    }
+    case SDFIBM_SHAPE_CIRCLE_TWOTAIL: {
+        const d3 q = d3{K.com[0], K.com[1], K.com[2]} + p;
+        const bool dc = q.x * q.x + q.y * q.y + 0.0 * 0.0 < K.k[1];
+        // rot30 (sdf.h:155-158), then offset by (radiusa, 0, 0)
+        const double r1x = 0.866025404 * q.x + 0.5 * q.y, r1y = 0.866025404 * q.y - 0.5 * q.x;
+        const double r2x = 0.866025404 * q.x + 0.5 * (-q.y), r2y = 0.866025404 * (-q.y) - 0.5 * q.x;
+        const bool d1 = rect_bool(r1x - K.k[2], r1y - 0.0, K.k[2], K.k[3]);
+        const bool d2 = rect_bool(r2x - K.k[2], r2y - 0.0, K.k[2], K.k[3]);
+        return dc || d1 || d2;
+    }
+    default: return false;
+    }
+}
+
+__host__ __device__ inline bool inside_world(const SolidK& K, d3 p) { return inside_local(K, world_to_local(K, p)); }
+
+// ---- signed distance in R ---------------------------------------------------
+template <class R>
+__host__ __device__ inline R filter_sd(R phi) {
+    return (fabs(phi) < R(kTOL)) ? R(-kTOL) : phi;
+}
+
+template <class R>
+__host__ __device__ inline R rect_sd(R x, R y, R ra, R rb) {
+    const R dx = fabs(x) - ra;
+    const R dy = fabs(y) - rb;
+    const R dxp = fmax(R(0), dx);
+    const R dyp = fmax(R(0), dy);
+    return sqrt(dxp * dxp + dyp * dyp) + fmin(R(0), fmax(dx, dy));
+}
+
+// signedDistance of each shape, local coordinates given in R.
+template <class R>
+__host__ __device__ inline R sdf_local(const SolidK& K, R px, R py, R pz) {
+    const R cx = R(K.com[0]), cy = R(K.com[1]), cz = R(K.com[2]);
+    switch (K.shape) {
+    case SDFIBM_SHAPE_SPHERE: {
+        const R x = cx + px, y = cy + py, z = cz + pz;
+        return filter_sd<R>(sqrt(x * x + y * y + z * z) - R(K.k[0]));
+    }
+    case SDFIBM_SHAPE_CIRCLE: {
+        const R x = cx + px, y = cy + py, z = cz;
+        return filter_sd<R>(sqrt(x * x + y * y + z * z) - R(K.k[0]));
+    }
+    case SDFIBM_SHAPE_ELLIPSOID: {
+        const R ai = R(K.k[0]), bi = R(K.k[1]), ci = R(K.k[2]);
+        const R xa = px * px * ai, yb = py * py * bi, zc = pz * pz * ci;
+        return filter_sd<R>(R(0.5) * (xa + yb + zc - R(1)) / sqrt(xa * ai + yb * bi + zc * ci));
+    }
+    case SDFIBM_SHAPE_ELLIPSE: {
+        const R ai = R(K.k[0]), bi = R(K.k[1]);
+        const R x = cx + px, y = cy + py;
+        const R xa = x * x * ai, yb = y * y * bi;
+        return filter_sd<R>(R(0.5) * (xa + yb - R(1)) / sqrt(xa * ai + yb * bi));
+    }
+    case SDFIBM_SHAPE_BOX: {
+        const R x = cx + px, y = cy + py, z = cz + pz;
+        const R dx = fabs(x) - R(K.k[0]), dy = fabs(y) - R(K.k[1]), dz = fabs(z) - R(K.k[2]);
+        const R dxp = fmax(R(0), dx), dyp = fmax(R(0), dy), dzp = fmax(R(0), dz);
+        return filter_sd<R>(sqrt(dxp * dxp + dyp * dyp + dzp * dzp) + fmin(R(0), fmax(dz, fmax(dx, dy))));
+    }
+    case SDFIBM_SHAPE_RECTANGLE: {
+        const R x = cx + px, y = cy + py;
+        return filter_sd<R>(rect_sd<R>(x, y, R(K.k[0]), R(K.k[1])));
+    }
+    case SDFIBM_SHAPE_PLANE: return py;
+    case SDFIBM_SHAPE_CIRCLE_TAIL: {
+        const R x = cx + px, y = cy + py;
+        const R d1 = sqrt(x * x + y * y) - R(K.k[0]);
+        const R d2 = rect_sd<R>(x - R(K.k[2]), y, R(K.k[2]), R(K.k[3]));
+        return filter_sd<R>(fmin(d1, d2));
+    }
+    case SDFIBM_SHAPE_CIRCLE_TWOTAIL: {
+        const R x = cx + px, y = cy + py;
+        const R dc = sqrt(x * x + y * y) - R(K.k[0]);
+        const R c30 = R(0.866025404), h = R(0.5);
+        const R r1x = c30 * x + h * y, r1y = c30 * y - h * x;
+        const R r2x = c30 * x + h * (-y), r2y = c30 * (-y) - h * x;
+        const R d1 = rect_sd<R>(r1x - R(K.k[2]), r1y, R(K.k[2]), R(K.k[3]));
+        const R d2 = rect_sd<R>(r2x - R(K.k[2]), r2y, R(K.k[2]), R(K.k[3]));
+        return filter_sd<R>(fmin(fmin(dc, d1), d2));
+    }
+    default: return R(1);
+    }
+}
+
+// FP32 signed distance with the cancelling part in FP64: the value near the
+// surface is a small difference of O(1) terms (|p|^2 - r^2, sum x_i^2/a_i^2 - 1,
+// |x| - a), which is formed in FP64 and rounded once; only square roots and
+// divides of well-conditioned quantities run in FP32.  Same function as the
+// reference formula up to rounding.
+__host__ __device__ inline float sdf_local_mixed(const SolidK& K, d3 p) {
+    const double cx = K.com[0], cy = K.com[1], cz = K.com[2];
+    auto sph = [](double d2, double r, double r2) -> float {
+        const float num = (float)(d2 - r2);
+        return num / (sqrtf((float)d2) + (float)r);
+    };
+    auto rect = [](double x, double y, double ra, double rb) -> float {
+        const float dx = (float)(fabs(x) - ra), dy = (float)(fabs(y) - rb);
+        const float dxp = fmaxf(0.f, dx), dyp = fmaxf(0.f, dy);
+        return sqrtf(dxp * dxp + dyp * dyp) + fminf(0.f, fmaxf(dx, dy));
+    };
+    switch (K.shape) {
+    case SDFIBM_SHAPE_SPHERE: {
+        const double x = cx + p.x, y = cy + p.y, z = cz + p.z;
+        return filter_sd<float>(sph(x * x + y * y + z * z, K.k[0], K.k[1]));
+    }
+    case SDFIBM_SHAPE_CIRCLE: {
+        const double x = cx + p.x, y = cy + p.y, z = cz;
+        return filter_sd<float>(sph(x * x + y * y + z * z, K.k[0], K.k[1]));
+    }
+    case SDFIBM_SHAPE_ELLIPSOID: {
+        const double xa = p.x * p.x * K.k[0], yb = p.y * p.y * K.k[1], zc = p.z * p.z * K.k[2];
+        const float qm1 = (float)(xa + yb + zc - 1.0);
+        const float g = (float)(xa * K.k[0] + yb * K.k[1] + zc * K.k[2]);
+        return filter_sd<float>(0.5f * qm1 / sqrtf(g));
+    }
+    case SDFIBM_SHAPE_ELLIPSE: {
+        const double x = cx + p.x, y = cy + p.y;
+        const double xa = x * x * K.k[0], yb = y * y * K.k[1];
+        const float qm1 = (float)(xa + yb - 1.0);
+        const float g = (float)(xa * K.k[0] + yb * K.k[1]);
+        return filter_sd<float>(0.5f * qm1 / sqrtf(g));
+    }
+    case SDFIBM_SHAPE_BOX: {
+        const double x = cx + p.x, y = cy + p.y, z = cz + p.z;
+        const float dx = (float)(fabs(x) - K.k[0]), dy = (float)(fabs(y) - K.k[1]), dz = (float)(fabs(z) - K.k[2]);
+        const float dxp = fmaxf(0.f, dx), dyp = fmaxf(0.f, dy), dzp = fmaxf(0.f, dz);
+        return filter_sd<float>(sqrtf(dxp * dxp + dyp * dyp + dzp * dzp) + fminf(0.f, fmaxf(dz, fmaxf(dx, dy))));
+    }
+    case SDFIBM_SHAPE_RECTANGLE: return filter_sd<float>(rect(cx + p.x, cy + p.y, K.k[0], K.k[1]));
+    case SDFIBM_SHAPE_PLANE: return (float)p.y;
+    case SDFIBM_SHAPE_CIRCLE_TAIL: {
+        const double x = cx + p.x, y = cy + p.y;
+        const float d1 = sph(x * x + y * y, K.k[0], K.k[1]);
+        const float d2 = rect(x - K.k[2], y, K.k[2], K.k[3]);
+        return filter_sd<float>(fminf(d1, d2));
+    }
+    case SDFIBM_SHAPE_CIRCLE_TWOTAIL: {
+        const double x = cx + p.x, y = cy + p.y;
+        const float dc = sph(x * x + y * y, K.k[0], K.k[1]);
+        const double r1x = 0.866025404 * x + 0.5 * y, r1y = 0.866025404 * y - 0.5 * x;
+        const double r2x = 0.866025404 * x + 0.5 * (-y), r2y = 0.866025404 * (-y) - 0.5 * x;
+        const float d1 = rect(r1x - K.k[2], r1y, K.k[2], K.k[3]);
+        const float d2 = rect(r2x - K.k[2], r2y, K.k[2], K.k[3]);
+        return filter_sd<float>(fminf(fminf(dc, d1), d2));
+    }
+    default: return 1.f;
+    }
+}
+
+template <class R>
+__host__ __device__ inline R sdf_world(const SolidK& K, d3 p) {
+    const d3 l = world_to_local(K, p);
+    if constexpr (sizeof(R) == sizeof(float))
+        return sdf_local_mixed(K, l);
+    else
+        return sdf_local<R>(K, R(l.x), R(l.y), R(l.z));
+}
+
+// Local-frame axis-aligned bounds of the inside region: centre lc and half
+// extents lh (lh = +inf along z for the z-extruded 2-D shapes).  Returns false
+// for unbounded shapes (plane).
+__host__ __device__ inline bool local_bounds(const SolidK& K, d3& lc, d3& lh) {
+    const double INF = __builtin_huge_val();
+    const d3 mcom{-K.com[0], -K.com[1], -K.com[2]};
+    switch (K.shape) {
+    case SDFIBM_SHAPE_SPHERE: lc = mcom; lh = {K.k[0], K.k[0], K.k[0]}; return true;
+    case SDFIBM_SHAPE_CIRCLE: lc = mcom; lh = {K.k[0], K.k[0], INF}; return true;
+    case SDFIBM_SHAPE_ELLIPSOID: {
+        lc = {0, 0, 0};
+        lh = {1.0 / sqrt(K.k[0]), 1.0 / sqrt(K.k[1]), 1.0 / sqrt(K.k[2])};
+        return true;
+    }
+    case SDFIBM_SHAPE_ELLIPSE: lc = mcom; lh = {1.0 / sqrt(K.k[0]), 1.0 / sqrt(K.k[1]), INF}; return true;
+    case SDFIBM_SHAPE_BOX: lc = mcom; lh = {K.k[0], K.k[1], K.k[2]}; return true;
+    case SDFIBM_SHAPE_RECTANGLE: lc = mcom; lh = {K.k[0], K.k[1], INF}; return true;
+    case SDFIBM_SHAPE_CIRCLE_TAIL:
+    case SDFIBM_SHAPE_CIRCLE_TWOTAIL: {
+        // circle of radius r and tail rectangle(s) [0, 2 ra] x [-rb, rb] (rotated by
+        // about +-30 degrees for the two-tail shape): bound by the disc of radius
+        // max(r, |(2 ra, rb)|) around -com.
+        const double r = K.k[0], L = 2.0 * K.k[2], rb = K.k[3];
+        const double R = fmax(r, sqrt(L * L + rb * rb));
+        lc = mcom;
+        lh = {R, R, INF};
+        return true;
+    }
+    default: return false;
+    }
+}
+
+}  // namespace sdfibm_dev

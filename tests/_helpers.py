@@ -1,0 +1,77 @@
+"""Shared scene builders and comparisons for the tests (the oracle is the checker)."""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+import oracle
+from sdfibm_amd import mesh as M
+from sdfibm_amd import solids as S
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def toolvof_solids() -> np.ndarray:
+    """The 12 solids and 2 planes of tool_vof/example/solidDict (solids first, then planes,
+    the order the VOF tool processes them)."""
+    shapes = {"circle1": S.make_shape("Circle", radius=0.3),
+              "circle_tail1": S.make_shape("Circle_Tail", radius=0.3, ratio=1, thickness=0.1),
+              "ellipse1": S.make_shape("Ellipse", radiusa=0.3, radiusb=0.2),
+              "rectangle1": S.make_shape("Rectangle", radiusa=0.3, radiusb=0.2),
+              "plane1": S.make_shape("Plane")}
+    lst = [("circle_tail1", (0.5, 1.5, 0), -45), ("circle_tail1", (0.5, 2.5, 0), 0),
+           ("circle_tail1", (0.5, 3.5, 0), 45), ("rectangle1", (1.5, 1.5, 0), 0),
+           ("rectangle1", (1.5, 2.5, 0), 30), ("rectangle1", (1.5, 3.5, 0), 60),
+           ("ellipse1", (2.5, 1.5, 0), 0), ("ellipse1", (2.5, 2.5, 0), 60),
+           ("ellipse1", (2.5, 3.5, 0), 120), ("circle1", (3.5, 1.0, 0), 0),
+           ("circle1", (3.5, 2.5, 0), 0), ("circle1", (3.5, 3.5, 0), 0),
+           ("plane1", (0, 0, 0), 15), ("plane1", (0, 0, 0), -90)]
+    sol = S.make_solids(len(lst))
+    for i, (nm, pos, ang) in enumerate(lst):
+        sid, prm = shapes[nm]
+        sol[i]["shape"] = sid
+        sol[i]["prm"] = prm
+        sol[i]["center"] = pos
+        sol[i]["q"] = S.quaternion_xyz(np.array([0.0, 0.0, ang]) * math.pi / 180.0)
+    return sol
+
+
+def toolvof_mesh() -> tuple:
+    g = np.load(os.path.join(GOLDEN, "toolvof.npz"))
+    m = M.StructuredHexMesh(200, 200, 1, np.ascontiguousarray(g["points"]))
+    return m, g
+
+
+def cylinder_case():
+    g = np.load(os.path.join(GOLDEN, "cylinder_block5.npz"))
+    m = M.uniform_box((120, 120, 1), (-6.0, -6.0, -0.5), (6.0, 6.0, 0.5))
+    sol = S.make_solids(1)
+    sid, prm = S.make_shape("Circle", radius=1.0)
+    sol["shape"] = sid
+    sol["prm"] = prm
+    return m, sol, g
+
+
+def oracle_run(mesh, solids, U, two_d, rho_f=1.0, dt=1.0, mode=0, capture=True, om=None):
+    om = om or oracle.OracleMesh(mesh.to_polymesh())
+    return om, om.interact(solids, U, two_d=two_d, rho_f=rho_f, dt=dt, mode=mode, capture=capture)
+
+
+def split_candidates(r, ns):
+    """Per-solid (counts, cells, alpha) from an oracle capture."""
+    out = []
+    k = 0
+    for s in range(ns):
+        c = r["counts"][s]
+        n = int(c[:3].sum())
+        out.append((c, r["cells"][k:k + n], r["alpha"][k:k + n]))
+        k += n
+    return out
+
+
+def rel_err(a, b, floor):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))) if a.size else 0.0

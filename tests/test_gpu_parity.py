@@ -1,0 +1,177 @@
+"""HIP path vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Tolerances:
+  candidate lists (sets, classes, order, seed)  bit-exact, both precisions
+  FP64 mode   alpha / As / Ts  |d| <= 1e-12;  Fs, force, torque  rel 1e-9
+  FP32 mode   alpha / As / Ts  |d| <= 1e-6;   Fs rel 1e-5;  force, torque rel 1e-5
+(alpha is a fraction of the cell, so |d| <= 1e-6 is 1e-6 relative to a full cell)
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from _helpers import cylinder_case, split_candidates, toolvof_mesh, toolvof_solids
+from sdfibm_amd import _lib, scenes
+from sdfibm_amd import mesh as M
+from sdfibm_amd import solids as S
+from sdfibm_amd.cloud import SolidCloud
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": dict(a=1e-12, f=1e-9, ft=1e-9), "fp32": dict(a=1e-6, f=1e-5, ft=1e-5)}
+
+
+def _compare(cloud, ref, ft, ns, prec, fscale):
+    tol = TOL[prec]
+    got = cloud.fields()
+    parts = split_candidates(ref, ns)
+    for s in range(ns):
+        counts, cells, alpha = cloud.candidates(s)
+        rc, rcells, ralpha = parts[s]
+        assert np.array_equal(counts, rc), (s, counts, rc)
+        assert np.array_equal(cells, rcells), s
+        assert np.max(np.abs(alpha - ralpha), initial=0.0) <= tol["a"], (s, np.max(np.abs(alpha - ralpha)))
+    assert np.array_equal(got["Ct"], ref["Ct"])
+    assert np.max(np.abs(got["As"] - ref["As"])) <= tol["a"]
+    assert np.max(np.abs(got["Ts"] - ref["Ts"])) <= tol["a"] * 4
+    fs_scale = max(np.abs(ref["Fs"]).max(), 1e-30)
+    assert np.max(np.abs(got["Fs"] - ref["Fs"])) <= tol["f"] * fs_scale
+    ft_scale = np.maximum(np.abs(ref["ft"]), fscale)
+    assert np.max(np.abs(ft - ref["ft"]) / ft_scale) <= tol["ft"]
+
+
+def _run(mesh, sol, U, two_d, rho_f, dt, prec, structured=False):
+    with SolidCloud(mesh, two_d=two_d, rho_f=rho_f, precision=prec, force_structured=structured) as c:
+        c.set_solids(sol)
+        c.set_velocity(U)
+        ft = c.interact(0.0, dt)
+        yield c, ft
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("structured", [False, True])
+def test_mixed_shapes_3d(prec, structured):
+    sc = scenes.small_mixed_3d(24, seed=7)
+    om = oracle.OracleMesh(sc.mesh.to_polymesh())
+    ref = om.interact(sc.solids, sc.U, rho_f=sc.rho_f, dt=sc.dt, capture=True)
+    for c, ft in _run(sc.mesh, sc.solids, sc.U, False, sc.rho_f, sc.dt, prec, structured):
+        fscale = 1e-6 * np.abs(ref["ft"]).max()
+        _compare(c, ref, ft, sc.solids.size, prec, fscale)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_jittered_mesh(prec):
+    sc = scenes.small_mixed_3d(20, seed=3, jitter=0.2)
+    om = oracle.OracleMesh(sc.mesh.to_polymesh())
+    ref = om.interact(sc.solids, sc.U, rho_f=sc.rho_f, dt=sc.dt, capture=True)
+    for c, ft in _run(sc.mesh, sc.solids, sc.U, False, sc.rho_f, sc.dt, prec):
+        _compare(c, ref, ft, sc.solids.size, prec, 1e-6 * np.abs(ref["ft"]).max())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_toolvof_fixture_on_gpu(prec):
+    """The reference's alpha.water through the structured-mesh path (mesh points from the file)."""
+    m, g = toolvof_mesh()
+    sol = toolvof_solids()
+    U = np.zeros((m.n_cells, 3))
+    for c, ft in _run(m, sol, U, True, 1.0, 1.0, prec):
+        As = c.fields()["As"]
+        assert np.max(np.abs(As - g["alpha"])) <= (1e-12 if prec == "fp64" else 1e-6)
+
+
+def test_cylinder_fixture_on_gpu():
+    m, sol, g = cylinder_case()
+    for c, ft in _run(m, sol, np.zeros((m.n_cells, 3)), True, 1.0, 1.0, "fp64"):
+        As = c.fields()["As"]
+        bad = np.nonzero(np.abs(As - g["As"]) > 1e-9)[0]
+        assert list(bad) == [57 + 120 * 50]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_sedimentation_with_walls(prec):
+    """2-D discs + four walls on the boundary (walls rotated by 180 / +-90 degrees get rounding-
+    level inside vertices on the boundary row; the classification must match bit for bit)."""
+    sc = scenes.sedimentation_2d(100, 200, 16)
+    rng = np.random.default_rng(5)
+    U = rng.normal(size=(sc.mesh.n_cells, 3))
+    U[:, 2] = 0
+    sol = sc.solids.copy()
+    sol["vel"][:16, :2] = rng.normal(0, 0.1, (16, 2))
+    sol["omega"][:16, 2] = rng.normal(0, 1.0, 16)
+    om = oracle.OracleMesh(sc.mesh.to_polymesh())
+    ref = om.interact(sol, U, two_d=True, rho_f=1.0, dt=sc.dt, capture=True)
+    for c, ft in _run(sc.mesh, sol, U, True, 1.0, sc.dt, prec):
+        _compare(c, ref, ft, sol.size, prec, 1e-6 * np.abs(ref["ft"]).max())
+
+
+def test_seed_fallback_and_empty_solid():
+    n = 16
+    m = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
+    h = 1.0 / n
+    sol = S.make_solids(3)
+    sol["shape"] = S.SPHERE
+    sol["prm"][0, :4] = (1.3 * h, -3.5 * h, 0.0, 0.0)       # body away from the solid position
+    sol["center"][0] = (0.5, 0.5, 0.5)
+    sol["prm"][1, 0] = 0.2 * h                              # smaller than a cell: no inside vertex
+    sol["center"][1] = (0.5 + 0.5 * h, 0.5 + 0.5 * h, 0.5 + 0.5 * h)
+    sol["prm"][2, 0] = 0.3
+    sol["center"][2] = (1.2, 0.5, 0.5)                      # partly outside the mesh
+    U = np.random.default_rng(1).normal(size=(m.n_cells, 3))
+    om = oracle.OracleMesh(m.to_polymesh())
+    ref = om.interact(sol, U, capture=True)
+    assert ref["counts"][1, :3].sum() == 0
+    for c, ft in _run(m, sol, U, False, 1.0, 1.0, "fp64"):
+        _compare(c, ref, ft, 3, "fp64", 1e-6 * np.abs(ref["ft"]).max())
+        assert np.all(ft[1] == 0)
+
+
+def test_error_paths():
+    m = M.uniform_box((8, 8, 8), (0, 0, 0), (1, 1, 1))
+    with SolidCloud(m) as c:
+        with pytest.raises(_lib.SdfibmError) as e:
+            c.interact(0.0, 1.0)
+        assert e.value.status == -6  # ERR_STATE: interact before set_solids
+        bad = S.make_solids(1)
+        bad["shape"] = 42
+        with pytest.raises(_lib.SdfibmError) as e:
+            c.set_solids(bad)
+        assert e.value.status == -4
+        nan = S.make_solids(1)
+        nan["prm"][0, 0] = 0.1
+        nan["center"] = (np.nan, 0, 0)
+        with pytest.raises(_lib.SdfibmError) as e:
+            c.set_solids(nan)
+        assert e.value.status == -1
+        ok = S.make_solids(1)
+        ok["prm"][0, 0] = 0.2
+        ok["center"] = (0.5, 0.5, 0.5)
+        c.set_solids(ok)
+        with pytest.raises(_lib.SdfibmError) as e:
+            c.interact(0.0, 0.0)
+        assert e.value.status == -1
+        c.interact(0.0, 1.0)
+        huge = S.make_solids(1)
+        huge["shape"] = S.BOX
+        huge["prm"][0, :3] = (0.45, 0.45, 0.45)
+        huge["center"] = (0.5, 0.5, 0.5)
+    big = M.uniform_box((80, 8, 8), (0, 0, 0), (1, 0.1, 0.1))
+    with SolidCloud(big) as c:
+        with pytest.raises(_lib.SdfibmError) as e:
+            c.set_solids(huge)
+        assert e.value.status == -5  # tile wider than 63 cells
+
+
+def test_repeat_is_bitwise_deterministic():
+    sc = scenes.small_mixed_3d(24, seed=8)
+    with SolidCloud(sc.mesh, rho_f=sc.rho_f, precision="fp32") as c:
+        c.set_solids(sc.solids)
+        c.set_velocity(sc.U)
+        a = c.interact(0.0, sc.dt)
+        fa = c.fields()
+        b = c.interact(0.0, sc.dt)
+        fb = c.fields()
+    assert np.array_equal(a, b)
+    for k in fa:
+        assert np.array_equal(fa[k], fb[k])
