@@ -39,18 +39,6 @@ __device__ inline R rmag(r3<R> a) {
     return sqrt(rdot(a, a));
 }
 
-// Face vertex lists (indices into the 8 cell vertices).
-// +x, +y, +z (internal or boundary), -x/-y/-z internal, -x/-y/-z boundary.
-__constant__ const unsigned char kFacePX[4] = {1, 3, 7, 5};
-__constant__ const unsigned char kFacePY[4] = {2, 6, 7, 3};
-__constant__ const unsigned char kFacePZ[4] = {4, 5, 7, 6};
-__constant__ const unsigned char kFaceMXi[4] = {0, 2, 6, 4};
-__constant__ const unsigned char kFaceMYi[4] = {0, 4, 5, 1};
-__constant__ const unsigned char kFaceMZi[4] = {0, 1, 3, 2};
-__constant__ const unsigned char kFaceMXb[4] = {0, 4, 6, 2};
-__constant__ const unsigned char kFaceMYb[4] = {0, 1, 5, 4};
-__constant__ const unsigned char kFaceMZb[4] = {0, 2, 3, 1};
-
 // Line fraction (geometrictools.cpp:13-23).
 template <class R>
 __device__ inline R line_fraction(R pa, R pb) {
@@ -60,67 +48,123 @@ __device__ inline R line_fraction(R pa, R pb) {
     return -pa / (pb - pa);
 }
 
-// Apex rule (geometrictools.cpp:25-45): A = first listed vertex, B = first
-// later vertex with phi_A*phi_B <= 0 (else the last one).
-template <class R, int N>
-__device__ inline r3<R> apex_of(const r3<R>* P, const R* phi, const unsigned char* lst) {
-    const r3<R> A = P[lst[0]];
-    const R pA = phi[lst[0]];
-    r3<R> B = P[lst[1]];
-    R pB = phi[lst[1]];
-#pragma unroll
-    for (int t = 1; t < N; ++t) {
-        B = P[lst[t]];
-        pB = phi[lst[t]];
-        if (pA * pB <= R(0)) break;
-    }
+template <class R>
+__device__ inline r3<R> apex_point(r3<R> A, R pA, r3<R> B, R pB) {
     const R s = fabs(pA) / (R(kSMALL) + fabs(pA) + fabs(pB));
     return {A.x - s * (A.x - B.x), A.y - s * (A.y - B.y), A.z - s * (A.z - B.z)};
 }
 
-// Area fraction of a quad face (geometrictools.cpp:74-116); sf = face area vector.
+// The 8 vertices of a hex cell (relative to vertex 0) and their signed
+// distances, in cellPoints order: offsets (0,0,0) (1,0,0) (0,1,0) (1,1,0)
+// (0,0,1) (1,0,1) (0,1,1) (1,1,1).  Named members with compile-time access keep
+// them in registers (an array would be selected through memory).
 template <class R>
-__device__ inline R face_area_fraction(const r3<R>* P, const R* phi, const unsigned char* f, r3<R> sf) {
-    int ss = 0;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) ss += (phi[f[t]] > R(0)) ? 1 : -1;
-    if (ss == 4) return R(0);
-    if (ss == -4) return R(1);
-    const r3<R> ap = apex_of<R, 4>(P, phi, f);
-    R area = R(0);
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int a = f[t], b = f[(t + 1) & 3];
-        const r3<R> O = P[a], A = P[b];
-        const R tri = fabs(R(0.5) * rmag(rcross(sub(A, O), sub(ap, O))));
-        area += tri * line_fraction<R>(phi[a], phi[b]);
-    }
-    return area / rmag(sf);
+struct Hex8 {
+    r3<R> p0, p1, p2, p3, p4, p5, p6, p7;
+    R f0, f1, f2, f3, f4, f5, f6, f7;
+};
+template <int I, class R>
+__device__ inline r3<R> hp(const Hex8<R>& h) {
+    if constexpr (I == 0) return h.p0;
+    else if constexpr (I == 1) return h.p1;
+    else if constexpr (I == 2) return h.p2;
+    else if constexpr (I == 3) return h.p3;
+    else if constexpr (I == 4) return h.p4;
+    else if constexpr (I == 5) return h.p5;
+    else if constexpr (I == 6) return h.p6;
+    else return h.p7;
+}
+template <int I, class R>
+__device__ inline R hf(const Hex8<R>& h) {
+    if constexpr (I == 0) return h.f0;
+    else if constexpr (I == 1) return h.f1;
+    else if constexpr (I == 2) return h.f2;
+    else if constexpr (I == 3) return h.f3;
+    else if constexpr (I == 4) return h.f4;
+    else if constexpr (I == 5) return h.f5;
+    else if constexpr (I == 6) return h.f6;
+    else return h.f7;
 }
 
-// Geometry of the 6 faces of a cell, in the cell's OpenFOAM face order.
+// Area fraction of the quad face (V0 V1 V2 V3) (geometrictools.cpp:74-116).
+// The apex rule (geometrictools.cpp:25-45): A = first listed vertex, B = first
+// later vertex with phi_A*phi_B <= 0, else the last one.
+template <class R, int V0, int V1, int V2, int V3>
+__device__ inline R face_fraction(const Hex8<R>& h, R sf_mag) {
+    const R p0 = hf<V0>(h), p1 = hf<V1>(h), p2 = hf<V2>(h), p3 = hf<V3>(h);
+    const int ss = (p0 > 0 ? 1 : -1) + (p1 > 0 ? 1 : -1) + (p2 > 0 ? 1 : -1) + (p3 > 0 ? 1 : -1);
+    if (ss == 4) return R(0);
+    if (ss == -4) return R(1);
+    const r3<R> q0 = hp<V0>(h), q1 = hp<V1>(h), q2 = hp<V2>(h), q3 = hp<V3>(h);
+    // later assignments win: the first vertex in list order with a sign change
+    r3<R> B = q3;
+    R pB = p3;
+    if (p0 * p2 <= 0) {
+        B = q2;
+        pB = p2;
+    }
+    if (p0 * p1 <= 0) {
+        B = q1;
+        pB = p1;
+    }
+    const r3<R> ap = apex_point(q0, p0, B, pB);
+    auto tri = [&](r3<R> O, r3<R> A, R pO, R pA) {
+        return fabs(R(0.5) * rmag(rcross(sub(A, O), sub(ap, O)))) * line_fraction<R>(pO, pA);
+    };
+    const R area = tri(q0, q1, p0, p1) + tri(q1, q2, p1, p2) + tri(q2, q3, p2, p3) + tri(q3, q0, p3, p0);
+    return area / sf_mag;
+}
+
+// Face data of one hex cell: centres (relative to vertex 0) and area vectors in
+// the order -x, +x, -y, +y, -z, +z, and whether each minus face is on the
+// boundary (its vertex list then starts/winds differently: an internal face
+// takes its owner's -- the lower-index neighbour's -- listing).
 template <class R>
-struct CellFaces {
-    r3<R> fc[6];  // centres relative to vertex 0
-    r3<R> sf[6];  // area vectors
-    const unsigned char* lst[6];
-    int n;
+struct HexFaces {
+    r3<R> fc[6];
+    r3<R> sf[6];
+    bool bnd[3];
 };
 
-// Volume fraction: sum over faces of (1/3) eps_f |(apex - fc) . Sf|, divided by V
-// (geometrictools.cpp:47-72).  apex_z_rel: if two_d, the apex z (relative) is
-// forced to this value, i.e. world z = 0 (geometrictools.cpp:57-58).
+// Solid volume fraction of a hex cell (geometrictools.cpp:47-72): sum over faces
+// of (1/3) eps_f |(apex - fc).Sf|, divided by V.  two_d: the apex z is forced to
+// apex_z_rel (world z = 0, geometrictools.cpp:57-58).
 template <class R>
-__device__ inline R cell_fraction(const r3<R>* P, const R* phi, const CellFaces<R>& F, R V, bool two_d,
-                                  R apex_z_rel) {
-    const unsigned char kCell[8] = {0, 1, 2, 3, 4, 5, 6, 7};
-    r3<R> ap = apex_of<R, 8>(P, phi, kCell);
+__device__ inline R hex_fraction(const Hex8<R>& h, const HexFaces<R>& F, R V, bool two_d, R apex_z_rel) {
+    // cell apex: A = vertex 0, B = first later vertex with a sign change, else vertex 7
+    // B is blended as sum_t w_t P_t with exactly one weight 1 and the rest 0, which is
+    // exact and keeps the vertices in registers (a select chain is lowered to an
+    // indexed load from memory).
+    const R f0 = h.f0;
+    const bool c1 = f0 * h.f1 <= 0, c2 = f0 * h.f2 <= 0, c3 = f0 * h.f3 <= 0, c4 = f0 * h.f4 <= 0,
+               c5 = f0 * h.f5 <= 0, c6 = f0 * h.f6 <= 0;
+    const R w1 = c1 ? R(1) : R(0);
+    const R w2 = (!c1 && c2) ? R(1) : R(0);
+    const R w3 = (!c1 && !c2 && c3) ? R(1) : R(0);
+    const R w4 = (!c1 && !c2 && !c3 && c4) ? R(1) : R(0);
+    const R w5 = (!c1 && !c2 && !c3 && !c4 && c5) ? R(1) : R(0);
+    const R w6 = (!c1 && !c2 && !c3 && !c4 && !c5 && c6) ? R(1) : R(0);
+    const R w7 = R(1) - (w1 + w2 + w3 + w4 + w5 + w6);
+    auto blend = [&](R a1, R a2, R a3, R a4, R a5, R a6, R a7) {
+        return w1 * a1 + w2 * a2 + w3 * a3 + w4 * a4 + w5 * a5 + w6 * a6 + w7 * a7;
+    };
+    const r3<R> B{blend(h.p1.x, h.p2.x, h.p3.x, h.p4.x, h.p5.x, h.p6.x, h.p7.x),
+                  blend(h.p1.y, h.p2.y, h.p3.y, h.p4.y, h.p5.y, h.p6.y, h.p7.y),
+                  blend(h.p1.z, h.p2.z, h.p3.z, h.p4.z, h.p5.z, h.p6.z, h.p7.z)};
+    const R pB = blend(h.f1, h.f2, h.f3, h.f4, h.f5, h.f6, h.f7);
+    r3<R> ap = apex_point(h.p0, f0, B, pB);
     if (two_d) ap.z = apex_z_rel;
+    R eps[6];
+    eps[0] = F.bnd[0] ? face_fraction<R, 0, 4, 6, 2>(h, rmag(F.sf[0])) : face_fraction<R, 0, 2, 6, 4>(h, rmag(F.sf[0]));
+    eps[1] = face_fraction<R, 1, 3, 7, 5>(h, rmag(F.sf[1]));
+    eps[2] = F.bnd[1] ? face_fraction<R, 0, 1, 5, 4>(h, rmag(F.sf[2])) : face_fraction<R, 0, 4, 5, 1>(h, rmag(F.sf[2]));
+    eps[3] = face_fraction<R, 2, 6, 7, 3>(h, rmag(F.sf[3]));
+    eps[4] = F.bnd[2] ? face_fraction<R, 0, 2, 3, 1>(h, rmag(F.sf[4])) : face_fraction<R, 0, 1, 3, 2>(h, rmag(F.sf[4]));
+    eps[5] = face_fraction<R, 4, 5, 7, 6>(h, rmag(F.sf[5]));
     R vol = R(0);
-    for (int f = 0; f < F.n; ++f) {
-        const R eps = face_area_fraction<R>(P, phi, F.lst[f], F.sf[f]);
-        if (eps != R(0)) vol += (R(1) / R(3)) * eps * fabs(rdot(sub(ap, F.fc[f]), F.sf[f]));
-    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+        if (eps[f] != R(0)) vol += (R(1) / R(3)) * eps[f] * fabs(rdot(sub(ap, F.fc[f]), F.sf[f]));
     return vol / V;
 }
 

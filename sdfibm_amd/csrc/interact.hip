@@ -498,7 +498,7 @@ __global__ void k_plane_member(const int* __restrict__ list, const int* __restri
 }
 
 // Field pass: one workgroup per brick of BX*BY*BZ cells.
-template <class R, int MK, int BX, int BY, int BZ>
+template <class R, int MK, int BX, int BY, int BZ, bool HASP>
 __global__ void __launch_bounds__(BX* BY* BZ)
     k_fields(MeshDev m, int nbx, int nby, const long long* __restrict__ boff, const int* __restrict__ blist,
              long long pair_cap, const SolidK* __restrict__ Ks, const TileInfo* __restrict__ Ts,
@@ -549,9 +549,10 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     bool have_u = false;
     d3 uf{0, 0, 0}, cc{0, 0, 0};
     double V = 0.0;
-    double pacc[kMaxPlanes][6];
+    constexpr int NPACC = HASP ? kMaxPlanes : 1;
+    double pacc[NPACC][6];
 #pragma unroll
-    for (int p = 0; p < kMaxPlanes; ++p)
+    for (int p = 0; p < NPACC; ++p)
 #pragma unroll
         for (int t = 0; t < 6; ++t) pacc[p][t] = 0.0;
 
@@ -572,7 +573,7 @@ __global__ void __launch_bounds__(BX* BY* BZ)
         sF[1] += f.y * dtINV;
         sF[2] += f.z * dtINV;
         ct = ctype;
-        if (plane_slot >= 0) {
+        if (HASP && plane_slot >= 0) {
             const d3 tq = cross(r, f);
             double* a = pacc[plane_slot];
             a[0] += (f.x * V) * dtINV;
@@ -585,9 +586,10 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     };
 
     int ia = 0, ip = 0;
+    if (!HASP) nplanes = 0;
     while (ia < cnt || ip < nplanes) {
         const int sa = ia < cnt ? lst[ia] : 0x7fffffff;
-        const int sp = ip < nplanes ? planes[ip] : 0x7fffffff;
+        const int sp = (HASP && ip < nplanes) ? planes[ip] : 0x7fffffff;
         if (sa < sp) {
             ++ia;
             if (!valid) continue;
@@ -602,7 +604,7 @@ __global__ void __launch_bounds__(BX* BY* BZ)
                 contribute(K, 1.0, sa + 4, -1);
             else
                 contribute(K, (double)fabs(code), signbit(code) ? 3 : 2, -1);
-        } else {
+        } else if constexpr (HASP) {
             const int slot = ip;
             ++ip;
             const SolidK K = Ks[sp];
@@ -641,7 +643,7 @@ __global__ void __launch_bounds__(BX* BY* BZ)
         Tsf[c] = R(sT);
         Ct[c] = ct;
     }
-    if (nplanes > 0) {
+    if (HASP && nplanes > 0) {
         __syncthreads();
         for (int p = 0; p < nplanes; ++p) {
             if (!pl_any[p]) {
@@ -950,19 +952,24 @@ int launch_step(sdfibm_ctx* c, double dt) {
                                           c->d_pscalars + 1, member, c->d_pseed + p);
     }
     (void)hipEventRecord(c->ev[2], st);
+#define SDFIBM_FIELDS(BX_, BY_, BZ_, HP_)                                                                       \
+    k_fields<R, MK, BX_, BY_, BZ_, HP_><<<(unsigned)c->nbricks, BX_ * BY_ * BZ_, 0, st>>>(                          \
+        c->m, c->nbx, c->nby, c->boff, c->d_blist, c->pair_cap, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np, \
+        c->d_pmember, c->member_words, c->brick_box, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],      \
+        (R*)c->As, (R*)c->Fs[0], (R*)c->Fs[1], (R*)c->Fs[2], (R*)c->Tsf, c->Ct, dt, c->d_plane_partial, c->nbricks, \
+        c->err)
     if (c->nz == 1) {
-        k_fields<R, MK, kBX2, kBY2, kBZ2><<<(unsigned)c->nbricks, kBX2 * kBY2 * kBZ2, 0, st>>>(
-            c->m, c->nbx, c->nby, c->boff, c->d_blist, c->pair_cap, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np,
-            c->d_pmember, c->member_words,
-            c->brick_box, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], (R*)c->As, (R*)c->Fs[0],
-            (R*)c->Fs[1], (R*)c->Fs[2], (R*)c->Tsf, c->Ct, dt, c->d_plane_partial, c->nbricks, c->err);
+        if (np > 0)
+            SDFIBM_FIELDS(kBX2, kBY2, kBZ2, true);
+        else
+            SDFIBM_FIELDS(kBX2, kBY2, kBZ2, false);
     } else {
-        k_fields<R, MK, kBX3, kBY3, kBZ3><<<(unsigned)c->nbricks, kBX3 * kBY3 * kBZ3, 0, st>>>(
-            c->m, c->nbx, c->nby, c->boff, c->d_blist, c->pair_cap, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np,
-            c->d_pmember, c->member_words,
-            c->brick_box, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], (R*)c->As, (R*)c->Fs[0],
-            (R*)c->Fs[1], (R*)c->Fs[2], (R*)c->Tsf, c->Ct, dt, c->d_plane_partial, c->nbricks, c->err);
+        if (np > 0)
+            SDFIBM_FIELDS(kBX3, kBY3, kBZ3, true);
+        else
+            SDFIBM_FIELDS(kBX3, kBY3, kBZ3, false);
     }
+#undef SDFIBM_FIELDS
     if (np > 0) k_plane_reduce<<<np, 256, 0, st>>>(c->d_plane_partial, c->nbricks, c->d_planes, c->rho_f, c->d_ft);
     (void)hipEventRecord(c->ev[3], st);
     return 0;

@@ -69,13 +69,11 @@ __device__ inline void cell_vertices(const MeshDev& m, int i, int j, int k, d3* 
     for (int v = 0; v < 8; ++v) P[v] = vertex_pos<MK>(m, i + (v & 1), j + ((v >> 1) & 1), k + ((v >> 2) & 1));
 }
 
-// Face data of cell (i,j,k) in the cell's OpenFOAM face order: owned internal
-// faces (+x, +y, +z), boundary faces in patch order (xmin, xmax, ymin, ymax,
-// zmin, zmax), then neighbour internal faces (-z, -y, -x).
+// Face centres (relative to v0) and area vectors of cell (i,j,k), order
+// -x, +x, -y, +y, -z, +z.
 template <class R, int MK>
-__device__ inline void cell_faces(const MeshDev& m, int i, int j, int k, d3 v0, CellFaces<R>& F) {
-    int n = 0;
-    auto put = [&](int fam, int a, int b, int c, const unsigned char* lst) {
+__device__ inline void hex_faces(const MeshDev& m, int i, int j, int k, d3 v0, HexFaces<R>& F) {
+    auto put = [&](int slot, int fam, int a, int b, int c) {
         d3 fc, sf;
         if constexpr (MK == MK_RECT) {
             if (fam == 0) {
@@ -101,43 +99,56 @@ __device__ inline void cell_faces(const MeshDev& m, int i, int j, int k, d3 v0, 
             fc = {pc[0], pc[1], pc[2]};
             sf = {ps[0], ps[1], ps[2]};
         }
-        F.fc[n] = {R(fc.x - v0.x), R(fc.y - v0.y), R(fc.z - v0.z)};
-        F.sf[n] = {R(sf.x), R(sf.y), R(sf.z)};
-        F.lst[n] = lst;
-        ++n;
+        F.fc[slot] = {R(fc.x - v0.x), R(fc.y - v0.y), R(fc.z - v0.z)};
+        F.sf[slot] = {R(sf.x), R(sf.y), R(sf.z)};
     };
-    if (i < m.nx - 1) put(0, i + 1, j, k, kFacePX);
-    if (j < m.ny - 1) put(1, i, j + 1, k, kFacePY);
-    if (k < m.nz - 1) put(2, i, j, k + 1, kFacePZ);
-    if (i == 0) put(0, i, j, k, kFaceMXb);
-    if (i == m.nx - 1) put(0, i + 1, j, k, kFacePX);
-    if (j == 0) put(1, i, j, k, kFaceMYb);
-    if (j == m.ny - 1) put(1, i, j + 1, k, kFacePY);
-    if (k == 0) put(2, i, j, k, kFaceMZb);
-    if (k == m.nz - 1) put(2, i, j, k + 1, kFacePZ);
-    if (k > 0) put(2, i, j, k, kFaceMZi);
-    if (j > 0) put(1, i, j, k, kFaceMYi);
-    if (i > 0) put(0, i, j, k, kFaceMXi);
-    F.n = n;
+    put(0, 0, i, j, k);
+    put(1, 0, i + 1, j, k);
+    put(2, 1, i, j, k);
+    put(3, 1, i, j + 1, k);
+    put(4, 2, i, j, k);
+    put(5, 2, i, j, k + 1);
+    F.bnd[0] = i == 0;
+    F.bnd[1] = j == 0;
+    F.bnd[2] = k == 0;
+}
+
+// Signed distance at world point p for the fraction: local = R^T (p - c) with
+// the solid's rotation matrix (the inside tests keep the quaternion form).
+template <class R>
+__device__ inline R sdf_fast(const SolidK& K, d3 p) {
+    const double dx = p.x - K.c[0], dy = p.y - K.c[1], dz = p.z - K.c[2];
+    const d3 l{K.rt[0] * dx + K.rt[1] * dy + K.rt[2] * dz, K.rt[3] * dx + K.rt[4] * dy + K.rt[5] * dz,
+               K.rt[6] * dx + K.rt[7] * dy + K.rt[8] * dz};
+    if constexpr (sizeof(R) == sizeof(float))
+        return sdf_local_mixed(K, l);
+    else
+        return sdf_local<R>(K, R(l.x), R(l.y), R(l.z));
 }
 
 // Volume fraction of cell (i,j,k) inside solid K (calcCellVolume / V), with the
 // 8 corner signed distances evaluated in R.
 template <class R, int MK>
 __device__ inline R fraction_of_cell(const MeshDev& m, const SolidK& K, int i, int j, int k) {
-    d3 Pw[8];
-    cell_vertices<MK>(m, i, j, k, Pw);
-    r3<R> P[8];
-    R phi[8];
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-        P[v] = {R(Pw[v].x - Pw[0].x), R(Pw[v].y - Pw[0].y), R(Pw[v].z - Pw[0].z)};
-        phi[v] = sdf_world<R>(K, Pw[v]);
-    }
-    CellFaces<R> F;
-    cell_faces<R, MK>(m, i, j, k, Pw[0], F);
+    const d3 w0 = vertex_pos<MK>(m, i, j, k);
+    Hex8<R> h;
+    auto vtx = [&](int di, int dj, int dk, r3<R>& p, R& f) {
+        const d3 w = vertex_pos<MK>(m, i + di, j + dj, k + dk);
+        p = {R(w.x - w0.x), R(w.y - w0.y), R(w.z - w0.z)};
+        f = sdf_fast<R>(K, w);
+    };
+    vtx(0, 0, 0, h.p0, h.f0);
+    vtx(1, 0, 0, h.p1, h.f1);
+    vtx(0, 1, 0, h.p2, h.f2);
+    vtx(1, 1, 0, h.p3, h.f3);
+    vtx(0, 0, 1, h.p4, h.f4);
+    vtx(1, 0, 1, h.p5, h.f5);
+    vtx(0, 1, 1, h.p6, h.f6);
+    vtx(1, 1, 1, h.p7, h.f7);
+    HexFaces<R> F;
+    hex_faces<R, MK>(m, i, j, k, w0, F);
     const R V = R(cell_volume<MK>(m, i, j, k));
-    return cell_fraction<R>(P, phi, F, V, m.two_d != 0, R(0.0 - Pw[0].z));
+    return hex_fraction<R>(h, F, V, m.two_d != 0, R(0.0 - w0.z));
 }
 
 // ---------------------------------------------------------------------------

@@ -43,6 +43,7 @@ struct SolidK {
     double k[6];    // shape constants, see make_solid_k
     double v[3];    // velocity
     double w[3];    // angular velocity
+    double rt[9];   // R(q)^T row-major: local = rt (p - c), for the signed distance
 };
 
 __host__ __device__ inline SolidK make_solid_k(const sdfibm_solid& s) {
@@ -102,6 +103,19 @@ __host__ __device__ inline SolidK make_solid_k(const sdfibm_solid& s) {
         K.com[2] = p[5];
         break;
     default: break;
+    }
+    {
+        const double w = s.q[0], x = s.q[1], y = s.q[2], z = s.q[3];
+        // rows of R^T = columns of R(q)
+        K.rt[0] = 1 - 2 * (y * y + z * z);
+        K.rt[1] = 2 * (x * y + w * z);
+        K.rt[2] = 2 * (x * z - w * y);
+        K.rt[3] = 2 * (x * y - w * z);
+        K.rt[4] = 1 - 2 * (x * x + z * z);
+        K.rt[5] = 2 * (y * z + w * x);
+        K.rt[6] = 2 * (x * z + w * y);
+        K.rt[7] = 2 * (y * z - w * x);
+        K.rt[8] = 1 - 2 * (x * x + y * y);
     }
     return K;
 }
