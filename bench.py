@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scene-cache", default="", help="npz file to reuse the generated scene")
+    ap.add_argument("--copy-test", type=float, default=0.0,
+                    help="only run a plain device copy of this many GB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the coupling step is benchmarked on a single MI355X "
@@ -113,9 +116,32 @@ def main():
     from sdfibm_amd import scenes
     from sdfibm_amd.cloud import SolidCloud
 
+    if args.copy_test > 0:
+        n = int(args.copy_test * 1e9) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}), flush=True)
+        return
+
     t0 = time.time()
-    sc = scenes.cloud_3d(args.n, args.solids)
-    log(f"[bench] scene {sc.name} generated in {time.time() - t0:.1f}s: {sc.info}")
+    sc = None
+    if args.scene_cache and os.path.exists(args.scene_cache):
+        z = np.load(args.scene_cache, allow_pickle=False)
+        if int(z["n"]) == args.n and int(z["ns"]) == args.solids:
+            from sdfibm_amd import mesh as M
+            sc = scenes.Scene(f"cloud3d_{args.n}_{args.solids}",
+                              M.uniform_box((args.n,) * 3, (0, 0, 0), (1, 1, 1)), z["solids"], z["U"],
+                              False, 1.0, 1e-3, json.loads(str(z["info"])))
+    if sc is None:
+        sc = scenes.cloud_3d(args.n, args.solids)
+        if args.scene_cache:
+            np.savez(args.scene_cache, n=args.n, ns=args.solids, solids=sc.solids, U=sc.U,
+                     info=json.dumps({k: (float(v) if isinstance(v, (np.floating,)) else v)
+                                      for k, v in sc.info.items()}))
+    log(f"[bench] scene {sc.name} ready in {time.time() - t0:.1f}s: {sc.info}")
     N = sc.mesh.n_cells
     cloud = SolidCloud(sc.mesh, two_d=sc.two_d, rho_f=sc.rho_f, precision=args.precision)
     cloud.set_solids(sc.solids)

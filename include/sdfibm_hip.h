@@ -126,6 +126,19 @@ int sdfibm_set_velocity(sdfibm_ctx* ctx, const double* U_aos);
  * Fields stay on the device. */
 int sdfibm_interact(sdfibm_ctx* ctx, double time, double dt, double* force_torque);
 
+/* The step after interact in the reference's solver loop: U = U - Fs*dt over
+ * the resident fields (src/main.cpp:70). */
+int sdfibm_update_velocity(sdfibm_ctx* ctx, double dt);
+
+/* SolidCloud::fixInternal (src/solidcloud.cpp:288-301): in every cell with
+ * Ct >= 4 (wholly inside solid Ct-4, from the last interact) U becomes that
+ * solid's rigid velocity v + omega x (cc - centre), with the solid state last
+ * passed to sdfibm_set_solids. */
+int sdfibm_fix_internal(sdfibm_ctx* ctx);
+
+/* Device -> host copy of U as AoS FP64 [N*3]. */
+int sdfibm_get_velocity(sdfibm_ctx* ctx, double* U_aos);
+
 /* Device -> host copies of the solid fields as FP64 (any pointer may be NULL):
  * As [N], Fs [N*3] AoS, Ts [N], Ct [N] (Ct stored as the reference's double field). */
 int sdfibm_get_fields(sdfibm_ctx* ctx, double* As, double* Fs_aos, double* Ts, double* Ct);

@@ -59,6 +59,8 @@ def lib():
         L.orc_interact_range.restype = C.c_int
         L.orc_interact_range.argtypes = [P, C.c_int, C.c_double, C.c_double, i64, P, i64, i64, P,
                                          P, P, P, P, P, C.c_int]
+        L.orc_update_velocity.argtypes = [i64, P, P, C.c_double]
+        L.orc_fix_internal.argtypes = [P, P, P, i64, P]
         del dp
         _lib = L
     return _lib
@@ -157,6 +159,19 @@ class OracleMesh:
             out["cells"] = out["cells"][:cap.used]
             out["alpha"] = out["alpha"][:cap.used]
         return out
+
+    def update_velocity(self, U, Fs, dt):
+        U = np.ascontiguousarray(U, dtype=np.float64).copy()
+        Fs = np.ascontiguousarray(Fs, dtype=np.float64)
+        lib().orc_update_velocity(self.n_cells, _ptr(U), _ptr(Fs), float(dt))
+        return U
+
+    def fix_internal(self, U, Ct, solids):
+        U = np.ascontiguousarray(U, dtype=np.float64).copy()
+        Ct = np.ascontiguousarray(Ct, dtype=np.float64)
+        solids = np.ascontiguousarray(solids)
+        lib().orc_fix_internal(self.h, _ptr(U), _ptr(Ct), solids.size, _ptr(solids))
+        return U
 
     def interact_range(self, solids, U, s0, s1, *, two_d=False, rho_f=1.0, dt=1.0, mode=0,
                        fields=None):

@@ -1031,4 +1031,27 @@ int orc_interact(const orc_mesh* om, int two_d, double rho_f, double dt, int64_t
     return rc;
 }
 
+// U = U - Fs*dt (src/main.cpp:70) on the internal field.
+void orc_update_velocity(int64_t n, double* U, const double* Fs, double dt) {
+    for (int64_t c = 0; c < 3 * n; ++c) U[c] = U[c] - Fs[c] * dt;
+}
+
+// fixInternal (src/solidcloud.cpp:288-301): cells fully inside a solid (Ct >= 4)
+// take that solid's rigid velocity v + omega x (cc - centre).
+void orc_fix_internal(const orc_mesh* om, double* U, const double* Ct, int64_t ns, const orc_solid* solids) {
+    const Mesh& m = om->m;
+    for (int64_t c = 0; c < m.nC; ++c) {
+        if (Ct[c] >= 4) {
+            const int64_t id = (int64_t)Ct[c] - 4;
+            if (id < 0 || id >= ns) continue;
+            const orc_solid& s = solids[id];
+            const V3 v = V3{s.vel[0], s.vel[1], s.vel[2]} +
+                         cross(V3{s.omega[0], s.omega[1], s.omega[2]}, m.C[c] - V3{s.center[0], s.center[1], s.center[2]});
+            U[3 * c] = v.x;
+            U[3 * c + 1] = v.y;
+            U[3 * c + 2] = v.z;
+        }
+    }
+}
+
 }  // extern "C"

@@ -108,6 +108,11 @@ int orc_interact_range(const orc_mesh* m, int two_d, double rho_f, double dt,
                        const double* U, double* As, double* Fs, double* Ts, double* Ct,
                        double* force_torque, int mode);
 
+/* The steps around interact: U -= Fs*dt (main.cpp:70) and fixInternal
+ * (solidcloud.cpp:288-301).  U, Fs AoS [3N]; Ct as the double field. */
+void orc_update_velocity(int64_t n, double* U, const double* Fs, double dt);
+void orc_fix_internal(const orc_mesh* m, double* U, const double* Ct, int64_t ns, const orc_solid* solids);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,7 @@ EXPORTS = (
     "sdfibm_create", "sdfibm_destroy", "sdfibm_set_solids", "sdfibm_set_velocity",
     "sdfibm_interact", "sdfibm_get_fields", "sdfibm_field_pointers", "sdfibm_candidates",
     "sdfibm_last_timing", "sdfibm_set_stream", "sdfibm_last_error", "sdfibm_status_string",
-    "sdfibm_abi_version",
+    "sdfibm_abi_version", "sdfibm_update_velocity", "sdfibm_fix_internal", "sdfibm_get_velocity",
 )
 
 SDFIBM_OK = 0
@@ -70,6 +70,9 @@ def load():
         "sdfibm_last_error": (C.c_char_p, [P]),
         "sdfibm_status_string": (C.c_char_p, [i32]),
         "sdfibm_abi_version": (i32, []),
+        "sdfibm_update_velocity": (i32, [P, d]),
+        "sdfibm_fix_internal": (i32, [P]),
+        "sdfibm_get_velocity": (i32, [P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

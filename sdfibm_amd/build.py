@@ -41,7 +41,24 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + res.stderr[-8000:])
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+DEMO = os.path.join(HERE, "host", "coupling_demo")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ host example linked against the library (plain g++, no HIP headers)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           "-o", DEMO, os.path.join(HERE, "host", "coupling_demo.cpp"), "-L", HERE, "-lsdfibm_hip",
+           "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + res.stderr[-4000:])
+    return DEMO
 
 
 if __name__ == "__main__":

@@ -113,6 +113,19 @@ class SolidCloud:
         """interact without copying force/torque back (it still waits for the step)."""
         _lib.check(_lib.load().sdfibm_interact(self._h, float(time), float(dt), None), self._h)
 
+    def update_velocity(self, dt: float) -> None:
+        """U -= Fs*dt on the device (the solver step after interact, src/main.cpp:70)."""
+        _lib.check(_lib.load().sdfibm_update_velocity(self._h, float(dt)), self._h)
+
+    def fix_internal(self) -> None:
+        """SolidCloud::fixInternal: U = rigid velocity in cells wholly inside a solid."""
+        _lib.check(_lib.load().sdfibm_fix_internal(self._h), self._h)
+
+    def velocity(self) -> np.ndarray:
+        U = np.zeros((self.n_cells, 3))
+        _lib.check(_lib.load().sdfibm_get_velocity(self._h, _p(U)), self._h)
+        return U
+
     def fields(self) -> dict:
         n = self.n_cells
         out = dict(As=np.zeros(n), Fs=np.zeros((n, 3)), Ts=np.zeros(n), Ct=np.zeros(n))

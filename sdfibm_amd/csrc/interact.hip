@@ -699,6 +699,38 @@ __global__ void k_plane_codes(MeshDev m, SolidK K, const unsigned* __restrict__ 
     }
 }
 
+// U -= Fs*dt (src/main.cpp:70), FP64 arithmetic, stored in R.
+template <class R>
+__global__ void k_update_velocity(long long n, R* __restrict__ Ux, R* __restrict__ Uy, R* __restrict__ Uz,
+                                  const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+                                  double dt) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    Ux[c] = R((double)Ux[c] - (double)Fx[c] * dt);
+    Uy[c] = R((double)Uy[c] - (double)Fy[c] * dt);
+    Uz[c] = R((double)Uz[c] - (double)Fz[c] * dt);
+}
+
+// fixInternal (src/solidcloud.cpp:288-301): where Ct >= 4 the cell lies wholly in
+// solid Ct-4 and U takes that solid's rigid velocity at the cell centre.
+template <class R, int MK>
+__global__ void k_fix_internal(MeshDev m, const int* __restrict__ Ct, const sdfibm_solid* __restrict__ S, int ns,
+                               R* __restrict__ Ux, R* __restrict__ Uy, R* __restrict__ Uz) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long N = (long long)m.nx * m.ny * m.nz;
+    if (c >= N) return;
+    const int t = Ct[c];
+    if (t < 4 || t - 4 >= ns) return;
+    const sdfibm_solid& s = S[t - 4];
+    const int i = (int)(c % m.nx), j = (int)((c / m.nx) % m.ny), k = (int)(c / ((long long)m.nx * m.ny));
+    const d3 cc = cell_centre<MK>(m, i, j, k);
+    const d3 r{cc.x - s.center[0], cc.y - s.center[1], cc.z - s.center[2]};
+    const d3 u = d3{s.vel[0], s.vel[1], s.vel[2]} + cross(d3{s.omega[0], s.omega[1], s.omega[2]}, r);
+    Ux[c] = R(u.x);
+    Uy[c] = R(u.y);
+    Uz[c] = R(u.z);
+}
+
 template <class R>
 __global__ void k_aos_to_soa(const double* __restrict__ in, long long n, R* __restrict__ x, R* __restrict__ y,
                              R* __restrict__ z) {
@@ -1407,6 +1439,62 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
         return fail(c, (flags & EF_TILE) ? SDFIBM_ERR_TILE : SDFIBM_ERR_CAPACITY, err_text(flags));
     }
     c->have_step = true;
+    return SDFIBM_OK;
+}
+
+int sdfibm_update_velocity(sdfibm_ctx* c, double dt) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite");
+    const unsigned g = grid_for(c->N, 256);
+    if (c->prec == SDFIBM_FP64)
+        k_update_velocity<double><<<g, 256, 0, c->stream>>>(c->N, (double*)c->U[0], (double*)c->U[1], (double*)c->U[2],
+                                                            (const double*)c->Fs[0], (const double*)c->Fs[1],
+                                                            (const double*)c->Fs[2], dt);
+    else
+        k_update_velocity<float><<<g, 256, 0, c->stream>>>(c->N, (float*)c->U[0], (float*)c->U[1], (float*)c->U[2],
+                                                           (const float*)c->Fs[0], (const float*)c->Fs[1],
+                                                           (const float*)c->Fs[2], dt);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SDFIBM_OK;
+}
+
+int sdfibm_fix_internal(sdfibm_ctx* c) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!c->have_step) return fail(c, SDFIBM_ERR_STATE, "fix_internal needs the Ct field of a successful interact");
+    const unsigned g = grid_for(c->N, 256);
+    const int ns = (int)c->ns;
+#define SDFIBM_FIX(R_, MK_)                                                                                  \
+    k_fix_internal<R_, MK_><<<g, 256, 0, c->stream>>>(c->m, c->Ct, c->d_solids, ns, (R_*)c->U[0], (R_*)c->U[1], \
+                                                     (R_*)c->U[2])
+    if (c->prec == SDFIBM_FP64) {
+        if (c->kind == SDFIBM_MESH_RECTILINEAR)
+            SDFIBM_FIX(double, MK_RECT);
+        else
+            SDFIBM_FIX(double, MK_STRUCT);
+    } else {
+        if (c->kind == SDFIBM_MESH_RECTILINEAR)
+            SDFIBM_FIX(float, MK_RECT);
+        else
+            SDFIBM_FIX(float, MK_STRUCT);
+    }
+#undef SDFIBM_FIX
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SDFIBM_OK;
+}
+
+int sdfibm_get_velocity(sdfibm_ctx* c, double* U) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!U) return fail(c, SDFIBM_ERR_ARG, "U is NULL");
+    const size_t rs = rsize(c);
+    std::vector<unsigned char> buf(c->N * rs);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) {
+        HIPCHK(c, hipMemcpy(buf.data(), c->U[a], c->N * rs, hipMemcpyDeviceToHost));
+        for (long long q = 0; q < c->N; ++q)
+            U[3 * q + a] = rs == 8 ? ((const double*)buf.data())[q] : (double)((const float*)buf.data())[q];
+    }
     return SDFIBM_OK;
 }
 

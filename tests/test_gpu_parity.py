@@ -175,3 +175,48 @@ def test_repeat_is_bitwise_deterministic():
     assert np.array_equal(a, b)
     for k in fa:
         assert np.array_equal(fa[k], fb[k])
+
+
+def test_cpp_host_demo():
+    """The C++ mirror of SolidCloud (include/sdfibm_hip.hpp) driving interact from a g++ program:
+    step 0 on a resting disc in uniform U = (1,0,0) gives F = rho_f (sum alpha V) U / dt."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sdfibm_amd", "host",
+                       "coupling_demo")
+    out = subprocess.run([exe, "2"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [list(map(float, ln.split())) for ln in out.stdout.strip().splitlines()]
+    assert len(rows) == 2
+    step, vol, fx, fy = rows[0]
+    assert abs(vol / (np.pi * 0.25) - 1) < 0.01
+    assert abs(fx - vol * 1.0 / 0.01) < 1e-9 * abs(fx)
+    assert abs(fy) < 1e-9 * abs(fx)
+    assert abs(rows[1][2]) < abs(fx)  # U -= Fs dt removed the slip inside the disc
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_update_velocity_and_fix_internal(prec):
+    """U -= Fs dt (main.cpp:70) then fixInternal with moved solids (solidcloud.cpp:288-301)."""
+    sc = scenes.small_mixed_3d(20, seed=4)
+    om = oracle.OracleMesh(sc.mesh.to_polymesh())
+    ref = om.interact(sc.solids, sc.U, rho_f=sc.rho_f, dt=sc.dt)
+    U1 = om.update_velocity(sc.U, ref["Fs"], sc.dt)
+    moved = sc.solids.copy()
+    moved["vel"] += 0.05
+    moved["omega"] *= -1.0
+    U2 = om.fix_internal(U1, ref["Ct"], moved)
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    with SolidCloud(sc.mesh, rho_f=sc.rho_f, precision=prec) as c:
+        c.set_solids(sc.solids)
+        c.set_velocity(sc.U)
+        c.interact(0.0, sc.dt)
+        c.update_velocity(sc.dt)
+        g1 = c.velocity()
+        c.set_solids(moved)
+        c.fix_internal()
+        g2 = c.velocity()
+    scale = np.abs(U1).max()
+    assert np.max(np.abs(g1 - U1)) <= tol * scale
+    assert np.max(np.abs(g2 - U2)) <= tol * scale
+    assert np.any(ref["Ct"] >= 4)
