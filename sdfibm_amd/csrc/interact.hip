@@ -113,7 +113,7 @@ __global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, 
     TileInfo t = make_tile<MK>(m, k);
     t.seed = t.cls == TC_EMPTY ? -1 : nearest_cell<MK>(m, k.c);
     if (t.cls == TC_TOOBIG) atomicOr(err, EF_TILE);
-    tile_cells[s] = (t.cls == TC_SMALL || t.cls == TC_LARGE) ? (long long)t.n[0] * t.n[1] * t.n[2] : 0;
+    tile_cells[s] = tile_is_finite(t.cls) ? (long long)t.n[0] * t.n[1] * t.n[2] : 0;
     T[s] = t;
     for (int q = 0; q < 6; ++q) ft[6 * s + q] = 0.0;  // solids that touch no cell exert nothing
 }
@@ -153,7 +153,51 @@ __global__ void k_tile_offsets(TileInfo* __restrict__ T, const long long* __rest
 }
 
 // Per-solid tile pass.  One workgroup per solid of class CLS.
-template <class R, int MK, int MAXR, int CLS>
+// Exclusive prefix over n rows of count(r) into pre[0..n] (block-wide, kBlock threads).
+template <class F>
+__device__ inline void block_row_scan(int n, F count, int* pre, int* part) {
+    const int tid = threadIdx.x;
+    const int chunk = (n + kBlock - 1) / kBlock;
+    const int r0 = tid * chunk, r1 = min(r0 + chunk, n);
+    int sum = 0;
+    for (int r = r0; r < r1; ++r) sum += count(r);
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = tid > 0 ? part[tid - 1] : 0;
+    for (int r = r0; r < r1; ++r) {
+        pre[r] = run;
+        run += count(r);
+    }
+    if (tid == kBlock - 1) pre[n] = part[kBlock - 1];
+    __syncthreads();
+}
+
+// Position of row containing the q-th set bit: pre[r] <= q < pre[r+1].
+__device__ inline int row_of(const int* pre, int n, int q) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= q)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Index of the t-th set bit of w.
+__device__ inline int select_bit(unsigned long long w, int t) {
+    for (; t > 0; --t) w &= w - 1;
+    return __builtin_ctzll(w);
+}
+
+template <class R, int MK, int MAXR, int PHICAP, int CLS>
 __global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const SolidK* __restrict__ Ks,
                                                   const TileInfo* __restrict__ Ts, const R* __restrict__ Ux,
                                                   const R* __restrict__ Uy, const R* __restrict__ Uz,
@@ -257,29 +301,43 @@ __global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const Solid
     }
 
     // 5. cut cells (reached, not all inside): exclusive row prefix counts
-    {
-        __shared__ int part[kBlock];
-        const int chunk = (CR + kBlock - 1) / kBlock;
-        const int r0 = tid * chunk, r1 = min(r0 + chunk, CR);
-        int sum = 0;
-        for (int r = r0; r < r1; ++r) sum += __builtin_popcountll(reach[r] & ~all8[r]);
-        part[tid] = sum;
-        __syncthreads();
-        for (int off = 1; off < kBlock; off <<= 1) {
-            const int v = tid >= off ? part[tid - off] : 0;
-            __syncthreads();
-            part[tid] += v;
-            __syncthreads();
-        }
-        int run = tid > 0 ? part[tid - 1] : 0;
-        for (int r = r0; r < r1; ++r) {
-            rowpre[r] = run;
-            run += __builtin_popcountll(reach[r] & ~all8[r]);
-        }
-        if (tid == kBlock - 1) rowpre[CR] = part[kBlock - 1];
-        __syncthreads();
-    }
+    __shared__ int part[kBlock];
+    block_row_scan(CR, [&](int r) { return __builtin_popcountll(reach[r] & ~all8[r]); }, rowpre, part);
     const int ncut = rowpre[CR];
+
+    // 5b. signed distance of every vertex of a cut cell, once (the reference's per-solid
+    //     vertex cache, geometrictools.cpp:6-11).  vin is reused for the needed-vertex rows.
+    constexpr int VPRE = PHICAP > 0 ? MAXR + 1 : 1;
+    __shared__ int vpre[VPRE];
+    __shared__ R phic[PHICAP > 0 ? PHICAP : 1];
+    bool cached = false;
+    if constexpr (PHICAP > 0) {
+        for (int vr = tid; vr < VR; vr += kBlock) {
+            const int vj = vr % VY, vk = vr / VY;
+            unsigned long long need = 0ull;
+            for (int k = vk - 1; k <= vk; ++k) {
+                if (k < 0 || k >= nz) continue;
+                for (int j = vj - 1; j <= vj; ++j) {
+                    if (j < 0 || j >= ny) continue;
+                    const unsigned long long cut = reach[k * ny + j] & ~all8[k * ny + j];
+                    need |= cut | (cut << 1);
+                }
+            }
+            vin[vr] = need;
+        }
+        __syncthreads();
+        block_row_scan(VR, [&](int r) { return __builtin_popcountll(vin[r]); }, vpre, part);
+        const int nneed = vpre[VR];
+        if (nneed <= PHICAP) {
+            cached = true;
+            for (int q = tid; q < nneed; q += kBlock) {
+                const int vr = row_of(vpre, VR, q);
+                const int i = select_bit(vin[vr], q - vpre[vr]);
+                phic[q] = sdf_fast<R>(K, vertex_pos<MK>(m, T.lo[0] + i, T.lo[1] + vr % VY, T.lo[2] + vr / VY));
+            }
+            __syncthreads();
+        }
+    }
 
     const double dtINV = 1.0 / dt;
     double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -316,20 +374,20 @@ __global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const Solid
     }
     // 6b. compacted pass over the cut cells: fraction, centre test, forcing
     for (int q = tid; q < ncut; q += kBlock) {
-        int lo = 0, hi = CR;  // row r with rowpre[r] <= q < rowpre[r+1]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (rowpre[mid] <= q)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const int r = lo;
-        unsigned long long bits = reach[r] & ~all8[r];
-        for (int t = q - rowpre[r]; t > 0; --t) bits &= bits - 1;
-        const int i = __builtin_ctzll(bits);
+        const int r = row_of(rowpre, CR, q);
+        const int i = select_bit(reach[r] & ~all8[r], q - rowpre[r]);
         const int gi = T.lo[0] + i, gj = T.lo[1] + r % ny, gk = T.lo[2] + r / ny;
-        const R alpha = fraction_of_cell<R, MK>(m, K, gi, gj, gk);
+        R alpha;
+        if (cached) {
+            const int lj = r % ny, lk = r / ny;
+            alpha = fraction_with_phi<R, MK>(m, gi, gj, gk, [&](int di, int dj, int dk, d3) {
+                const int vr = (lk + dk) * VY + (lj + dj);
+                const int b = i + di;
+                return phic[vpre[vr] + __builtin_popcountll(vin[vr] & ((1ull << b) - 1ull))];
+            });
+        } else {
+            alpha = fraction_of_cell<R, MK>(m, K, gi, gj, gk);
+        }
         const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
         tcodes[(long long)r * nx + i] = cin ? alpha : -alpha;
         force(gi, gj, gk, (double)alpha);
@@ -346,7 +404,7 @@ __global__ void k_bin(int ns, MeshDev m, const TileInfo* __restrict__ T, int nbx
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     const TileInfo t = T[s];
-    if (t.cls != TC_SMALL && t.cls != TC_LARGE) return;
+    if (!tile_is_finite(t.cls)) return;
     const int b0x = t.lo[0] / BX, b1x = (t.lo[0] + t.n[0] - 1) / BX;
     const int b0y = t.lo[1] / BY, b1y = (t.lo[1] + t.n[1] - 1) / BY;
     const int b0z = t.lo[2] / BZ, b1z = (t.lo[2] + t.n[2] - 1) / BZ;
@@ -791,7 +849,7 @@ struct sdfibm_ctx {
     int* err = nullptr;
     // solids
     long long ns = 0;
-    bool have_solids = false, have_step = false;
+    bool have_solids = false, have_step = false, have_ct = false;
     std::vector<sdfibm_solid> h_solids;
     sdfibm_solid* d_solids = nullptr;
     SolidK* d_K = nullptr;
@@ -941,10 +999,13 @@ int launch_step(sdfibm_ctx* c, double dt) {
     (void)hipEventRecord(c->ev[1], st);
     R* codes = (R*)c->d_codes;
     if (ns > 0) {
-        k_solid<R, MK, kRowsSmall, TC_SMALL><<<ns, kBlock, 0, st>>>(
+        k_solid<R, MK, kRowsSmall, 2048, TC_SMALL><<<ns, kBlock, 0, st>>>(
             c->m, ns, c->d_K, c->d_T, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap,
             dt, c->rho_f, c->d_ft);
-        k_solid<R, MK, kRowsLarge, TC_LARGE><<<ns, kBlock, 0, st>>>(
+        k_solid<R, MK, kRowsMedium, 4096, TC_MEDIUM><<<ns, kBlock, 0, st>>>(
+            c->m, ns, c->d_K, c->d_T, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap,
+            dt, c->rho_f, c->d_ft);
+        k_solid<R, MK, kRowsLarge, 0, TC_LARGE><<<ns, kBlock, 0, st>>>(
             c->m, ns, c->d_K, c->d_T, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap,
             dt, c->rho_f, c->d_ft);
     }
@@ -1436,9 +1497,11 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     c->timing[3] = (double)ms[0] + ms[1] + ms[2];
     if (flags) {
         c->have_step = false;
+        c->have_ct = false;
         return fail(c, (flags & EF_TILE) ? SDFIBM_ERR_TILE : SDFIBM_ERR_CAPACITY, err_text(flags));
     }
     c->have_step = true;
+    c->have_ct = true;
     return SDFIBM_OK;
 }
 
@@ -1461,7 +1524,7 @@ int sdfibm_update_velocity(sdfibm_ctx* c, double dt) {
 
 int sdfibm_fix_internal(sdfibm_ctx* c) {
     if (!c) return SDFIBM_ERR_ARG;
-    if (!c->have_step) return fail(c, SDFIBM_ERR_STATE, "fix_internal needs the Ct field of a successful interact");
+    if (!c->have_ct) return fail(c, SDFIBM_ERR_STATE, "fix_internal needs the Ct field of a successful interact");
     const unsigned g = grid_for(c->N, 256);
     const int ns = (int)c->ns;
 #define SDFIBM_FIX(R_, MK_)                                                                                  \
@@ -1587,7 +1650,7 @@ int sdfibm_candidates(sdfibm_ctx* c, int64_t solid, int64_t* counts, int64_t* ce
         n[0] = c->nx;
         n[1] = c->ny;
         n[2] = c->nz;
-    } else if (T.cls == TC_SMALL || T.cls == TC_LARGE) {
+    } else if (tile_is_finite(T.cls)) {
         nt = (long long)T.n[0] * T.n[1] * T.n[2];
         raw.resize(nt * rs);
         HIPCHK(c, hipMemcpy(raw.data(), (unsigned char*)c->d_codes + T.off * rs, nt * rs, hipMemcpyDeviceToHost));

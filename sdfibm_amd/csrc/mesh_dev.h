@@ -126,16 +126,16 @@ __device__ inline R sdf_fast(const SolidK& K, d3 p) {
         return sdf_local<R>(K, R(l.x), R(l.y), R(l.z));
 }
 
-// Volume fraction of cell (i,j,k) inside solid K (calcCellVolume / V), with the
-// 8 corner signed distances evaluated in R.
-template <class R, int MK>
-__device__ inline R fraction_of_cell(const MeshDev& m, const SolidK& K, int i, int j, int k) {
+// Volume fraction of cell (i,j,k) (calcCellVolume / V) from its 8 corner signed
+// distances; phi_at(di, dj, dk) returns the distance at vertex (i+di, j+dj, k+dk).
+template <class R, int MK, class PhiAt>
+__device__ inline R fraction_with_phi(const MeshDev& m, int i, int j, int k, PhiAt phi_at) {
     const d3 w0 = vertex_pos<MK>(m, i, j, k);
     Hex8<R> h;
     auto vtx = [&](int di, int dj, int dk, r3<R>& p, R& f) {
         const d3 w = vertex_pos<MK>(m, i + di, j + dj, k + dk);
         p = {R(w.x - w0.x), R(w.y - w0.y), R(w.z - w0.z)};
-        f = sdf_fast<R>(K, w);
+        f = phi_at(di, dj, dk, w);
     };
     vtx(0, 0, 0, h.p0, h.f0);
     vtx(1, 0, 0, h.p1, h.f1);
@@ -151,10 +151,18 @@ __device__ inline R fraction_of_cell(const MeshDev& m, const SolidK& K, int i, i
     return hex_fraction<R>(h, F, V, m.two_d != 0, R(0.0 - w0.z));
 }
 
+// Volume fraction of cell (i,j,k) inside solid K, corner distances evaluated here.
+template <class R, int MK>
+__device__ inline R fraction_of_cell(const MeshDev& m, const SolidK& K, int i, int j, int k) {
+    return fraction_with_phi<R, MK>(m, i, j, k, [&](int, int, int, d3 w) { return sdf_fast<R>(K, w); });
+}
+
 // ---------------------------------------------------------------------------
 // per-solid tiles
 // ---------------------------------------------------------------------------
-enum TileClass { TC_EMPTY = 0, TC_SMALL = 1, TC_LARGE = 2, TC_PLANE = 3, TC_TOOBIG = 4 };
+enum TileClass { TC_EMPTY = 0, TC_SMALL = 1, TC_LARGE = 2, TC_PLANE = 3, TC_TOOBIG = 4, TC_MEDIUM = 5 };
+
+__host__ __device__ inline bool tile_is_finite(int cls) { return cls == TC_SMALL || cls == TC_MEDIUM || cls == TC_LARGE; }
 
 struct TileInfo {
     int lo[3];
@@ -165,8 +173,9 @@ struct TileInfo {
 };
 
 constexpr int kMaxTileX = 63;      // cells along x per tile (vertex row fits one 64-bit word)
-constexpr int kRowsSmall = 1024;   // vertex rows (ny+1)(nz+1) of a small tile
-constexpr int kRowsLarge = 4096;   // vertex rows of a large tile
+constexpr int kRowsSmall = 400;    // vertex rows (ny+1)(nz+1) of a small tile
+constexpr int kRowsMedium = 1024;  // ... of a medium tile
+constexpr int kRowsLarge = 4096;   // ... of a large tile
 
 // first index in sorted a[0..n) with a[idx] >= v (n if none)
 __host__ __device__ inline int lower_bound_d(const double* a, int n, double v) {
@@ -335,7 +344,7 @@ __host__ __device__ inline TileInfo make_tile(const MeshDev& m, const SolidK& K)
     if (T.n[0] > kMaxTileX || vrows > kRowsLarge)
         T.cls = TC_TOOBIG;
     else
-        T.cls = vrows > kRowsSmall ? TC_LARGE : TC_SMALL;
+        T.cls = vrows > kRowsMedium ? TC_LARGE : (vrows > kRowsSmall ? TC_MEDIUM : TC_SMALL);
     return T;
 }
 
