@@ -43,7 +43,7 @@ using namespace sdfibm_dev;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxPerBrick = 256;  // solids per brick handled by k_fields
+constexpr int kMaxPerBrick = 128;  // solids per brick handled by k_fields
 constexpr int kMaxPlanes = 8;
 
 enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_PAIRCAP = 4, EF_BRICK = 8 };
@@ -102,10 +102,16 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
     if (i < n) p[i] = v;
 }
 
+// Work queues of the per-solid tile pass, one per tile class (small, medium,
+// large): qmeta[q] = number of solids queued, qmeta[4 + q] = next to take.
+__host__ __device__ inline int tile_queue(int cls) {
+    return cls == TC_SMALL ? 0 : (cls == TC_MEDIUM ? 1 : (cls == TC_LARGE ? 2 : -1));
+}
+
 template <int MK>
 __global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, SolidK* __restrict__ K,
                         TileInfo* __restrict__ T, long long* __restrict__ tile_cells, double* __restrict__ ft,
-                        int* __restrict__ err) {
+                        int* __restrict__ qmeta, int* __restrict__ qlist, int* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     const SolidK k = make_solid_k(in[s]);
@@ -116,17 +122,40 @@ __global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, 
     tile_cells[s] = tile_is_finite(t.cls) ? (long long)t.n[0] * t.n[1] * t.n[2] : 0;
     T[s] = t;
     for (int q = 0; q < 6; ++q) ft[6 * s + q] = 0.0;  // solids that touch no cell exert nothing
+    const int q = tile_queue(t.cls);
+    if (q >= 0) qlist[(long long)q * ns + atomicAdd(&qmeta[q], 1)] = s;
 }
 
-// Exclusive scan of n values (single block of 1024 threads).  out[n] = total.
+// Exclusive scan of n values in three passes: tile sums, a scan of the tile
+// sums, tile-local scans.  out[n] = total.  kScanTile values per workgroup.
+constexpr int kScanTile = 4096;
+
 template <class T>
-__global__ void k_scan(const T* __restrict__ in, long long* __restrict__ out, long long n) {
+__global__ void __launch_bounds__(256) k_scan_sums(const T* __restrict__ in, long long n, long long* __restrict__ sums) {
+    __shared__ long long part[256];
+    const long long base = (long long)blockIdx.x * kScanTile;
+    long long s = 0;
+    for (int t = threadIdx.x; t < kScanTile; t += 256) {
+        const long long i = base + t;
+        if (i < n) s += (long long)in[i];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = part[0];
+}
+
+// In-place exclusive scan of nb tile sums (one workgroup of 1024); sums[nb] = total.
+__global__ void __launch_bounds__(1024) k_scan_top(long long* __restrict__ sums, int nb) {
     __shared__ long long part[1024];
     const int t = threadIdx.x;
-    const long long chunk = (n + 1023) / 1024;
-    const long long b = t * chunk, e = b + chunk < n ? b + chunk : n;
+    const int chunk = (nb + 1023) / 1024;
+    const int b = t * chunk, e = min(b + chunk, nb);
     long long s = 0;
-    for (long long i = b; i < e; ++i) s += (long long)in[i];
+    for (int i = b; i < e; ++i) s += sums[i];
     part[t] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -136,12 +165,58 @@ __global__ void k_scan(const T* __restrict__ in, long long* __restrict__ out, lo
         __syncthreads();
     }
     long long run = t > 0 ? part[t - 1] : 0;
-    for (long long i = b; i < e; ++i) {
-        const long long x = (long long)in[i];
-        out[i] = run;
+    for (int i = b; i < e; ++i) {
+        const long long x = sums[i];
+        sums[i] = run;
         run += x;
     }
-    if (t == 1023) out[n] = part[1023];
+    if (t == 1023) sums[nb] = part[1023];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_scan_tiles(const T* __restrict__ in, long long n,
+                                                    const long long* __restrict__ sums, int nb,
+                                                    long long* __restrict__ out) {
+    constexpr int PER = kScanTile / 256;
+    __shared__ long long part[256];
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * PER;
+    long long v[PER];
+    long long s = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const long long i = base + q;
+        v[q] = i < n ? (long long)in[i] : 0;
+        s += v[q];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const long long x = (int)threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    long long run = sums[blockIdx.x] + (threadIdx.x > 0 ? part[threadIdx.x - 1] : 0);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const long long i = base + q;
+        if (i < n) out[i] = run;
+        run += v[q];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sums[nb];
+}
+
+template <class T>
+void scan(const T* in, long long n, long long* out, long long* tmp, hipStream_t st) {
+    const int nb = (int)((n + kScanTile - 1) / kScanTile);
+    if (nb == 0) {
+        k_scan_top<<<1, 1024, 0, st>>>(tmp, 0);
+        k_scan_tiles<T><<<1, 256, 0, st>>>(in, 0, tmp, 0, out);
+        return;
+    }
+    k_scan_sums<T><<<nb, 256, 0, st>>>(in, n, tmp);
+    k_scan_top<<<1, 1024, 0, st>>>(tmp, nb);
+    k_scan_tiles<T><<<nb, 256, 0, st>>>(in, n, tmp, nb, out);
 }
 
 __global__ void k_tile_offsets(TileInfo* __restrict__ T, const long long* __restrict__ off, int ns,
@@ -197,28 +272,35 @@ __device__ inline int select_bit(unsigned long long w, int t) {
     return __builtin_ctzll(w);
 }
 
-template <class R, int MK, int MAXR, int PHICAP, int CLS>
+// Persistent: the workgroups of one launch take the solids of queue Q in turn
+// until the queue is empty (every workgroup reaches the exit).
+template <class R, int MK, int MAXR, int PHICAP, int Q>
 __global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const SolidK* __restrict__ Ks,
-                                                  const TileInfo* __restrict__ Ts, const R* __restrict__ Ux,
+                                                  const TileInfo* __restrict__ Ts, int* __restrict__ qmeta,
+                                                  const int* __restrict__ qlist, const R* __restrict__ Ux,
                                                   const R* __restrict__ Uy, const R* __restrict__ Uz,
                                                   R* __restrict__ codes, long long code_cap, double dt,
                                                   double rho_f, double* __restrict__ ft) {
-    const int s = blockIdx.x;
-    if (s >= ns) return;
-    const TileInfo T = Ts[s];
-    if (T.cls != CLS) return;
-    if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) return;
-
     __shared__ unsigned long long vin[MAXR];
     __shared__ unsigned long long act[MAXR];
     __shared__ unsigned long long all8[MAXR];
     __shared__ unsigned long long reach[MAXR];
     __shared__ int rowpre[MAXR + 1];
     __shared__ double red[6 * 16];
-    __shared__ int sh_first, sh_changed;
+    __shared__ int sh_first, sh_changed, sh_item;
+    const int tid = threadIdx.x;
+    const int nq = qmeta[Q];
+    for (;;) {
+    __syncthreads();
+    if (tid == 0) sh_item = atomicAdd(&qmeta[4 + Q], 1);
+    __syncthreads();
+    const int item = sh_item;
+    if (item >= nq) break;
+    const int s = __builtin_amdgcn_readfirstlane(qlist[(long long)Q * ns + item]);
+    const TileInfo T = Ts[s];
+    if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) continue;
 
     const SolidK K = Ks[s];
-    const int tid = threadIdx.x;
     const int nx = T.n[0], ny = T.n[1], nz = T.n[2];
     const int VX = nx + 1, VY = ny + 1;
     const int VR = VY * (nz + 1), CR = ny * nz;
@@ -395,32 +477,37 @@ __global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const Solid
     block_reduce<6>(acc, red);
     if (tid == 0)
         for (int t = 0; t < 6; ++t) ft[6 * s + t] = acc[t] * rho_f;
+    }  // work queue
 }
 
+// Brick lists: one wave per solid, its lanes over the bricks its tile overlaps.
+// fill = 0 counts per brick; fill = 1 writes the solid into each brick's list.
 template <int BX, int BY, int BZ>
-__global__ void k_bin(int ns, MeshDev m, const TileInfo* __restrict__ T, int nbx, int nby, int* __restrict__ count,
+__global__ void k_bin(int ns, const TileInfo* __restrict__ T, int nbx, int nby, int* __restrict__ count,
                       const long long* __restrict__ boff, int* __restrict__ blist, long long pair_cap,
                       int* __restrict__ err, int fill) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
     if (s >= ns) return;
     const TileInfo t = T[s];
     if (!tile_is_finite(t.cls)) return;
     const int b0x = t.lo[0] / BX, b1x = (t.lo[0] + t.n[0] - 1) / BX;
     const int b0y = t.lo[1] / BY, b1y = (t.lo[1] + t.n[1] - 1) / BY;
     const int b0z = t.lo[2] / BZ, b1z = (t.lo[2] + t.n[2] - 1) / BZ;
-    for (int bz = b0z; bz <= b1z; ++bz)
-        for (int by = b0y; by <= b1y; ++by)
-            for (int bx = b0x; bx <= b1x; ++bx) {
-                const long long b = bx + (long long)nbx * (by + (long long)nby * bz);
-                const int slot = atomicAdd(&count[b], 1);
-                if (fill) {
-                    const long long at = boff[b] + slot;
-                    if (at < pair_cap)
-                        blist[at] = s;
-                    else
-                        atomicOr(err, EF_PAIRCAP);
-                }
-            }
+    const int ex = b1x - b0x + 1, ey = b1y - b0y + 1, ez = b1z - b0z + 1;
+    const int nb = ex * ey * ez;
+    for (int q = lane; q < nb; q += 64) {
+        const int bx = b0x + q % ex, by = b0y + (q / ex) % ey, bz = b0z + q / (ex * ey);
+        const long long b = bx + (long long)nbx * (by + (long long)nby * bz);
+        const int slot = atomicAdd(&count[b], 1);
+        if (fill) {
+            const long long at = boff[b] + slot;
+            if (at < pair_cap)
+                blist[at] = s;
+            else
+                atomicOr(err, EF_PAIRCAP);
+        }
+    }
 }
 
 // Brick culling for a plane (inside: local y < 0, local y = n . (p - c) with
@@ -569,6 +656,8 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     constexpr int NT = BX * BY * BZ;
     __shared__ int raw[kMaxPerBrick];
     __shared__ int lst[kMaxPerBrick];
+    __shared__ TileInfo sTile[kMaxPerBrick];
+    __shared__ double sK[kMaxPerBrick][9];  // centre, velocity, angular velocity
     __shared__ double red[6 * 16];
     __shared__ int pl_any[kMaxPlanes];
 
@@ -594,6 +683,13 @@ __global__ void __launch_bounds__(BX* BY* BZ)
         int rank = 0;
         for (int u = 0; u < cnt; ++u) rank += raw[u] < v;
         lst[rank] = v;
+        sTile[rank] = Ts[v];
+        const SolidK& K = Ks[v];
+        for (int d = 0; d < 3; ++d) {
+            sK[rank][d] = K.c[d];
+            sK[rank][3 + d] = K.v[d];
+            sK[rank][6 + d] = K.w[d];
+        }
     }
     __syncthreads();
 
@@ -614,7 +710,7 @@ __global__ void __launch_bounds__(BX* BY* BZ)
 #pragma unroll
         for (int t = 0; t < 6; ++t) pacc[p][t] = 0.0;
 
-    auto contribute = [&](const SolidK& K, double alpha, int ctype, int plane_slot) {
+    auto contribute = [&](const double* cvw, double alpha, int ctype, int plane_slot) {
         if (!have_u) {
             const long long c = cell_id(m, i, j, k);
             uf = {(double)Ux[c], (double)Uy[c], (double)Uz[c]};
@@ -622,8 +718,8 @@ __global__ void __launch_bounds__(BX* BY* BZ)
             V = cell_volume<MK>(m, i, j, k);
             have_u = true;
         }
-        const d3 r = {cc.x - K.c[0], cc.y - K.c[1], cc.z - K.c[2]};
-        const d3 us = d3{K.v[0], K.v[1], K.v[2]} + cross(d3{K.w[0], K.w[1], K.w[2]}, r);
+        const d3 r = {cc.x - cvw[0], cc.y - cvw[1], cc.z - cvw[2]};
+        const d3 us = d3{cvw[3], cvw[4], cvw[5]} + cross(d3{cvw[6], cvw[7], cvw[8]}, r);
         const d3 f = alpha * (uf - us);
         sA += alpha;
         sT += alpha;
@@ -645,23 +741,41 @@ __global__ void __launch_bounds__(BX* BY* BZ)
 
     int ia = 0, ip = 0;
     if (!HASP) nplanes = 0;
+    // codes of the next kBatch listed solids, loaded together (independent loads)
+    constexpr int kBatch = 8;
+    R cb[kBatch];
+    int cb0 = -kBatch;
     while (ia < cnt || ip < nplanes) {
-        const int sa = ia < cnt ? lst[ia] : 0x7fffffff;
+        const int sa = __builtin_amdgcn_readfirstlane(ia < cnt ? lst[ia] : 0x7fffffff);
         const int sp = (HASP && ip < nplanes) ? planes[ip] : 0x7fffffff;
         if (sa < sp) {
+            if (ia >= cb0 + kBatch) {
+                cb0 = ia;
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q) {
+                    R code = code_none<R>();
+                    const int a = ia + q;
+                    if (valid && a < cnt) {
+                        const TileInfo& T = sTile[a];
+                        const int li = i - T.lo[0], lj = j - T.lo[1], lk = k - T.lo[2];
+                        if (li >= 0 && li < T.n[0] && lj >= 0 && lj < T.n[1] && lk >= 0 && lk < T.n[2] &&
+                            T.off + (long long)T.n[0] * T.n[1] * T.n[2] <= code_cap)
+                            code = codes[T.off + (long long)li + (long long)T.n[0] * (lj + (long long)T.n[1] * lk)];
+                    }
+                    cb[q] = code;
+                }
+            }
+            R code = cb[0];
+#pragma unroll
+            for (int q = 1; q < kBatch; ++q)
+                if (ia - cb0 == q) code = cb[q];
+            const double* cvw = sK[ia];
             ++ia;
-            if (!valid) continue;
-            const TileInfo T = Ts[sa];
-            if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) continue;  // skipped by k_solid
-            const int li = i - T.lo[0], lj = j - T.lo[1], lk = k - T.lo[2];
-            if (li < 0 || li >= T.n[0] || lj < 0 || lj >= T.n[1] || lk < 0 || lk >= T.n[2]) continue;
-            const R code = codes[T.off + (long long)li + (long long)T.n[0] * (lj + (long long)T.n[1] * lk)];
             if (code != code) continue;  // NaN: not a candidate
-            const SolidK K = Ks[sa];
             if (isinf(code))
-                contribute(K, 1.0, sa + 4, -1);
+                contribute(cvw, 1.0, sa + 4, -1);
             else
-                contribute(K, (double)fabs(code), signbit(code) ? 3 : 2, -1);
+                contribute(cvw, (double)fabs(code), signbit(code) ? 3 : 2, -1);
         } else if constexpr (HASP) {
             const int slot = ip;
             ++ip;
@@ -682,12 +796,13 @@ __global__ void __launch_bounds__(BX* BY* BZ)
             }
             if (nin == 0) continue;
             pl_any[slot] = 1;
+            const double cvw[9] = {K.c[0], K.c[1], K.c[2], K.v[0], K.v[1], K.v[2], K.w[0], K.w[1], K.w[2]};
             if (nin == 8) {
-                contribute(K, 1.0, sp + 4, slot);
+                contribute(cvw, 1.0, sp + 4, slot);
             } else {
                 const R alpha = fraction_of_cell<R, MK>(m, K, i, j, k);
                 const bool cin = inside_world(K, cell_centre<MK>(m, i, j, k));
-                contribute(K, (double)alpha, cin ? 2 : 3, slot);
+                contribute(cvw, (double)alpha, cin ? 2 : 3, slot);
             }
         }
     }
@@ -830,6 +945,7 @@ struct sdfibm_ctx {
     int nx = 0, ny = 0, nz = 0;
     long long N = 0;
     double rho_f = 1.0;
+    int n_cu = 256;
     MeshDev m{};   // device pointers
     MeshDev hm{};  // host pointers (tile estimates)
     std::vector<double> h_xs, h_ys, h_zs;
@@ -846,6 +962,9 @@ struct sdfibm_ctx {
     long long nbricks = 0;
     int* bcount = nullptr;
     long long* boff = nullptr;
+    long long* scan_tmp = nullptr;  // tile sums of the scans
+    int* qmeta = nullptr;           // tile-class work queues: counts [0..2], heads [4..6]
+    int* qlist = nullptr;
     int* err = nullptr;
     // solids
     long long ns = 0;
@@ -973,41 +1092,45 @@ int launch_step(sdfibm_ctx* c, double dt) {
     hipStream_t st = c->stream;
     (void)hipEventRecord(c->ev[0], st);
     k_fill_i32<<<1, 64, 0, st>>>(c->err, 1, 0);
+    k_fill_i32<<<1, 64, 0, st>>>(c->qmeta, 8, 0);
     if (ns > 0) {
         k_setup<MK><<<grid_for(ns, 128), 128, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_cells,
-                                                       c->d_ft, c->err);
-        k_scan<long long><<<1, 1024, 0, st>>>(c->d_tile_cells, c->d_tile_off, ns);
+                                                       c->d_ft, c->qmeta, c->qlist, c->err);
+        scan<long long>(c->d_tile_cells, ns, c->d_tile_off, c->scan_tmp, st);
         k_tile_offsets<<<grid_for(ns, 128), 128, 0, st>>>(c->d_T, c->d_tile_off, ns, c->code_cap, c->err);
     }
     k_fill_i32<<<grid_for(c->nbricks, 256), 256, 0, st>>>(c->bcount, c->nbricks, 0);
     if (ns > 0) {
+        const unsigned gb = grid_for(64LL * ns, 256);
         for (int pass = 0; pass < 2; ++pass) {
             if (c->nz == 1)
-                k_bin<kBX2, kBY2, kBZ2><<<grid_for(ns, 128), 128, 0, st>>>(
-                    ns, c->m, c->d_T, c->nbx, c->nby, c->bcount, c->boff, c->d_blist, c->pair_cap, c->err, pass);
+                k_bin<kBX2, kBY2, kBZ2><<<gb, 256, 0, st>>>(ns, c->d_T, c->nbx, c->nby, c->bcount, c->boff,
+                                                            c->d_blist, c->pair_cap, c->err, pass);
             else
-                k_bin<kBX3, kBY3, kBZ3><<<grid_for(ns, 128), 128, 0, st>>>(
-                    ns, c->m, c->d_T, c->nbx, c->nby, c->bcount, c->boff, c->d_blist, c->pair_cap, c->err, pass);
+                k_bin<kBX3, kBY3, kBZ3><<<gb, 256, 0, st>>>(ns, c->d_T, c->nbx, c->nby, c->bcount, c->boff,
+                                                            c->d_blist, c->pair_cap, c->err, pass);
             if (pass == 0) {
-                k_scan<int><<<1, 1024, 0, st>>>(c->bcount, c->boff, c->nbricks);
+                scan<int>(c->bcount, c->nbricks, c->boff, c->scan_tmp, st);
                 k_fill_i32<<<grid_for(c->nbricks, 256), 256, 0, st>>>(c->bcount, c->nbricks, 0);
             }
         }
     } else {
-        k_scan<int><<<1, 1024, 0, st>>>(c->bcount, c->boff, c->nbricks);
+        scan<int>(c->bcount, c->nbricks, c->boff, c->scan_tmp, st);
     }
     (void)hipEventRecord(c->ev[1], st);
     R* codes = (R*)c->d_codes;
     if (ns > 0) {
-        k_solid<R, MK, kRowsSmall, 2048, TC_SMALL><<<ns, kBlock, 0, st>>>(
-            c->m, ns, c->d_K, c->d_T, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap,
-            dt, c->rho_f, c->d_ft);
-        k_solid<R, MK, kRowsMedium, 4096, TC_MEDIUM><<<ns, kBlock, 0, st>>>(
-            c->m, ns, c->d_K, c->d_T, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap,
-            dt, c->rho_f, c->d_ft);
-        k_solid<R, MK, kRowsLarge, 0, TC_LARGE><<<ns, kBlock, 0, st>>>(
-            c->m, ns, c->d_K, c->d_T, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap,
-            dt, c->rho_f, c->d_ft);
+        // persistent launches: a few workgroups per CU take solids from the class queues
+        const unsigned cus = (unsigned)c->n_cu;
+        k_solid<R, MK, kRowsSmall, 2048, 0><<<cus * 6, kBlock, 0, st>>>(
+            c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],
+            codes, c->code_cap, dt, c->rho_f, c->d_ft);
+        k_solid<R, MK, kRowsMedium, 4096, 1><<<cus * 2, kBlock, 0, st>>>(
+            c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],
+            codes, c->code_cap, dt, c->rho_f, c->d_ft);
+        k_solid<R, MK, kRowsLarge, 0, 2><<<cus, kBlock, 0, st>>>(
+            c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],
+            codes, c->code_cap, dt, c->rho_f, c->d_ft);
     }
     const int np = (int)c->h_planes.size();
     for (int p = 0; p < np; ++p) {
@@ -1237,7 +1360,7 @@ std::string err_text(int flags) {
     if (flags & EF_TILE) s += "a solid's cell tile exceeds 63 cells along x or 4096 vertex rows; ";
     if (flags & EF_CODECAP) s += "tile code buffer too small; ";
     if (flags & EF_PAIRCAP) s += "brick pair buffer too small; ";
-    if (flags & EF_BRICK) s += "more than 256 solids overlap one brick; ";
+    if (flags & EF_BRICK) s += "more than 128 solids overlap one brick; ";
     return s;
 }
 
@@ -1331,6 +1454,14 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
     if ((rc = dalloc(c, (void**)&c->bcount, c->nbricks * 4))) return bail(rc);
     if ((rc = dalloc(c, (void**)&c->boff, (c->nbricks + 1) * 8))) return bail(rc);
     if ((rc = dalloc(c, (void**)&c->err, 64))) return bail(rc);
+    if ((rc = dalloc(c, (void**)&c->qmeta, 64))) return bail(rc);
+    if ((rc = dalloc(c, (void**)&c->scan_tmp, ((c->nbricks + kScanTile - 1) / kScanTile + 4096) * 8)))
+        return bail(rc);
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->n_cu = prop.multiProcessorCount;
+    }
     *out = c;
     return SDFIBM_OK;
 }
@@ -1401,6 +1532,8 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         if ((rc = dalloc(c, (void**)&c->d_tile_cells, cap * 8))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_tile_off, (cap + 1) * 8))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_ft, cap * 6 * 8))) return rc;
+        dfree(c, c->qlist);
+        if ((rc = dalloc(c, (void**)&c->qlist, cap * 3 * 4))) return rc;
         c->solid_cap = cap;
     }
     const long long want_codes = cells + cells / 20 + 1024;

@@ -177,29 +177,29 @@ constexpr int kRowsSmall = 400;    // vertex rows (ny+1)(nz+1) of a small tile
 constexpr int kRowsMedium = 1024;  // ... of a medium tile
 constexpr int kRowsLarge = 4096;   // ... of a large tile
 
-// first index in sorted a[0..n) with a[idx] >= v (n if none)
+// Searches in an increasing array a[0..n): start from a linear-interpolation
+// guess and step to the exact answer, which takes one or two probes on a
+// uniform axis (a binary search would need a chain of log2(n) dependent loads).
+__host__ __device__ inline int guess_index(const double* a, int n, double v) {
+    const double a0 = a[0], a1 = a[n - 1];
+    double g = (a1 > a0) ? (v - a0) / (a1 - a0) * (double)(n - 1) : 0.0;
+    if (!(g > 0.0)) g = 0.0;
+    if (g > (double)(n - 1)) g = (double)(n - 1);
+    return (int)g;
+}
+// first index with a[idx] >= v (n if none)
 __host__ __device__ inline int lower_bound_d(const double* a, int n, double v) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < v)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
+    int g = guess_index(a, n, v);
+    while (g > 0 && a[g - 1] >= v) --g;
+    while (g < n && a[g] < v) ++g;
+    return g;
 }
 // last index with a[idx] <= v (-1 if none)
 __host__ __device__ inline int last_le_d(const double* a, int n, double v) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] <= v)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo - 1;
+    int g = guess_index(a, n, v);
+    while (g < n && a[g] <= v) ++g;
+    while (g > 0 && a[g - 1] > v) --g;
+    return g - 1;
 }
 
 // Rotation matrix of q (local -> world).
@@ -265,6 +265,10 @@ __host__ __device__ inline bool world_bounds(const SolidK& K, double* wc, double
     } else {
         d3 lc, lh;
         if (!local_bounds(K, lc, lh)) return false;
+        // ellipsoids and ellipses: exact support function sqrt(sum (R_de l_e)^2);
+        // boxes and rectangles: sum |R_de| l_e (the rotated corners)
+        const bool quadric = K.shape == SDFIBM_SHAPE_ELLIPSOID || K.shape == SDFIBM_SHAPE_ELLIPSE ||
+                             K.shape == SDFIBM_SHAPE_SPHERE || K.shape == SDFIBM_SHAPE_CIRCLE;
         const double lcv[3] = {lc.x, lc.y, lc.z}, lhv[3] = {lh.x, lh.y, lh.z};
         for (int d = 0; d < 3; ++d) {
             const double w = K.c[d] + (Rm[d][0] * lcv[0] + Rm[d][1] * lcv[1] + Rm[d][2] * lcv[2]);
@@ -274,9 +278,10 @@ __host__ __device__ inline bool world_bounds(const SolidK& K, double* wc, double
                 if (lhv[f] > 1e300) {
                     if (rr > 1e-12) e = INF;
                 } else {
-                    e += rr * lhv[f];
+                    e += quadric ? (rr * lhv[f]) * (rr * lhv[f]) : rr * lhv[f];
                 }
             }
+            if (quadric && e < INF) e = sqrt(e);
             lo[d] = w - e;
             hi[d] = w + e;
         }
@@ -362,14 +367,11 @@ __host__ __device__ inline int nearest_cell(const MeshDev& m, const double* p) {
         // per axis the nearest centre is one of two neighbours; search that 2x2x2 block
         for (int d = 0; d < 3; ++d) {
             const double* xs = d == 0 ? m.xs : (d == 1 ? m.ys : m.zs);
-            int lo = 0, hi = nn[d] - 1;
-            while (lo < hi) {  // first centre >= p
-                const int mid = (lo + hi) >> 1;
-                if ((xs[mid] + xs[mid + 1]) * 0.5 < p[d])
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
+            // first cell whose centre is >= p (centres increase), by guess and step
+            int lo = guess_index(xs, nn[d] + 1, p[d]);
+            if (lo > nn[d] - 1) lo = nn[d] - 1;
+            while (lo > 0 && (xs[lo - 1] + xs[lo]) * 0.5 >= p[d]) --lo;
+            while (lo < nn[d] - 1 && (xs[lo] + xs[lo + 1]) * 0.5 < p[d]) ++lo;
             c0[d] = lo > 0 ? lo - 1 : 0;
         }
         r = 1;
