@@ -59,6 +59,19 @@ def touched_stats(mesh, Ct):
     return nc, int(v.sum())
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the plain-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        f = ln.split()
+        if len(f) >= 4 and f[0] == "traffic" and f[1] == workload and f[2] == kernel:
+            return float(f[3])
+    return None
+
+
 def cpu_baseline(scene, n_sample, budget_s):
     """Reference-cost CPU oracle on the first solids in index order until the sample or the
     time budget is used; returns cell-updates/s and the sample actually timed."""
@@ -149,7 +162,7 @@ def main():
     for _ in range(args.warmup):
         cloud.interact_device(0.0, sc.dt)
     torch.cuda.synchronize()
-    phases = np.zeros(4)
+    phases = np.zeros(6)
     t_start = time.perf_counter()
     for _ in range(args.steps):
         cloud.interact_device(0.0, sc.dt)
@@ -158,7 +171,7 @@ def main():
     t_total = time.perf_counter() - t_start
     ms_per_step = 1e3 * t_total / args.steps
     phases /= args.steps
-    log(f"[bench] {ms_per_step:.3f} ms/step; phases (setup+bin, tiles, fields, sum) ms = {phases}")
+    log(f"[bench] {ms_per_step:.3f} ms/step; phases (setup+bin, tiles, fields, sum, classify, cells) ms = {phases}")
 
     ft = cloud.interact(0.0, sc.dt)
     f = cloud.fields()
@@ -166,16 +179,21 @@ def main():
     nc, nv = touched_stats(sc.mesh, f["Ct"])
     value = n_pairs / (ms_per_step * 1e-3)
     rb = 4 if args.precision == "fp32" else 8
-    # algorithmic bytes per interact: write As, Fs(3), Ts (R each) and Ct (int32) for every cell;
-    # read U(3) for every candidate cell, and its centre (3) + volume + 8 vertex reads in the
-    # reference's mesh model (here: closed-form geometry of the rectilinear mesh)
+    # Algorithmic HBM bytes (each array element counted once): the field pass writes
+    # As, Fs(3), Ts (rb each) and Ct (int32) for every cell and reads U(3) of the touched
+    # cells; the cut-cell/forcing pass reads U(3) of the touched cells and the vertices (3)
+    # of the touched cells in the reference's mesh model; the whole step adds the cell
+    # centre (3) and volume of the touched cells.
     B_total = (5 * rb + 4) * N + (3 * rb + 3 * rb + rb) * nc + 3 * rb * nv
-    B_fields = (5 * rb + 4) * N + 3 * rb * nc
-    B_tiles = 3 * rb * nc + 3 * rb * nv
-    dom = int(np.argmax(phases[:3]))
-    names = ["setup_bin", "k_solid", "k_fields"]
-    per_kernel_bytes = {1: B_tiles, 2: B_fields, 0: 160 * sc.solids.size}
-    achieved = per_kernel_bytes[dom] / (phases[dom] * 1e-3)
+    kernels = {
+        "k_fields": {"ms": phases[2], "bytes": (5 * rb + 4) * N + 3 * rb * nc},
+        "k_cells": {"ms": phases[5], "bytes": 3 * rb * nc + 3 * rb * nv},
+        "k_classify": {"ms": phases[4], "bytes": None},
+        "setup_bin": {"ms": phases[0], "bytes": None},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"] if kernels[k]["bytes"] else -1.0)
+    achieved = kernels[dom]["bytes"] / (kernels[dom]["ms"] * 1e-3)
+    traffic = measured_traffic(dom, f"cloud3d_{args.n}_{args.solids}_{args.precision}")
     line = {
         "metric": METRIC,
         "value": value,
@@ -195,11 +213,13 @@ def main():
                    "inside_pairs": n_inside, "cut_pairs": n_cut, "touched_cells": nc,
                    "touched_vertices": nv, "solid_volume_fraction": float(sc.info["solid_volume_fraction"]),
                    "parallelism": "single GPU"},
-        "phases_ms": {"setup_bin": phases[0], "k_solid": phases[1], "k_fields": phases[2]},
+        "phases_ms": {"setup_bin": phases[0], "k_classify": phases[4], "k_cells": phases[5],
+                      "k_fields": phases[2]},
+        "mesh_cells_per_s": N / (ms_per_step * 1e-3),
         "hbm_frac_whole_step": B_total / (ms_per_step * 1e-3) / HBM_PEAK,
-        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "algorithmic_bytes": per_kernel_bytes[dom]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes": kernels[dom]["bytes"]},
     }
     if not args.no_cpu:
         rate, ns_t, pairs_t, el = cpu_baseline(sc, args.cpu_sample, args.cpu_budget)

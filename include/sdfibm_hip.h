@@ -157,8 +157,10 @@ int sdfibm_candidates(sdfibm_ctx* ctx, int64_t solid, int64_t* counts, int64_t* 
                       double* alpha, int64_t capacity);
 
 /* Timing of the phases of the last interact (milliseconds, HIP events on the
- * context's stream): [0] setup+binning, [1] per-solid tiles, [2] field pass,
- * [3] total.  Returns the number of values written (<= n). */
+ * context's stream): [0] setup+binning, [1] per-solid tiles (classification,
+ * cut cells and forcing, plane labelling), [2] field pass, [3] total,
+ * [4] tile classification alone, [5] cut cells + forcing + plane labelling.
+ * Returns the number of values written (<= n, at most 6). */
 int sdfibm_last_timing(sdfibm_ctx* ctx, double* ms, int n);
 
 /* Launch the work of interact on a caller-provided hipStream_t (NULL = the

@@ -146,8 +146,10 @@ class SolidCloud:
         return counts, cells, alpha
 
     def timing(self) -> np.ndarray:
-        ms = np.zeros(4)
-        _lib.load().sdfibm_last_timing(self._h, _p(ms), 4)
+        """Phase times of the last interact in ms: setup+binning, tile pass, field pass, total,
+        tile classification, cut cells + forcing (+ plane labelling)."""
+        ms = np.zeros(6)
+        _lib.load().sdfibm_last_timing(self._h, _p(ms), 6)
         return ms
 
     def device_pointers(self) -> dict:

@@ -168,4 +168,87 @@ __device__ inline R hex_fraction(const Hex8<R>& h, const HexFaces<R>& F, R V, bo
     return vol / V;
 }
 
+// ---------------------------------------------------------------------------
+// Axis-aligned box cells (rectilinear meshes): hex_fraction in coordinates
+// normalised by the cell size.  Every face is a unit square whose apex lies in
+// its plane, so the triangle (O, A, apex) on an edge has area dist/2, dist = the
+// apex's distance from the edge's line, and |(apex - fc).Sf| / V is the cell
+// apex's distance from the face's plane.  Same apex rules, face listings and
+// sums as above; no square roots, and each of the 12 edge line fractions is
+// formed once (calcLineFraction is symmetric in its two arguments).
+// ---------------------------------------------------------------------------
+template <class R>
+struct Phi8 {
+    R f0, f1, f2, f3, f4, f5, f6, f7;
+};
+template <int I, class R>
+__device__ inline R pf(const Phi8<R>& h) {
+    if constexpr (I == 0) return h.f0;
+    else if constexpr (I == 1) return h.f1;
+    else if constexpr (I == 2) return h.f2;
+    else if constexpr (I == 3) return h.f3;
+    else if constexpr (I == 4) return h.f4;
+    else if constexpr (I == 5) return h.f5;
+    else if constexpr (I == 6) return h.f6;
+    else return h.f7;
+}
+
+// Area fraction of the unit face listed (V0 V1 V2 V3); AU, AV are its in-plane
+// axes (0 x, 1 y, 2 z; vertex bit a of the cellPoints index is the coordinate
+// along axis a); lu0, lu1, lv0, lv1 are the line fractions of its edges on the
+// lines u = 0, u = 1, v = 0, v = 1.
+template <class R, int V0, int V1, int V2, int V3, int AU, int AV>
+__device__ inline R unit_face_fraction(const Phi8<R>& h, R lu0, R lu1, R lv0, R lv1) {
+    const R p0 = pf<V0>(h), p1 = pf<V1>(h), p2 = pf<V2>(h), p3 = pf<V3>(h);
+    const int ss = (p0 > 0 ? 1 : -1) + (p1 > 0 ? 1 : -1) + (p2 > 0 ? 1 : -1) + (p3 > 0 ? 1 : -1);
+    if (ss == 4) return R(0);
+    if (ss == -4) return R(1);
+    const bool c1 = p0 * p1 <= 0, c2 = p0 * p2 <= 0;
+    const R pB = c1 ? p1 : (c2 ? p2 : p3);
+    constexpr R u0 = R((V0 >> AU) & 1), v0 = R((V0 >> AV) & 1);
+    const R bu = c1 ? R((V1 >> AU) & 1) : (c2 ? R((V2 >> AU) & 1) : R((V3 >> AU) & 1));
+    const R bv = c1 ? R((V1 >> AV) & 1) : (c2 ? R((V2 >> AV) & 1) : R((V3 >> AV) & 1));
+    const R s = fabs(p0) / (R(kSMALL) + fabs(p0) + fabs(pB));
+    const R au = u0 - s * (u0 - bu), av = v0 - s * (v0 - bv);
+    return R(0.5) * (lv0 * av + lu1 * (R(1) - au) + lv1 * (R(1) - av) + lu0 * au);
+}
+
+// Volume fraction of an axis-aligned box cell; bx, by, bz: the cell's -x, -y,
+// -z faces lie on the mesh boundary; two_d: the apex z is az_2d (world z = 0 in
+// the cell's normalised z coordinate).
+template <class R>
+__device__ inline R box_fraction(const Phi8<R>& h, bool bx, bool by, bool bz, bool two_d, R az_2d) {
+    const R x01 = line_fraction<R>(h.f0, h.f1), x23 = line_fraction<R>(h.f2, h.f3),
+            x45 = line_fraction<R>(h.f4, h.f5), x67 = line_fraction<R>(h.f6, h.f7);
+    const R y02 = line_fraction<R>(h.f0, h.f2), y13 = line_fraction<R>(h.f1, h.f3),
+            y46 = line_fraction<R>(h.f4, h.f6), y57 = line_fraction<R>(h.f5, h.f7);
+    const R z04 = line_fraction<R>(h.f0, h.f4), z15 = line_fraction<R>(h.f1, h.f5),
+            z26 = line_fraction<R>(h.f2, h.f6), z37 = line_fraction<R>(h.f3, h.f7);
+    const R exm = bx ? unit_face_fraction<R, 0, 4, 6, 2, 1, 2>(h, z04, z26, y02, y46)
+                     : unit_face_fraction<R, 0, 2, 6, 4, 1, 2>(h, z04, z26, y02, y46);
+    const R exp_ = unit_face_fraction<R, 1, 3, 7, 5, 1, 2>(h, z15, z37, y13, y57);
+    const R eym = by ? unit_face_fraction<R, 0, 1, 5, 4, 0, 2>(h, z04, z15, x01, x45)
+                     : unit_face_fraction<R, 0, 4, 5, 1, 0, 2>(h, z04, z15, x01, x45);
+    const R eyp = unit_face_fraction<R, 2, 6, 7, 3, 0, 2>(h, z26, z37, x23, x67);
+    const R ezm = bz ? unit_face_fraction<R, 0, 2, 3, 1, 0, 1>(h, y02, y13, x01, x23)
+                     : unit_face_fraction<R, 0, 1, 3, 2, 0, 1>(h, y02, y13, x01, x23);
+    const R ezp = unit_face_fraction<R, 4, 5, 7, 6, 0, 1>(h, y46, y57, x45, x67);
+    // cell apex: A = vertex 0 (the origin), B = first later vertex with a sign change, else 7
+    const R f0 = h.f0;
+    int b = 7;
+    R pB = h.f7;
+    if (f0 * h.f6 <= 0) { b = 6; pB = h.f6; }
+    if (f0 * h.f5 <= 0) { b = 5; pB = h.f5; }
+    if (f0 * h.f4 <= 0) { b = 4; pB = h.f4; }
+    if (f0 * h.f3 <= 0) { b = 3; pB = h.f3; }
+    if (f0 * h.f2 <= 0) { b = 2; pB = h.f2; }
+    if (f0 * h.f1 <= 0) { b = 1; pB = h.f1; }
+    const R s = fabs(f0) / (R(kSMALL) + fabs(f0) + fabs(pB));
+    const R ax = (b & 1) ? s : R(0), ay = (b & 2) ? s : R(0);
+    const R az = two_d ? az_2d : ((b & 4) ? s : R(0));
+    const R vol = exm * fabs(ax) + exp_ * fabs(ax - R(1)) + eym * fabs(ay) + eyp * fabs(ay - R(1)) +
+                  ezm * fabs(az) + ezp * fabs(az - R(1));
+    return (R(1) / R(3)) * vol;
+}
+
 }  // namespace sdfibm_dev

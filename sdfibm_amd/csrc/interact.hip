@@ -7,13 +7,16 @@
 //                  (index box holding every cell with a vertex strictly inside),
 //                  the seed cell (nearest cell centre, solidcloud.cpp:363).
 //   k_scan         tile sizes -> offsets into the per-tile code buffer.
-//   k_solid        one workgroup per finite solid: inside-vertex bits of the
+//   k_classify     one workgroup per finite solid: inside-vertex bits of the
 //                  tile in LDS (64-bit rows along x), cells with >= 1 inside
 //                  vertex, the connected component grown from the seed (or the
 //                  first such cell when the seed has none; cellenumerator.cpp:
-//                  6-78) by bit-parallel flood fill, classification, volume
-//                  fractions of the cut cells, and the solid's force and torque
-//                  (FP64 block reduction).  Writes one code per tile cell.
+//                  6-78) by bit-parallel flood fill, ALL_INSIDE / cut
+//                  classification, compacted candidate lists.
+//   k_cells        one workgroup per finite solid over its candidate list:
+//                  volume fractions of the cut cells, the centre test, the
+//                  forcing and the solid's force and torque (FP64 block
+//                  reduction).  Together the two write one code per tile cell.
 //   k_bin_*        per-brick lists of the solids whose tiles overlap the brick.
 //   k_fields       one workgroup per brick of cells: every cell of the mesh is
 //                  written exactly once -- As (clipped to 1), Fs, Ts, Ct summed
@@ -86,6 +89,21 @@ __device__ inline R code_none() {
     return R(__builtin_nan(""));
 }
 
+// q = a / b, r = a % b for 0 <= a < 2^22 and 1 <= b < 2^12, rb = 1.0f / b: the
+// truncated float product is within one of the quotient (relative error below
+// 2^-23 on a value below 2^22); one step corrects it.
+__device__ inline void divmod_small(int a, int b, float rb, int& q, int& r) {
+    q = (int)((float)a * rb);
+    r = a - q * b;
+    if (r < 0) {
+        --q;
+        r += b;
+    } else if (r >= b) {
+        ++q;
+        r -= b;
+    }
+}
+
 // Run fill inside a row: bits of `a` reachable from `s` (s subset of a) along x.
 __device__ inline unsigned long long run_fill(unsigned long long a, unsigned long long s) {
     const unsigned long long up = (((a + s) ^ a) & a) | s;
@@ -103,7 +121,7 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
 }
 
 // Work queues of the per-solid tile pass, one per tile class (small, medium,
-// large): qmeta[q] = number of solids queued, qmeta[4 + q] = next to take.
+// large): qmeta[q] = number of solids queued, qlist[q * ns + t] = the t-th of them.
 __host__ __device__ inline int tile_queue(int cls) {
     return cls == TC_SMALL ? 0 : (cls == TC_MEDIUM ? 1 : (cls == TC_LARGE ? 2 : -1));
 }
@@ -122,8 +140,19 @@ __global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, 
     tile_cells[s] = tile_is_finite(t.cls) ? (long long)t.n[0] * t.n[1] * t.n[2] : 0;
     T[s] = t;
     for (int q = 0; q < 6; ++q) ft[6 * s + q] = 0.0;  // solids that touch no cell exert nothing
+    // queue the solid: one atomic per wave and queue (the lanes of a wave are
+    // consecutive solids, queued in index order)
     const int q = tile_queue(t.cls);
-    if (q >= 0) qlist[(long long)q * ns + atomicAdd(&qmeta[q], 1)] = s;
+    const int lane = threadIdx.x & 63;
+    for (int qq = 0; qq < 3; ++qq) {
+        const unsigned long long b = __ballot(q == qq);
+        if (!b) continue;
+        const int leader = __builtin_ctzll(b);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&qmeta[qq], __popcll(b));
+        base = __shfl(base, leader, 64);
+        if (q == qq) qlist[(long long)qq * ns + base + __popcll(b & ((1ull << lane) - 1ull))] = s;
+    }
 }
 
 // Exclusive scan of n values in three passes: tile sums, a scan of the tile
@@ -227,75 +256,95 @@ __global__ void k_tile_offsets(TileInfo* __restrict__ T, const long long* __rest
     T[s].off = off[s];
 }
 
-// Per-solid tile pass.  One workgroup per solid of class CLS.
-// Exclusive prefix over n rows of count(r) into pre[0..n] (block-wide, kBlock threads).
-template <class F>
-__device__ inline void block_row_scan(int n, F count, int* pre, int* part) {
-    const int tid = threadIdx.x;
-    const int chunk = (n + kBlock - 1) / kBlock;
-    const int r0 = tid * chunk, r1 = min(r0 + chunk, n);
-    int sum = 0;
-    for (int r = r0; r < r1; ++r) sum += count(r);
-    part[tid] = sum;
+// ---------------------------------------------------------------------------
+// per-solid tile pass, in two kernels
+//
+//   k_classify  (workgroup per solid, LDS rows, few registers): inside bit of
+//               every tile vertex, cells with >= 1 inside vertex, the seed's
+//               connected component (flood fill), codes of the non-candidate and
+//               ALL_INSIDE cells, and the compacted candidate list of the tile:
+//               ALL_INSIDE cells first, then the cut cells, each part ascending.
+//   k_cells     (workgroup per solid, no LDS rows, the arithmetic): volume
+//               fraction and centre test of every cut cell, the forcing of every
+//               candidate and the solid's FP64 force/torque.
+//
+// Both take their solids from work queues (persistent workgroups).
+// ---------------------------------------------------------------------------
+
+// Block-wide exclusive scan of two per-thread counts (kBlock threads): wave
+// prefix sums by shuffles, then the wave totals through LDS.
+__device__ inline void block_exscan2(int a, int b, int& pa, int& pb, int& ta, int& tb, int* lds /* [8] */) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int ia = a, ib = b;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int ya = __shfl_up(ia, off, 64), yb = __shfl_up(ib, off, 64);
+        if (lane >= off) {
+            ia += ya;
+            ib += yb;
+        }
+    }
+    if (lane == 63) {
+        lds[wv] = ia;
+        lds[4 + wv] = ib;
+    }
     __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-        const int v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    int ba = 0, bb = 0, sa = 0, sb = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const int x = lds[w], y = lds[4 + w];
+        if (w < wv) {
+            ba += x;
+            bb += y;
+        }
+        sa += x;
+        sb += y;
     }
-    int run = tid > 0 ? part[tid - 1] : 0;
-    for (int r = r0; r < r1; ++r) {
-        pre[r] = run;
-        run += count(r);
-    }
-    if (tid == kBlock - 1) pre[n] = part[kBlock - 1];
+    pa = ba + ia - a;
+    pb = bb + ib - b;
+    ta = sa;
+    tb = sb;
     __syncthreads();
 }
 
-// Position of row containing the q-th set bit: pre[r] <= q < pre[r+1].
-__device__ inline int row_of(const int* pre, int n, int q) {
-    int lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (pre[mid] <= q)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
+// Solid of item t of the tile pass, largest tile class first.
+__device__ inline int queue_item(const int* qmeta, const int* qlist, int ns, int t) {
+    const int n2 = qmeta[2], n1 = qmeta[1];
+    if (t < n2) return qlist[2LL * ns + t];
+    if (t < n2 + n1) return qlist[(long long)ns + (t - n2)];
+    return qlist[t - n2 - n1];
 }
 
-// Index of the t-th set bit of w.
-__device__ inline int select_bit(unsigned long long w, int t) {
-    for (; t > 0; --t) w &= w - 1;
-    return __builtin_ctzll(w);
-}
+// Candidate entry: the cell's tile-local (i, j, k) packed as i | j << 6 | k << 19
+// (i < 64, j and k < 2^13: a tile has at most 4096 vertex rows); the ALL_INSIDE
+// entries come first in the list, the count of each part is in ncand[s] =
+// {inside, cut}.
+constexpr int kStageYZ = 256;  // tile coordinates staged in LDS up to this many per axis
 
-// Persistent: the workgroups of one launch take the solids of queue Q in turn
-// until the queue is empty (every workgroup reaches the exit).
-template <class R, int MK, int MAXR, int PHICAP, int Q>
-__global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const SolidK* __restrict__ Ks,
-                                                  const TileInfo* __restrict__ Ts, int* __restrict__ qmeta,
-                                                  const int* __restrict__ qlist, const R* __restrict__ Ux,
-                                                  const R* __restrict__ Uy, const R* __restrict__ Uz,
-                                                  R* __restrict__ codes, long long code_cap, double dt,
-                                                  double rho_f, double* __restrict__ ft) {
+// Workgroup b takes the solids b, b + G, b + 2G, ... of queue Q (G = grid size;
+// for the small class G is the number of solids, one each, and the hardware
+// dispatcher balances the uneven tiles).  MAXR bounds the vertex rows
+// (ny+1)(nz+1) of the tiles of the queue.  No shared work counter: that would
+// cost a device-scope atomic (slow across the XCDs) per solid.
+template <class R, int MK, int MAXR, int Q>
+__global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const SolidK* __restrict__ Ks,
+                                                     const TileInfo* __restrict__ Ts, int* __restrict__ qmeta,
+                                                     const int* __restrict__ qlist, R* __restrict__ codes,
+                                                     int* __restrict__ cand, long long code_cap,
+                                                     int2* __restrict__ ncand) {
     __shared__ unsigned long long vin[MAXR];
     __shared__ unsigned long long act[MAXR];
     __shared__ unsigned long long all8[MAXR];
     __shared__ unsigned long long reach[MAXR];
-    __shared__ int rowpre[MAXR + 1];
-    __shared__ double red[6 * 16];
-    __shared__ int sh_first, sh_changed, sh_item;
-    const int tid = threadIdx.x;
+    constexpr int NSX = MK == MK_RECT ? kMaxTileX + 2 : 1;
+    constexpr int NSYZ = MK == MK_RECT ? (MAXR / 2 + 1 < kStageYZ ? MAXR / 2 + 1 : kStageYZ) : 1;
+    __shared__ double sx[NSX], sy[NSYZ], sz[NSYZ];
+    __shared__ int scan_lds[8];
+    __shared__ int sh_first, sh_changed;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nq = qmeta[Q];
-    for (;;) {
-    __syncthreads();
-    if (tid == 0) sh_item = atomicAdd(&qmeta[4 + Q], 1);
-    __syncthreads();
-    const int item = sh_item;
-    if (item >= nq) break;
+    for (int item = blockIdx.x; item < nq; item += gridDim.x) {
+    __syncthreads();  // LDS of the previous solid is free
     const int s = __builtin_amdgcn_readfirstlane(qlist[(long long)Q * ns + item]);
     const TileInfo T = Ts[s];
     if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) continue;
@@ -305,32 +354,65 @@ __global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const Solid
     const int VX = nx + 1, VY = ny + 1;
     const int VR = VY * (nz + 1), CR = ny * nz;
     const unsigned long long cmask = (nx >= 64) ? ~0ull : ((1ull << nx) - 1ull);
+    const float rnx = 1.0f / (float)nx, rny = 1.0f / (float)ny, rVY = 1.0f / (float)VY;
 
-    for (int r = tid; r < VR; r += kBlock) vin[r] = 0ull;
+    // tile coordinates of a rectilinear mesh in LDS
+    const bool staged = MK == MK_RECT && VY <= NSYZ && nz + 1 <= NSYZ;
+    if (staged) {
+        for (int t = tid; t < VX; t += kBlock) sx[t] = m.xs[T.lo[0] + t];
+        for (int t = tid; t < VY; t += kBlock) sy[t] = m.ys[T.lo[1] + t];
+        for (int t = tid; t <= nz; t += kBlock) sz[t] = m.zs[T.lo[2] + t];
+    }
     if (tid == 0) sh_first = 0x7fffffff;
     __syncthreads();
 
-    // 1. inside test at every tile vertex (FP64, reference operation order)
-    const long long NV = (long long)VX * VR;
-    for (long long v = tid; v < NV; v += kBlock) {
-        const int i = (int)(v % VX);
-        const int r = (int)(v / VX);
-        const int j = r % VY, k = r / VY;
-        const d3 p = vertex_pos<MK>(m, T.lo[0] + i, T.lo[1] + j, T.lo[2] + k);
-        if (inside_world(K, p)) atomicOr(&vin[r], 1ull << i);
+    // 1. inside test at every tile vertex (FP64, reference operation order).  A
+    //    wave covers g = 64 / VX vertex rows per pass; one ballot gives their bits.
+    {
+        const int g = 64 / VX;
+        const int rr = lane / VX, vi = lane - rr * VX;
+        const unsigned long long vmask = VX >= 64 ? ~0ull : ((1ull << VX) - 1ull);
+        for (int base = wv * g; base < VR; base += (kBlock / 64) * g) {
+            const int r = base + rr;
+            const bool mine = rr < g && r < VR;
+            bool in = false;
+            if (mine) {
+                int vj, vk;
+                divmod_small(r, VY, rVY, vk, vj);
+                const d3 p = staged ? d3{sx[vi], sy[vj], sz[vk]}
+                                    : vertex_pos<MK>(m, T.lo[0] + vi, T.lo[1] + vj, T.lo[2] + vk);
+                in = inside_world(K, p);
+            }
+            const unsigned long long bal = __ballot(in);
+            if (mine && vi == 0) vin[r] = (bal >> (rr * VX)) & vmask;
+        }
     }
     __syncthreads();
 
-    // 2. per cell row: >= 1 inside vertex / all 8 inside
-    for (int r = tid; r < CR; r += kBlock) {
-        const int j = r % ny, k = r / ny;
-        const unsigned long long a = vin[k * VY + j], b = vin[k * VY + j + 1], c = vin[(k + 1) * VY + j],
-                                 d = vin[(k + 1) * VY + j + 1];
-        const unsigned long long o = a | b | c | d, n = a & b & c & d;
-        act[r] = (o | (o >> 1)) & cmask;
-        all8[r] = (n & (n >> 1)) & cmask;
-        reach[r] = 0ull;
-        if (act[r]) atomicMin(&sh_first, r * 64 + __builtin_ctzll(act[r]));
+    // 2. per cell row: >= 1 inside vertex / all 8 inside; the first such cell
+    //    (one LDS atomic per wave)
+    {
+        bool found = false;
+        for (int r0 = wv * 64; r0 < CR; r0 += kBlock) {
+            const int r = r0 + lane;
+            unsigned long long av = 0ull;
+            if (r < CR) {
+                int j, k;
+                divmod_small(r, ny, rny, k, j);
+                const unsigned long long a = vin[k * VY + j], b = vin[k * VY + j + 1], c = vin[(k + 1) * VY + j],
+                                         d = vin[(k + 1) * VY + j + 1];
+                const unsigned long long o = a | b | c | d, n = a & b & c & d;
+                av = (o | (o >> 1)) & cmask;
+                act[r] = av;
+                all8[r] = (n & (n >> 1)) & cmask;
+                reach[r] = 0ull;
+            }
+            const unsigned long long b = __ballot(av != 0ull);
+            if (b && !found) {
+                found = true;
+                if (lane == __builtin_ctzll(b)) atomicMin(&sh_first, r * 64 + __builtin_ctzll(av));
+            }
+        }
     }
     __syncthreads();
 
@@ -365,7 +447,8 @@ __global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const Solid
         __syncthreads();
         int ch = 0;
         for (int r = tid; r < CR; r += kBlock) {
-            const int j = r % ny, k = r / ny;
+            int j, k;
+            divmod_small(r, ny, rny, k, j);
             unsigned long long nb = reach[r];
             if (j > 0) nb |= reach[r - 1];
             if (j < ny - 1) nb |= reach[r + 1];
@@ -382,102 +465,110 @@ __global__ void __launch_bounds__(kBlock) k_solid(MeshDev m, int ns, const Solid
         __syncthreads();
     }
 
-    // 5. cut cells (reached, not all inside): exclusive row prefix counts
-    __shared__ int part[kBlock];
-    block_row_scan(CR, [&](int r) { return __builtin_popcountll(reach[r] & ~all8[r]); }, rowpre, part);
-    const int ncut = rowpre[CR];
-
-    // 5b. signed distance of every vertex of a cut cell, once (the reference's per-solid
-    //     vertex cache, geometrictools.cpp:6-11).  vin is reused for the needed-vertex rows.
-    constexpr int VPRE = PHICAP > 0 ? MAXR + 1 : 1;
-    __shared__ int vpre[VPRE];
-    __shared__ R phic[PHICAP > 0 ? PHICAP : 1];
-    bool cached = false;
-    if constexpr (PHICAP > 0) {
-        for (int vr = tid; vr < VR; vr += kBlock) {
-            const int vj = vr % VY, vk = vr / VY;
-            unsigned long long need = 0ull;
-            for (int k = vk - 1; k <= vk; ++k) {
-                if (k < 0 || k >= nz) continue;
-                for (int j = vj - 1; j <= vj; ++j) {
-                    if (j < 0 || j >= ny) continue;
-                    const unsigned long long cut = reach[k * ny + j] & ~all8[k * ny + j];
-                    need |= cut | (cut << 1);
-                }
-            }
-            vin[vr] = need;
+    // 5. candidate list: each thread takes a contiguous run of rows; ALL_INSIDE
+    //    cells first, then the cut cells (reached, not all inside), each ascending
+    {
+        const int chunk = (CR + kBlock - 1) / kBlock;
+        const int r0 = min(tid * chunk, CR), r1 = min(r0 + chunk, CR);
+        int nin = 0, ncut = 0;
+        for (int r = r0; r < r1; ++r) {
+            const unsigned long long b = reach[r], a = all8[r];
+            nin += __builtin_popcountll(b & a);
+            ncut += __builtin_popcountll(b & ~a);
         }
-        __syncthreads();
-        block_row_scan(VR, [&](int r) { return __builtin_popcountll(vin[r]); }, vpre, part);
-        const int nneed = vpre[VR];
-        if (nneed <= PHICAP) {
-            cached = true;
-            for (int q = tid; q < nneed; q += kBlock) {
-                const int vr = row_of(vpre, VR, q);
-                const int i = select_bit(vin[vr], q - vpre[vr]);
-                phic[q] = sdf_fast<R>(K, vertex_pos<MK>(m, T.lo[0] + i, T.lo[1] + vr % VY, T.lo[2] + vr / VY));
+        int qi, qc, tin, tcut;
+        block_exscan2(nin, ncut, qi, qc, tin, tcut, scan_lds);
+        qc += tin;
+        int* tl = cand + T.off;
+        for (int r = r0; r < r1; ++r) {
+            const unsigned long long a = all8[r];
+            unsigned long long b = reach[r];
+            int j, k;
+            divmod_small(r, ny, rny, k, j);
+            const int jk = (j << 6) | (k << 19);
+            while (b) {
+                const int i = __builtin_ctzll(b);
+                b &= b - 1ull;
+                if ((a >> i) & 1ull)
+                    tl[qi++] = jk | i;
+                else
+                    tl[qc++] = jk | i;
             }
-            __syncthreads();
         }
+        if (tid == 0) ncand[s] = make_int2(tin, tcut);
     }
 
+    // 6. codes of the non-candidate and ALL_INSIDE cells (k_cells writes the cut ones)
+    R* tcodes = codes + T.off;
+    const int NC = nx * CR;
+    for (int c = tid; c < NC; c += kBlock) {
+        int i, r;
+        divmod_small(c, nx, rnx, r, i);
+        const unsigned long long bit = 1ull << i;
+        if (!(reach[r] & bit))
+            tcodes[c] = code_none<R>();
+        else if (all8[r] & bit)
+            tcodes[c] = R(__builtin_huge_val());
+    }
+    }  // solids of the workgroup
+}
+
+// Forcing, fractions and the force/torque reduction over the candidate lists:
+// one workgroup per queued solid, largest tile class first.
+template <class R, int MK>
+__global__ void __launch_bounds__(kBlock) k_cells(MeshDev m, int ns, const SolidK* __restrict__ Ks,
+                                                  const TileInfo* __restrict__ Ts, int* __restrict__ qmeta,
+                                                  const int* __restrict__ qlist, const int* __restrict__ cand,
+                                                  const int2* __restrict__ ncand, const R* __restrict__ Ux,
+                                                  const R* __restrict__ Uy, const R* __restrict__ Uz,
+                                                  R* __restrict__ codes, long long code_cap, double dt,
+                                                  double rho_f, double* __restrict__ ft) {
+    __shared__ double red[6 * 16];
+    const int tid = threadIdx.x;
+    const int item = blockIdx.x;
+    if (item >= qmeta[0] + qmeta[1] + qmeta[2]) return;
     const double dtINV = 1.0 / dt;
+    const int s = __builtin_amdgcn_readfirstlane(queue_item(qmeta, qlist, ns, item));
+    const TileInfo T = Ts[s];
+    if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) return;
+    const SolidK K = Ks[s];
+    const int2 cnt = ncand[s];
+    const int nin = cnt.x, ntot = cnt.x + cnt.y;
+    const int nx = T.n[0], ny = T.n[1];
+    const int* tl = cand + T.off;
+    R* tcodes = codes + T.off;
+
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    auto force = [&](int gi, int gj, int gk, double alpha) {
+    for (int q = tid; q < ntot; q += kBlock) {
+        const int e = tl[q];
+        const int i = e & 63, j = (e >> 6) & 8191, k = e >> 19;
+        const int li = i + nx * (j + ny * k);
+        const int gi = T.lo[0] + i, gj = T.lo[1] + j, gk = T.lo[2] + k;
+        double alpha = 1.0;
+        if (q >= nin) {
+            const R a = cell_fraction<R, MK>(m, K, gi, gj, gk);
+            const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
+            tcodes[li] = cin ? a : -a;
+            alpha = (double)a;
+        }
         const long long c = cell_id(m, gi, gj, gk);
         const d3 cc = cell_centre<MK>(m, gi, gj, gk);
         const double V = cell_volume<MK>(m, gi, gj, gk);
-        const d3 r = {cc.x - K.c[0], cc.y - K.c[1], cc.z - K.c[2]};
-        const d3 us = d3{K.v[0], K.v[1], K.v[2]} + cross(d3{K.w[0], K.w[1], K.w[2]}, r);
+        const d3 rv = {cc.x - K.c[0], cc.y - K.c[1], cc.z - K.c[2]};
+        const d3 us = d3{K.v[0], K.v[1], K.v[2]} + cross(d3{K.w[0], K.w[1], K.w[2]}, rv);
         const d3 uf{(double)Ux[c], (double)Uy[c], (double)Uz[c]};
         const d3 f = alpha * (uf - us);
-        const d3 tq = cross(r, f);
+        const d3 tq = cross(rv, f);
         acc[0] += (f.x * V) * dtINV;
         acc[1] += (f.y * V) * dtINV;
         acc[2] += (f.z * V) * dtINV;
         acc[3] += (tq.x * V) * dtINV;
         acc[4] += (tq.y * V) * dtINV;
         acc[5] += (tq.z * V) * dtINV;
-    };
-
-    R* tcodes = codes + T.off;
-    const long long NC = (long long)nx * CR;
-    // 6a. raster pass: codes of non-candidates and ALL_INSIDE cells, their forcing
-    for (long long c = tid; c < NC; c += kBlock) {
-        const int i = (int)(c % nx);
-        const int r = (int)(c / nx);
-        const unsigned long long bit = 1ull << i;
-        if (!(reach[r] & bit)) {
-            tcodes[c] = code_none<R>();
-        } else if (all8[r] & bit) {
-            tcodes[c] = R(__builtin_huge_val());
-            force(T.lo[0] + i, T.lo[1] + r % ny, T.lo[2] + r / ny, 1.0);
-        }
-    }
-    // 6b. compacted pass over the cut cells: fraction, centre test, forcing
-    for (int q = tid; q < ncut; q += kBlock) {
-        const int r = row_of(rowpre, CR, q);
-        const int i = select_bit(reach[r] & ~all8[r], q - rowpre[r]);
-        const int gi = T.lo[0] + i, gj = T.lo[1] + r % ny, gk = T.lo[2] + r / ny;
-        R alpha;
-        if (cached) {
-            const int lj = r % ny, lk = r / ny;
-            alpha = fraction_with_phi<R, MK>(m, gi, gj, gk, [&](int di, int dj, int dk, d3) {
-                const int vr = (lk + dk) * VY + (lj + dj);
-                const int b = i + di;
-                return phic[vpre[vr] + __builtin_popcountll(vin[vr] & ((1ull << b) - 1ull))];
-            });
-        } else {
-            alpha = fraction_of_cell<R, MK>(m, K, gi, gj, gk);
-        }
-        const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
-        tcodes[(long long)r * nx + i] = cin ? alpha : -alpha;
-        force(gi, gj, gk, (double)alpha);
     }
     block_reduce<6>(acc, red);
     if (tid == 0)
         for (int t = 0; t < 6; ++t) ft[6 * s + t] = acc[t] * rho_f;
-    }  // work queue
 }
 
 // Brick lists: one wave per solid, its lanes over the bricks its tile overlaps.
@@ -800,7 +891,7 @@ __global__ void __launch_bounds__(BX* BY* BZ)
             if (nin == 8) {
                 contribute(cvw, 1.0, sp + 4, slot);
             } else {
-                const R alpha = fraction_of_cell<R, MK>(m, K, i, j, k);
+                const R alpha = cell_fraction<R, MK>(m, K, i, j, k);
                 const bool cin = inside_world(K, cell_centre<MK>(m, i, j, k));
                 contribute(cvw, (double)alpha, cin ? 2 : 3, slot);
             }
@@ -867,7 +958,7 @@ __global__ void k_plane_codes(MeshDev m, SolidK K, const unsigned* __restrict__ 
     } else if (nin == 8) {
         out[c] = R(__builtin_huge_val());
     } else {
-        const R a = fraction_of_cell<R, MK>(m, K, i, j, k);
+        const R a = cell_fraction<R, MK>(m, K, i, j, k);
         out[c] = inside_world(K, cell_centre<MK>(m, i, j, k)) ? a : -a;
     }
 }
@@ -963,7 +1054,7 @@ struct sdfibm_ctx {
     int* bcount = nullptr;
     long long* boff = nullptr;
     long long* scan_tmp = nullptr;  // tile sums of the scans
-    int* qmeta = nullptr;           // tile-class work queues: counts [0..2], heads [4..6]
+    int* qmeta = nullptr;           // tile-class work queues: counts [0..2]
     int* qlist = nullptr;
     int* err = nullptr;
     // solids
@@ -977,7 +1068,9 @@ struct sdfibm_ctx {
     long long* d_tile_off = nullptr;
     long long solid_cap = 0;
     void* d_codes = nullptr;
+    int* d_cand = nullptr;  // candidate lists, same offsets as the codes
     long long code_cap = 0;
+    int2* d_ncand = nullptr;  // per solid: ALL_INSIDE and cut candidates
     int* d_blist = nullptr;
     long long pair_cap = 0;
     double* d_ft = nullptr;
@@ -993,7 +1086,7 @@ struct sdfibm_ctx {
     int* d_pseed = nullptr;
     long long member_words = 0;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    double timing[4] = {0, 0, 0, 0};
+    double timing[6] = {0, 0, 0, 0, 0, 0};
     std::string msg;
 };
 
@@ -1094,7 +1187,7 @@ int launch_step(sdfibm_ctx* c, double dt) {
     k_fill_i32<<<1, 64, 0, st>>>(c->err, 1, 0);
     k_fill_i32<<<1, 64, 0, st>>>(c->qmeta, 8, 0);
     if (ns > 0) {
-        k_setup<MK><<<grid_for(ns, 128), 128, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_cells,
+        k_setup<MK><<<grid_for(ns, 64), 64, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_cells,
                                                        c->d_ft, c->qmeta, c->qlist, c->err);
         scan<long long>(c->d_tile_cells, ns, c->d_tile_off, c->scan_tmp, st);
         k_tile_offsets<<<grid_for(ns, 128), 128, 0, st>>>(c->d_T, c->d_tile_off, ns, c->code_cap, c->err);
@@ -1121,16 +1214,21 @@ int launch_step(sdfibm_ctx* c, double dt) {
     R* codes = (R*)c->d_codes;
     if (ns > 0) {
         // persistent launches: a few workgroups per CU take solids from the class queues
-        const unsigned cus = (unsigned)c->n_cu;
-        k_solid<R, MK, kRowsSmall, 2048, 0><<<cus * 6, kBlock, 0, st>>>(
-            c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],
-            codes, c->code_cap, dt, c->rho_f, c->d_ft);
-        k_solid<R, MK, kRowsMedium, 4096, 1><<<cus * 2, kBlock, 0, st>>>(
-            c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],
-            codes, c->code_cap, dt, c->rho_f, c->d_ft);
-        k_solid<R, MK, kRowsLarge, 0, 2><<<cus, kBlock, 0, st>>>(
-            c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],
-            codes, c->code_cap, dt, c->rho_f, c->d_ft);
+        // the class sizes live on the device: the small-tile grid covers every solid (one
+        // each), the medium and large grids (big LDS, few solids) loop over their queues
+        const unsigned g = (unsigned)ns, cus = (unsigned)c->n_cu;
+        k_classify<R, MK, kRowsLarge, 2><<<std::min(g, cus), kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
+                                                               c->d_cand, c->code_cap, c->d_ncand);
+        k_classify<R, MK, kRowsMedium, 1><<<std::min(g, 4 * cus), kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
+                                                                c->d_cand, c->code_cap, c->d_ncand);
+        k_classify<R, MK, kRowsSmall, 0><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
+                                                               c->d_cand, c->code_cap, c->d_ncand);
+        (void)hipEventRecord(c->ev[4], st);
+        k_cells<R, MK><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
+                                                   c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
+                                                   (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft);
+    } else {
+        (void)hipEventRecord(c->ev[4], st);
     }
     const int np = (int)c->h_planes.size();
     for (int p = 0; p < np; ++p) {
@@ -1532,6 +1630,8 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         if ((rc = dalloc(c, (void**)&c->d_tile_cells, cap * 8))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_tile_off, (cap + 1) * 8))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_ft, cap * 6 * 8))) return rc;
+        dfree(c, c->d_ncand);
+        if ((rc = dalloc(c, (void**)&c->d_ncand, cap * sizeof(int2)))) return rc;
         dfree(c, c->qlist);
         if ((rc = dalloc(c, (void**)&c->qlist, cap * 3 * 4))) return rc;
         c->solid_cap = cap;
@@ -1539,9 +1639,12 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     const long long want_codes = cells + cells / 20 + 1024;
     if (want_codes > c->code_cap) {
         dfree(c, c->d_codes);
+        dfree(c, c->d_cand);
         c->d_codes = nullptr;
+        c->d_cand = nullptr;
         c->code_cap = 0;
         if ((rc = dalloc(c, &c->d_codes, want_codes * rsize(c)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_cand, want_codes * 4))) return rc;
         c->code_cap = want_codes;
     }
     const long long want_pairs = pairs + pairs / 20 + 1024;
@@ -1620,14 +1723,18 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     HIPCHK(c, hipMemcpyAsync(&flags, c->err, 4, hipMemcpyDeviceToHost, c->stream));
     if (ft && c->ns > 0) HIPCHK(c, hipMemcpyAsync(ft, c->d_ft, c->ns * 48, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    float ms[3] = {0, 0, 0};
+    float ms[5] = {0, 0, 0, 0, 0};
     (void)hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]);
     (void)hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]);
     (void)hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]);
+    (void)hipEventElapsedTime(&ms[3], c->ev[1], c->ev[4]);
+    (void)hipEventElapsedTime(&ms[4], c->ev[4], c->ev[2]);
     c->timing[0] = ms[0];
     c->timing[1] = ms[1];
     c->timing[2] = ms[2];
     c->timing[3] = (double)ms[0] + ms[1] + ms[2];
+    c->timing[4] = ms[3];
+    c->timing[5] = ms[4];
     if (flags) {
         c->have_step = false;
         c->have_ct = false;
@@ -1696,7 +1803,7 @@ int sdfibm_get_velocity(sdfibm_ctx* c, double* U) {
 
 int sdfibm_last_timing(sdfibm_ctx* c, double* ms, int n) {
     if (!c || !ms || n < 0) return SDFIBM_ERR_ARG;
-    const int k = n < 4 ? n : 4;
+    const int k = n < 6 ? n : 6;
     for (int t = 0; t < k; ++t) ms[t] = c->timing[t];
     return k;
 }
