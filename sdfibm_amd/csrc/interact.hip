@@ -49,7 +49,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxPerBrick = 128;  // solids per brick handled by k_fields
 constexpr int kMaxPlanes = 8;
 
-enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_PAIRCAP = 4, EF_BRICK = 8 };
+enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_BRICK = 8 };
 
 // ---------------------------------------------------------------------------
 // small device helpers
@@ -572,11 +572,10 @@ __global__ void __launch_bounds__(kBlock) k_cells(MeshDev m, int ns, const Solid
 }
 
 // Brick lists: one wave per solid, its lanes over the bricks its tile overlaps.
-// fill = 0 counts per brick; fill = 1 writes the solid into each brick's list.
+// Brick b's list is blist[b * kMaxPerBrick + t], t < count[b] (in no order).
 template <int BX, int BY, int BZ>
 __global__ void k_bin(int ns, const TileInfo* __restrict__ T, int nbx, int nby, int* __restrict__ count,
-                      const long long* __restrict__ boff, int* __restrict__ blist, long long pair_cap,
-                      int* __restrict__ err, int fill) {
+                      int* __restrict__ blist, int* __restrict__ err) {
     const int s = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     if (s >= ns) return;
@@ -591,13 +590,10 @@ __global__ void k_bin(int ns, const TileInfo* __restrict__ T, int nbx, int nby, 
         const int bx = b0x + q % ex, by = b0y + (q / ex) % ey, bz = b0z + q / (ex * ey);
         const long long b = bx + (long long)nbx * (by + (long long)nby * bz);
         const int slot = atomicAdd(&count[b], 1);
-        if (fill) {
-            const long long at = boff[b] + slot;
-            if (at < pair_cap)
-                blist[at] = s;
-            else
-                atomicOr(err, EF_PAIRCAP);
-        }
+        if (slot < kMaxPerBrick)
+            blist[b * kMaxPerBrick + slot] = s;
+        else
+            atomicOr(err, EF_BRICK);
     }
 }
 
@@ -733,11 +729,14 @@ __global__ void k_plane_member(const int* __restrict__ list, const int* __restri
     if (t == 0) *seed_used = s;
 }
 
-// Field pass: one workgroup per brick of BX*BY*BZ cells.
+// Field pass: one workgroup per brick of BX*BY*BZ cells.  Each wave covers
+// 64/BX whole x-rows of one z-layer of the brick; it skips, with scalar tests on
+// the solid's tile record (scalar-cache loads), the listed solids whose tile
+// misses its rows.
 template <class R, int MK, int BX, int BY, int BZ, bool HASP>
 __global__ void __launch_bounds__(BX* BY* BZ)
-    k_fields(MeshDev m, int nbx, int nby, const long long* __restrict__ boff, const int* __restrict__ blist,
-             long long pair_cap, const SolidK* __restrict__ Ks, const TileInfo* __restrict__ Ts,
+    k_fields(MeshDev m, int nbx, int nby, const int* __restrict__ bcount, const int* __restrict__ blist,
+             const SolidK* __restrict__ Ks, const TileInfo* __restrict__ Ts,
              const R* __restrict__ codes, long long code_cap, const int* __restrict__ planes, int nplanes,
              const unsigned* __restrict__ plane_member, long long member_words,
              const double* __restrict__ brick_box, const R* __restrict__ Ux, const R* __restrict__ Uy,
@@ -745,10 +744,9 @@ __global__ void __launch_bounds__(BX* BY* BZ)
              R* __restrict__ Fz, R* __restrict__ Tsf, int* __restrict__ Ct, double dt,
              double* __restrict__ plane_partial, long long nbricks, int* __restrict__ err) {
     constexpr int NT = BX * BY * BZ;
+    static_assert(64 % BX == 0 && (BX * BY) % 64 == 0, "a wave covers whole x-rows of one layer");
     __shared__ int raw[kMaxPerBrick];
     __shared__ int lst[kMaxPerBrick];
-    __shared__ TileInfo sTile[kMaxPerBrick];
-    __shared__ double sK[kMaxPerBrick][9];  // centre, velocity, angular velocity
     __shared__ double red[6 * 16];
     __shared__ int pl_any[kMaxPlanes];
 
@@ -759,14 +757,22 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     const int i = bx * BX + ti, j = by * BY + tj, k = bz * BZ + tk;
     const bool valid = i < m.nx && j < m.ny && k < m.nz;
 
-    long long p0 = boff[b], p1 = boff[b + 1];
-    if (p1 > pair_cap) p1 = pair_cap;
-    int cnt = (int)(p1 - p0);
-    if (cnt > kMaxPerBrick) {
-        if (tid == 0) atomicOr(err, EF_BRICK);
-        cnt = kMaxPerBrick;
+    // list entries loaded together with the count (the first 32, one cache line, speculatively)
+    constexpr int kSpec = 32;
+    const int* bl = blist + b * kMaxPerBrick;
+    const int spec = tid < kSpec ? bl[tid] : 0;
+    const int cnt = min(bcount[b], kMaxPerBrick);  // k_bin flagged an overflow
+    // the cell's velocity, centre and volume, needed if a listed solid or a plane takes it
+    d3 uf{0, 0, 0}, cc{0, 0, 0};
+    double V = 0.0;
+    if (valid && (cnt > 0 || (HASP && nplanes > 0))) {
+        const long long c = cell_id(m, i, j, k);
+        uf = {(double)Ux[c], (double)Uy[c], (double)Uz[c]};
+        cc = cell_centre<MK>(m, i, j, k);
+        V = cell_volume<MK>(m, i, j, k);
     }
-    for (int t = tid; t < cnt; t += NT) raw[t] = blist[p0 + t];
+    if (tid < min(cnt, kSpec)) raw[tid] = spec;
+    for (int t = kSpec + tid; t < cnt; t += NT) raw[t] = bl[t];
     if (tid < kMaxPlanes) pl_any[tid] = 0;
     __syncthreads();
     for (int t = tid; t < cnt; t += NT) {  // rank sort (solid ids are distinct)
@@ -774,15 +780,13 @@ __global__ void __launch_bounds__(BX* BY* BZ)
         int rank = 0;
         for (int u = 0; u < cnt; ++u) rank += raw[u] < v;
         lst[rank] = v;
-        sTile[rank] = Ts[v];
-        const SolidK& K = Ks[v];
-        for (int d = 0; d < 3; ++d) {
-            sK[rank][d] = K.c[d];
-            sK[rank][3 + d] = K.v[d];
-            sK[rank][6 + d] = K.w[d];
-        }
     }
     __syncthreads();
+    // the wave's cell box (uniform)
+    const int w0 = (tid >> 6) << 6;
+    const int wk = bz * BZ + w0 / (BX * BY);
+    const int wj0 = by * BY + (w0 % (BX * BY)) / BX, wj1 = wj0 + 64 / BX - 1;
+    const int wi0 = bx * BX, wi1 = wi0 + BX - 1;
 
     // brick vertex box, for plane culling
     double bb[6];
@@ -791,9 +795,6 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     const double dtINV = 1.0 / dt;
     double sA = 0.0, sT = 0.0, sF[3] = {0.0, 0.0, 0.0};
     int ct = 0;
-    bool have_u = false;
-    d3 uf{0, 0, 0}, cc{0, 0, 0};
-    double V = 0.0;
     constexpr int NPACC = HASP ? kMaxPlanes : 1;
     double pacc[NPACC][6];
 #pragma unroll
@@ -802,13 +803,6 @@ __global__ void __launch_bounds__(BX* BY* BZ)
         for (int t = 0; t < 6; ++t) pacc[p][t] = 0.0;
 
     auto contribute = [&](const double* cvw, double alpha, int ctype, int plane_slot) {
-        if (!have_u) {
-            const long long c = cell_id(m, i, j, k);
-            uf = {(double)Ux[c], (double)Uy[c], (double)Uz[c]};
-            cc = cell_centre<MK>(m, i, j, k);
-            V = cell_volume<MK>(m, i, j, k);
-            have_u = true;
-        }
         const d3 r = {cc.x - cvw[0], cc.y - cvw[1], cc.z - cvw[2]};
         const d3 us = d3{cvw[3], cvw[4], cvw[5]} + cross(d3{cvw[6], cvw[7], cvw[8]}, r);
         const d3 f = alpha * (uf - us);
@@ -846,12 +840,17 @@ __global__ void __launch_bounds__(BX* BY* BZ)
                 for (int q = 0; q < kBatch; ++q) {
                     R code = code_none<R>();
                     const int a = ia + q;
-                    if (valid && a < cnt) {
-                        const TileInfo& T = sTile[a];
-                        const int li = i - T.lo[0], lj = j - T.lo[1], lk = k - T.lo[2];
-                        if (li >= 0 && li < T.n[0] && lj >= 0 && lj < T.n[1] && lk >= 0 && lk < T.n[2] &&
-                            T.off + (long long)T.n[0] * T.n[1] * T.n[2] <= code_cap)
-                            code = codes[T.off + (long long)li + (long long)T.n[0] * (lj + (long long)T.n[1] * lk)];
+                    if (a < cnt) {
+                        const int sq = __builtin_amdgcn_readfirstlane(lst[a]);
+                        const TileInfo T = Ts[sq];  // uniform: scalar loads
+                        const bool hit = T.lo[0] <= wi1 && T.lo[0] + T.n[0] > wi0 && T.lo[1] <= wj1 &&
+                                         T.lo[1] + T.n[1] > wj0 && T.lo[2] <= wk && T.lo[2] + T.n[2] > wk &&
+                                         T.off + (long long)T.n[0] * T.n[1] * T.n[2] <= code_cap;
+                        if (hit) {
+                            const unsigned li = (unsigned)(i - T.lo[0]), lj = (unsigned)(j - T.lo[1]);
+                            if (valid && li < (unsigned)T.n[0] && lj < (unsigned)T.n[1])
+                                code = codes[T.off + (long long)(li + T.n[0] * (lj + T.n[1] * (k - T.lo[2])))];
+                        }
                     }
                     cb[q] = code;
                 }
@@ -860,9 +859,11 @@ __global__ void __launch_bounds__(BX* BY* BZ)
 #pragma unroll
             for (int q = 1; q < kBatch; ++q)
                 if (ia - cb0 == q) code = cb[q];
-            const double* cvw = sK[ia];
             ++ia;
-            if (code != code) continue;  // NaN: not a candidate
+            if (!__any(code == code)) continue;  // no candidate in the wave (NaN codes)
+            const SolidK& Ka = Ks[sa];
+            const double cvw[9] = {Ka.c[0], Ka.c[1], Ka.c[2], Ka.v[0], Ka.v[1], Ka.v[2], Ka.w[0], Ka.w[1], Ka.w[2]};
+            if (code != code) continue;
             if (isinf(code))
                 contribute(cvw, 1.0, sa + 4, -1);
             else
@@ -1025,7 +1026,7 @@ __global__ void k_fields_to_f64(const R* __restrict__ As, const R* __restrict__ 
 // ===========================================================================
 // host side
 // ===========================================================================
-constexpr int kBX3 = 32, kBY3 = 4, kBZ3 = 2;  // brick of a 3-D mesh
+constexpr int kBX3 = 16, kBY3 = 4, kBZ3 = 4;  // brick of a 3-D mesh
 constexpr int kBX2 = 32, kBY2 = 8, kBZ2 = 1;  // brick of a one-cell-thick (2-D) mesh
 
 struct sdfibm_ctx {
@@ -1051,8 +1052,7 @@ struct sdfibm_ctx {
     int bx = 0, by = 0, bz = 0;
     int nbx = 0, nby = 0, nbz = 0;
     long long nbricks = 0;
-    int* bcount = nullptr;
-    long long* boff = nullptr;
+    int* bcount = nullptr;          // solids per brick
     long long* scan_tmp = nullptr;  // tile sums of the scans
     int* qmeta = nullptr;           // tile-class work queues: counts [0..2]
     int* qlist = nullptr;
@@ -1071,8 +1071,7 @@ struct sdfibm_ctx {
     int* d_cand = nullptr;  // candidate lists, same offsets as the codes
     long long code_cap = 0;
     int2* d_ncand = nullptr;  // per solid: ALL_INSIDE and cut candidates
-    int* d_blist = nullptr;
-    long long pair_cap = 0;
+    int* d_blist = nullptr;  // brick lists, kMaxPerBrick slots per brick
     double* d_ft = nullptr;
     std::vector<int> h_planes;
     int* d_planes = nullptr;
@@ -1195,20 +1194,10 @@ int launch_step(sdfibm_ctx* c, double dt) {
     k_fill_i32<<<grid_for(c->nbricks, 256), 256, 0, st>>>(c->bcount, c->nbricks, 0);
     if (ns > 0) {
         const unsigned gb = grid_for(64LL * ns, 256);
-        for (int pass = 0; pass < 2; ++pass) {
-            if (c->nz == 1)
-                k_bin<kBX2, kBY2, kBZ2><<<gb, 256, 0, st>>>(ns, c->d_T, c->nbx, c->nby, c->bcount, c->boff,
-                                                            c->d_blist, c->pair_cap, c->err, pass);
-            else
-                k_bin<kBX3, kBY3, kBZ3><<<gb, 256, 0, st>>>(ns, c->d_T, c->nbx, c->nby, c->bcount, c->boff,
-                                                            c->d_blist, c->pair_cap, c->err, pass);
-            if (pass == 0) {
-                scan<int>(c->bcount, c->nbricks, c->boff, c->scan_tmp, st);
-                k_fill_i32<<<grid_for(c->nbricks, 256), 256, 0, st>>>(c->bcount, c->nbricks, 0);
-            }
-        }
-    } else {
-        scan<int>(c->bcount, c->nbricks, c->boff, c->scan_tmp, st);
+        if (c->nz == 1)
+            k_bin<kBX2, kBY2, kBZ2><<<gb, 256, 0, st>>>(ns, c->d_T, c->nbx, c->nby, c->bcount, c->d_blist, c->err);
+        else
+            k_bin<kBX3, kBY3, kBZ3><<<gb, 256, 0, st>>>(ns, c->d_T, c->nbx, c->nby, c->bcount, c->d_blist, c->err);
     }
     (void)hipEventRecord(c->ev[1], st);
     R* codes = (R*)c->d_codes;
@@ -1268,7 +1257,7 @@ int launch_step(sdfibm_ctx* c, double dt) {
     (void)hipEventRecord(c->ev[2], st);
 #define SDFIBM_FIELDS(BX_, BY_, BZ_, HP_)                                                                       \
     k_fields<R, MK, BX_, BY_, BZ_, HP_><<<(unsigned)c->nbricks, BX_ * BY_ * BZ_, 0, st>>>(                          \
-        c->m, c->nbx, c->nby, c->boff, c->d_blist, c->pair_cap, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np, \
+        c->m, c->nbx, c->nby, c->bcount, c->d_blist, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np,          \
         c->d_pmember, c->member_words, c->brick_box, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],      \
         (R*)c->As, (R*)c->Fs[0], (R*)c->Fs[1], (R*)c->Fs[2], (R*)c->Tsf, c->Ct, dt, c->d_plane_partial, c->nbricks, \
         c->err)
@@ -1457,7 +1446,6 @@ std::string err_text(int flags) {
     std::string s;
     if (flags & EF_TILE) s += "a solid's cell tile exceeds 63 cells along x or 4096 vertex rows; ";
     if (flags & EF_CODECAP) s += "tile code buffer too small; ";
-    if (flags & EF_PAIRCAP) s += "brick pair buffer too small; ";
     if (flags & EF_BRICK) s += "more than 128 solids overlap one brick; ";
     return s;
 }
@@ -1550,11 +1538,10 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
     c->nbz = (c->nz + c->bz - 1) / c->bz;
     c->nbricks = (long long)c->nbx * c->nby * c->nbz;
     if ((rc = dalloc(c, (void**)&c->bcount, c->nbricks * 4))) return bail(rc);
-    if ((rc = dalloc(c, (void**)&c->boff, (c->nbricks + 1) * 8))) return bail(rc);
+    if ((rc = dalloc(c, (void**)&c->d_blist, c->nbricks * kMaxPerBrick * 4))) return bail(rc);
     if ((rc = dalloc(c, (void**)&c->err, 64))) return bail(rc);
     if ((rc = dalloc(c, (void**)&c->qmeta, 64))) return bail(rc);
-    if ((rc = dalloc(c, (void**)&c->scan_tmp, ((c->nbricks + kScanTile - 1) / kScanTile + 4096) * 8)))
-        return bail(rc);
+
     {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -1596,7 +1583,7 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if (n < 0 || (n > 0 && !solids)) return fail(c, SDFIBM_ERR_ARG, "bad solid array");
     if (n > (1LL << 30)) return fail(c, SDFIBM_ERR_ARG, "too many solids");
     std::vector<int> planes;
-    long long cells = 0, pairs = 0;
+    long long cells = 0;
     for (int64_t s = 0; s < n; ++s) {
         std::string why;
         const int v = validate_solid(solids[s], s, why);
@@ -1614,7 +1601,6 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         const long long ex = std::min(t.n[0] + 1, c->nx), ey = std::min(t.n[1] + 1, c->ny),
                         ez = std::min(t.n[2] + 1, c->nz);
         cells += ex * ey * ez;
-        pairs += (long long)(ex / c->bx + 2) * (ey / c->by + 2) * (ez / c->bz + 2);
     }
     if ((int)planes.size() > kMaxPlanes)
         return fail(c, SDFIBM_ERR_ARG, "at most " + std::to_string(kMaxPlanes) + " plane solids are supported");
@@ -1634,6 +1620,8 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         if ((rc = dalloc(c, (void**)&c->d_ncand, cap * sizeof(int2)))) return rc;
         dfree(c, c->qlist);
         if ((rc = dalloc(c, (void**)&c->qlist, cap * 3 * 4))) return rc;
+        dfree(c, c->scan_tmp);
+        if ((rc = dalloc(c, (void**)&c->scan_tmp, ((cap + kScanTile - 1) / kScanTile + 2) * 8))) return rc;
         c->solid_cap = cap;
     }
     const long long want_codes = cells + cells / 20 + 1024;
@@ -1646,14 +1634,6 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         if ((rc = dalloc(c, &c->d_codes, want_codes * rsize(c)))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_cand, want_codes * 4))) return rc;
         c->code_cap = want_codes;
-    }
-    const long long want_pairs = pairs + pairs / 20 + 1024;
-    if (want_pairs > c->pair_cap) {
-        dfree(c, c->d_blist);
-        c->d_blist = nullptr;
-        c->pair_cap = 0;
-        if ((rc = dalloc(c, (void**)&c->d_blist, want_pairs * 4))) return rc;
-        c->pair_cap = want_pairs;
     }
     dfree(c, c->d_planes);
     dfree(c, c->d_plane_partial);
