@@ -130,11 +130,15 @@ def main():
     from sdfibm_amd.cloud import SolidCloud
 
     if args.copy_test > 0:
+        # counter calibration: a device-to-device copy and an elementwise negation, each
+        # reading and writing 4 bytes per element exactly once
         n = int(args.copy_test * 1e9) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         for _ in range(5):
             b.copy_(a)
+        for _ in range(5):
+            torch.neg(a, out=b)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}), flush=True)
         return

@@ -271,9 +271,11 @@ __global__ void k_tile_offsets(TileInfo* __restrict__ T, const long long* __rest
 // Both take their solids from work queues (persistent workgroups).
 // ---------------------------------------------------------------------------
 
-// Block-wide exclusive scan of two per-thread counts (kBlock threads): wave
-// prefix sums by shuffles, then the wave totals through LDS.
-__device__ inline void block_exscan2(int a, int b, int& pa, int& pb, int& ta, int& tb, int* lds /* [8] */) {
+// Block-wide exclusive scan of two per-thread counts (NT threads): wave prefix
+// sums by shuffles, then the wave totals through LDS.
+template <int NT>
+__device__ inline void block_exscan2(int a, int b, int& pa, int& pb, int& ta, int& tb, int* lds /* [32] */) {
+    constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int ia = a, ib = b;
 #pragma unroll
@@ -286,13 +288,13 @@ __device__ inline void block_exscan2(int a, int b, int& pa, int& pb, int& ta, in
     }
     if (lane == 63) {
         lds[wv] = ia;
-        lds[4 + wv] = ib;
+        lds[16 + wv] = ib;
     }
     __syncthreads();
     int ba = 0, bb = 0, sa = 0, sb = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
-        const int x = lds[w], y = lds[4 + w];
+    for (int w = 0; w < NW; ++w) {
+        const int x = lds[w], y = lds[16 + w];
         if (w < wv) {
             ba += x;
             bb += y;
@@ -326,8 +328,8 @@ constexpr int kStageYZ = 256;  // tile coordinates staged in LDS up to this many
 // dispatcher balances the uneven tiles).  MAXR bounds the vertex rows
 // (ny+1)(nz+1) of the tiles of the queue.  No shared work counter: that would
 // cost a device-scope atomic (slow across the XCDs) per solid.
-template <class R, int MK, int MAXR, int Q>
-__global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const SolidK* __restrict__ Ks,
+template <class R, int MK, int MAXR, int Q, int NT>
+__global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK* __restrict__ Ks,
                                                      const TileInfo* __restrict__ Ts, int* __restrict__ qmeta,
                                                      const int* __restrict__ qlist, R* __restrict__ codes,
                                                      int* __restrict__ cand, long long code_cap,
@@ -339,7 +341,7 @@ __global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const So
     constexpr int NSX = MK == MK_RECT ? kMaxTileX + 2 : 1;
     constexpr int NSYZ = MK == MK_RECT ? (MAXR / 2 + 1 < kStageYZ ? MAXR / 2 + 1 : kStageYZ) : 1;
     __shared__ double sx[NSX], sy[NSYZ], sz[NSYZ];
-    __shared__ int scan_lds[8];
+    __shared__ int scan_lds[32];
     __shared__ int sh_first, sh_changed;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nq = qmeta[Q];
@@ -359,9 +361,9 @@ __global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const So
     // tile coordinates of a rectilinear mesh in LDS
     const bool staged = MK == MK_RECT && VY <= NSYZ && nz + 1 <= NSYZ;
     if (staged) {
-        for (int t = tid; t < VX; t += kBlock) sx[t] = m.xs[T.lo[0] + t];
-        for (int t = tid; t < VY; t += kBlock) sy[t] = m.ys[T.lo[1] + t];
-        for (int t = tid; t <= nz; t += kBlock) sz[t] = m.zs[T.lo[2] + t];
+        for (int t = tid; t < VX; t += NT) sx[t] = m.xs[T.lo[0] + t];
+        for (int t = tid; t < VY; t += NT) sy[t] = m.ys[T.lo[1] + t];
+        for (int t = tid; t <= nz; t += NT) sz[t] = m.zs[T.lo[2] + t];
     }
     if (tid == 0) sh_first = 0x7fffffff;
     __syncthreads();
@@ -372,7 +374,7 @@ __global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const So
         const int g = 64 / VX;
         const int rr = lane / VX, vi = lane - rr * VX;
         const unsigned long long vmask = VX >= 64 ? ~0ull : ((1ull << VX) - 1ull);
-        for (int base = wv * g; base < VR; base += (kBlock / 64) * g) {
+        for (int base = wv * g; base < VR; base += (NT / 64) * g) {
             const int r = base + rr;
             const bool mine = rr < g && r < VR;
             bool in = false;
@@ -393,7 +395,7 @@ __global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const So
     //    (one LDS atomic per wave)
     {
         bool found = false;
-        for (int r0 = wv * 64; r0 < CR; r0 += kBlock) {
+        for (int r0 = wv * 64; r0 < CR; r0 += NT) {
             const int r = r0 + lane;
             unsigned long long av = 0ull;
             if (r < CR) {
@@ -446,7 +448,7 @@ __global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const So
         if (tid == 0) sh_changed = 0;
         __syncthreads();
         int ch = 0;
-        for (int r = tid; r < CR; r += kBlock) {
+        for (int r = tid; r < CR; r += NT) {
             int j, k;
             divmod_small(r, ny, rny, k, j);
             unsigned long long nb = reach[r];
@@ -468,7 +470,7 @@ __global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const So
     // 5. candidate list: each thread takes a contiguous run of rows; ALL_INSIDE
     //    cells first, then the cut cells (reached, not all inside), each ascending
     {
-        const int chunk = (CR + kBlock - 1) / kBlock;
+        const int chunk = (CR + NT - 1) / NT;
         const int r0 = min(tid * chunk, CR), r1 = min(r0 + chunk, CR);
         int nin = 0, ncut = 0;
         for (int r = r0; r < r1; ++r) {
@@ -477,7 +479,7 @@ __global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const So
             ncut += __builtin_popcountll(b & ~a);
         }
         int qi, qc, tin, tcut;
-        block_exscan2(nin, ncut, qi, qc, tin, tcut, scan_lds);
+        block_exscan2<NT>(nin, ncut, qi, qc, tin, tcut, scan_lds);
         qc += tin;
         int* tl = cand + T.off;
         for (int r = r0; r < r1; ++r) {
@@ -501,7 +503,7 @@ __global__ void __launch_bounds__(kBlock) k_classify(MeshDev m, int ns, const So
     // 6. codes of the non-candidate and ALL_INSIDE cells (k_cells writes the cut ones)
     R* tcodes = codes + T.off;
     const int NC = nx * CR;
-    for (int c = tid; c < NC; c += kBlock) {
+    for (int c = tid; c < NC; c += NT) {
         int i, r;
         divmod_small(c, nx, rnx, r, i);
         const unsigned long long bit = 1ull << i;
@@ -1033,6 +1035,8 @@ struct sdfibm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t aux = nullptr;                   // second stream of the tile classification
+    hipEvent_t fork = nullptr, join = nullptr;  // its dependencies on the main stream
     int kind = 0, prec = 0, two_d = 0;
     int nx = 0, ny = 0, nz = 0;
     long long N = 0;
@@ -1205,13 +1209,19 @@ int launch_step(sdfibm_ctx* c, double dt) {
         // persistent launches: a few workgroups per CU take solids from the class queues
         // the class sizes live on the device: the small-tile grid covers every solid (one
         // each), the medium and large grids (big LDS, few solids) loop over their queues
+        // the few large and medium tiles (long per-solid latency) run beside the small
+        // ones on a second stream
         const unsigned g = (unsigned)ns, cus = (unsigned)c->n_cu;
-        k_classify<R, MK, kRowsLarge, 2><<<std::min(g, cus), kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
+        if (hipEventRecord(c->fork, st) != hipSuccess || hipStreamWaitEvent(c->aux, c->fork, 0) != hipSuccess)
+            return -1;
+        k_classify<R, MK, kRowsLarge, 2, 1024><<<std::min(g, cus), 1024, 0, c->aux>>>(
+            c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
+        k_classify<R, MK, kRowsMedium, 1, 1024><<<std::min(g, 4 * cus), 1024, 0, c->aux>>>(
+            c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
+        if (hipEventRecord(c->join, c->aux) != hipSuccess) return -1;
+        k_classify<R, MK, kRowsSmall, 0, kBlock><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
                                                                c->d_cand, c->code_cap, c->d_ncand);
-        k_classify<R, MK, kRowsMedium, 1><<<std::min(g, 4 * cus), kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
-                                                                c->d_cand, c->code_cap, c->d_ncand);
-        k_classify<R, MK, kRowsSmall, 0><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
-                                                               c->d_cand, c->code_cap, c->d_ncand);
+        if (hipStreamWaitEvent(st, c->join, 0) != hipSuccess) return -1;
         (void)hipEventRecord(c->ev[4], st);
         k_cells<R, MK><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
                                                    c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
@@ -1515,11 +1525,20 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
         return bail(SDFIBM_ERR_HIP);
     }
     c->own_stream = true;
+    if (hipStreamCreate(&c->aux) != hipSuccess) {
+        c->msg = "hipStreamCreate failed";
+        return bail(SDFIBM_ERR_HIP);
+    }
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) {
             c->msg = "hipEventCreate failed";
             return bail(SDFIBM_ERR_HIP);
         }
+    if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+        c->msg = "hipEventCreate failed";
+        return bail(SDFIBM_ERR_HIP);
+    }
     const size_t rs = rsize(c);
     for (int a = 0; a < 3; ++a) {
         if ((rc = dalloc(c, &c->U[a], c->N * rs))) return bail(rc);
@@ -1554,10 +1573,14 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
 int sdfibm_destroy(sdfibm_ctx* c) {
     if (!c) return SDFIBM_ERR_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SDFIBM_OK;
