@@ -39,18 +39,30 @@ __device__ inline R rmag(r3<R> a) {
     return sqrt(rdot(a, a));
 }
 
+// Quotient of the fraction arithmetic: exact division in FP64; in FP32 the
+// hardware reciprocal (1 ulp) times the numerator -- two operations instead of
+// the ten of a correctly rounded divide.  Every divisor here is positive.
+template <class R>
+__device__ inline R qdiv(R a, R b) {
+    return a / b;
+}
+template <>
+__device__ inline float qdiv<float>(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+}
+
 // Line fraction (geometrictools.cpp:13-23).
 template <class R>
 __device__ inline R line_fraction(R pa, R pb) {
     if (pa > 0 && pb > 0) return R(0);
     if (pa <= 0 && pb <= 0) return R(1);
-    if (pa > 0) return -pb / (pa - pb);
-    return -pa / (pb - pa);
+    if (pa > 0) return qdiv<R>(-pb, pa - pb);
+    return qdiv<R>(-pa, pb - pa);
 }
 
 template <class R>
 __device__ inline r3<R> apex_point(r3<R> A, R pA, r3<R> B, R pB) {
-    const R s = fabs(pA) / (R(kSMALL) + fabs(pA) + fabs(pB));
+    const R s = qdiv<R>(fabs(pA), R(kSMALL) + fabs(pA) + fabs(pB));
     return {A.x - s * (A.x - B.x), A.y - s * (A.y - B.y), A.z - s * (A.z - B.z)};
 }
 
@@ -208,7 +220,7 @@ __device__ inline R unit_face_fraction(const Phi8<R>& h, R lu0, R lu1, R lv0, R 
     constexpr R u0 = R((V0 >> AU) & 1), v0 = R((V0 >> AV) & 1);
     const R bu = c1 ? R((V1 >> AU) & 1) : (c2 ? R((V2 >> AU) & 1) : R((V3 >> AU) & 1));
     const R bv = c1 ? R((V1 >> AV) & 1) : (c2 ? R((V2 >> AV) & 1) : R((V3 >> AV) & 1));
-    const R s = fabs(p0) / (R(kSMALL) + fabs(p0) + fabs(pB));
+    const R s = qdiv<R>(fabs(p0), R(kSMALL) + fabs(p0) + fabs(pB));
     const R au = u0 - s * (u0 - bu), av = v0 - s * (v0 - bv);
     return R(0.5) * (lv0 * av + lu1 * (R(1) - au) + lv1 * (R(1) - av) + lu0 * au);
 }
@@ -243,7 +255,7 @@ __device__ inline R box_fraction(const Phi8<R>& h, bool bx, bool by, bool bz, bo
     if (f0 * h.f3 <= 0) { b = 3; pB = h.f3; }
     if (f0 * h.f2 <= 0) { b = 2; pB = h.f2; }
     if (f0 * h.f1 <= 0) { b = 1; pB = h.f1; }
-    const R s = fabs(f0) / (R(kSMALL) + fabs(f0) + fabs(pB));
+    const R s = qdiv<R>(fabs(f0), R(kSMALL) + fabs(f0) + fabs(pB));
     const R ax = (b & 1) ? s : R(0), ay = (b & 2) ? s : R(0);
     const R az = two_d ? az_2d : ((b & 4) ? s : R(0));
     const R vol = exm * fabs(ax) + exp_ * fabs(ax - R(1)) + eym * fabs(ay) + eyp * fabs(ay - R(1)) +

@@ -257,18 +257,19 @@ __host__ __device__ inline R sdf_local(const SolidK& K, R px, R py, R pz) {
 // FP32 signed distance with the cancelling part in FP64: the value near the
 // surface is a small difference of O(1) terms (|p|^2 - r^2, sum x_i^2/a_i^2 - 1,
 // |x| - a), which is formed in FP64 and rounded once; only square roots and
-// divides of well-conditioned quantities run in FP32.  Same function as the
+// divides of well-conditioned quantities run in FP32, on the hardware's 1-ulp
+// square root, reciprocal and reciprocal square root.  Same function as the
 // reference formula up to rounding.
-__host__ __device__ inline float sdf_local_mixed(const SolidK& K, d3 p) {
+__device__ inline float sdf_local_mixed(const SolidK& K, d3 p) {
     const double cx = K.com[0], cy = K.com[1], cz = K.com[2];
     auto sph = [](double d2, double r, double r2) -> float {
         const float num = (float)(d2 - r2);
-        return num / (sqrtf((float)d2) + (float)r);
+        return num * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf((float)d2) + (float)r);
     };
     auto rect = [](double x, double y, double ra, double rb) -> float {
         const float dx = (float)(fabs(x) - ra), dy = (float)(fabs(y) - rb);
         const float dxp = fmaxf(0.f, dx), dyp = fmaxf(0.f, dy);
-        return sqrtf(dxp * dxp + dyp * dyp) + fminf(0.f, fmaxf(dx, dy));
+        return __builtin_amdgcn_sqrtf(dxp * dxp + dyp * dyp) + fminf(0.f, fmaxf(dx, dy));
     };
     switch (K.shape) {
     case SDFIBM_SHAPE_SPHERE: {
@@ -283,20 +284,21 @@ __host__ __device__ inline float sdf_local_mixed(const SolidK& K, d3 p) {
         const double xa = p.x * p.x * K.k[0], yb = p.y * p.y * K.k[1], zc = p.z * p.z * K.k[2];
         const float qm1 = (float)(xa + yb + zc - 1.0);
         const float g = (float)(xa * K.k[0] + yb * K.k[1] + zc * K.k[2]);
-        return filter_sd<float>(0.5f * qm1 / sqrtf(g));
+        return filter_sd<float>(0.5f * qm1 * __builtin_amdgcn_rsqf(g));
     }
     case SDFIBM_SHAPE_ELLIPSE: {
         const double x = cx + p.x, y = cy + p.y;
         const double xa = x * x * K.k[0], yb = y * y * K.k[1];
         const float qm1 = (float)(xa + yb - 1.0);
         const float g = (float)(xa * K.k[0] + yb * K.k[1]);
-        return filter_sd<float>(0.5f * qm1 / sqrtf(g));
+        return filter_sd<float>(0.5f * qm1 * __builtin_amdgcn_rsqf(g));
     }
     case SDFIBM_SHAPE_BOX: {
         const double x = cx + p.x, y = cy + p.y, z = cz + p.z;
         const float dx = (float)(fabs(x) - K.k[0]), dy = (float)(fabs(y) - K.k[1]), dz = (float)(fabs(z) - K.k[2]);
         const float dxp = fmaxf(0.f, dx), dyp = fmaxf(0.f, dy), dzp = fmaxf(0.f, dz);
-        return filter_sd<float>(sqrtf(dxp * dxp + dyp * dyp + dzp * dzp) + fminf(0.f, fmaxf(dz, fmaxf(dx, dy))));
+        return filter_sd<float>(__builtin_amdgcn_sqrtf(dxp * dxp + dyp * dyp + dzp * dzp) +
+                                fminf(0.f, fmaxf(dz, fmaxf(dx, dy))));
     }
     case SDFIBM_SHAPE_RECTANGLE: return filter_sd<float>(rect(cx + p.x, cy + p.y, K.k[0], K.k[1]));
     case SDFIBM_SHAPE_PLANE: return (float)p.y;
@@ -320,7 +322,7 @@ __host__ __device__ inline float sdf_local_mixed(const SolidK& K, d3 p) {
 }
 
 template <class R>
-__host__ __device__ inline R sdf_world(const SolidK& K, d3 p) {
+__device__ inline R sdf_world(const SolidK& K, d3 p) {
     const d3 l = world_to_local(K, p);
     if constexpr (sizeof(R) == sizeof(float))
         return sdf_local_mixed(K, l);
