@@ -642,8 +642,8 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     k_plane_active(MeshDev m, int nbx, int nby, const double* __restrict__ brick_box, SolidK K,
                    unsigned* __restrict__ act, int* __restrict__ list, int* __restrict__ nlist, long long cap,
                    int* __restrict__ minidx) {
-    const long long b = blockIdx.x;
-    const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long long)nbx * nby));
+    const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    const long long b = bx + (long long)nbx * (by + (long long)nby * bz);
     double bb[6];
     brick_vertex_box<MK, BX, BY, BZ>(m, brick_box, b, bx, by, bz, bb);
     const int st = plane_box_state(K, bb);
@@ -731,12 +731,13 @@ __global__ void k_plane_member(const int* __restrict__ list, const int* __restri
     if (t == 0) *seed_used = s;
 }
 
-// Field pass: one workgroup per brick of BX*BY*BZ cells.  Each wave covers
-// 64/BX whole x-rows of one z-layer of the brick; it skips, with scalar tests on
-// the solid's tile record (scalar-cache loads), the listed solids whose tile
-// misses its rows.
-template <class R, int MK, int BX, int BY, int BZ, bool HASP>
-__global__ void __launch_bounds__(BX* BY* BZ)
+// Field pass: one workgroup per brick of BX*BY*BZ cells, CPT cells per thread
+// (the thread's cells are CPT layers apart along z: lane (ti, tj) of layer tk
+// takes layers tk, tk + BZ/CPT, ...).  Each wave covers 64/BX whole x-rows of
+// its layers; it skips, with scalar tests on the solid's tile record
+// (scalar-cache loads), the listed solids whose tile misses its rows.
+template <class R, int MK, int BX, int BY, int BZ, int CPT, bool HASP>
+__global__ void __launch_bounds__(BX* BY* BZ / CPT)
     k_fields(MeshDev m, int nbx, int nby, const int* __restrict__ bcount, const int* __restrict__ blist,
              const SolidK* __restrict__ Ks, const TileInfo* __restrict__ Ts,
              const R* __restrict__ codes, long long code_cap, const int* __restrict__ planes, int nplanes,
@@ -745,33 +746,47 @@ __global__ void __launch_bounds__(BX* BY* BZ)
              const R* __restrict__ Uz, R* __restrict__ As, R* __restrict__ Fx, R* __restrict__ Fy,
              R* __restrict__ Fz, R* __restrict__ Tsf, int* __restrict__ Ct, double dt,
              double* __restrict__ plane_partial, long long nbricks, int* __restrict__ err) {
-    constexpr int NT = BX * BY * BZ;
-    static_assert(64 % BX == 0 && (BX * BY) % 64 == 0, "a wave covers whole x-rows of one layer");
+    constexpr int NT = BX * BY * BZ / CPT;
+    constexpr int LZ = BZ / CPT;  // z distance between a thread's cells
+    static_assert(BZ % CPT == 0 && 64 % BX == 0 && (BX * BY) % 64 == 0 && NT % 64 == 0,
+                  "a wave covers whole x-rows of its layers");
     __shared__ int raw[kMaxPerBrick];
     __shared__ int lst[kMaxPerBrick];
     __shared__ double red[6 * 16];
     __shared__ int pl_any[kMaxPlanes];
 
-    const long long b = blockIdx.x;
-    const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long long)nbx * nby));
+    // grid (nbx, nby, nbz): no integer division for the brick coordinates
+    const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    const long long b = bx + (long long)nbx * (by + (long long)nby * bz);
     const int tid = threadIdx.x;
     const int ti = tid % BX, tj = (tid / BX) % BY, tk = tid / (BX * BY);
-    const int i = bx * BX + ti, j = by * BY + tj, k = bz * BZ + tk;
-    const bool valid = i < m.nx && j < m.ny && k < m.nz;
+    const int i = bx * BX + ti, j = by * BY + tj;
+    int kq[CPT];
+    bool valid[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        kq[q] = bz * BZ + tk + q * LZ;
+        valid[q] = i < m.nx && j < m.ny && kq[q] < m.nz;
+    }
 
     // list entries loaded together with the count (the first 32, one cache line, speculatively)
     constexpr int kSpec = 32;
     const int* bl = blist + b * kMaxPerBrick;
     const int spec = tid < kSpec ? bl[tid] : 0;
     const int cnt = min(bcount[b], kMaxPerBrick);  // k_bin flagged an overflow
-    // the cell's velocity, centre and volume, needed if a listed solid or a plane takes it
-    d3 uf{0, 0, 0}, cc{0, 0, 0};
-    double V = 0.0;
-    if (valid && (cnt > 0 || (HASP && nplanes > 0))) {
-        const long long c = cell_id(m, i, j, k);
-        uf = {(double)Ux[c], (double)Uy[c], (double)Uz[c]};
-        cc = cell_centre<MK>(m, i, j, k);
-        V = cell_volume<MK>(m, i, j, k);
+    // the cells' velocity, centre and volume (used if a listed solid or a plane takes them)
+    d3 uf[CPT], cc[CPT];
+    double V[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        uf[q] = cc[q] = d3{0, 0, 0};
+        V[q] = 0.0;
+        if (valid[q]) {  // issued with the list loads, before the count is known
+            const long long c = cell_id(m, i, j, kq[q]);
+            uf[q] = {(double)Ux[c], (double)Uy[c], (double)Uz[c]};
+            cc[q] = cell_centre<MK>(m, i, j, kq[q]);
+            V[q] = cell_volume<MK>(m, i, j, kq[q]);
+        }
     }
     if (tid < min(cnt, kSpec)) raw[tid] = spec;
     for (int t = kSpec + tid; t < cnt; t += NT) raw[t] = bl[t];
@@ -784,7 +799,7 @@ __global__ void __launch_bounds__(BX* BY* BZ)
         lst[rank] = v;
     }
     __syncthreads();
-    // the wave's cell box (uniform)
+    // the wave's cell rows (uniform): x in [wi0, wi1], y in [wj0, wj1], layers wk + q LZ
     const int w0 = (tid >> 6) << 6;
     const int wk = bz * BZ + w0 / (BX * BY);
     const int wj0 = by * BY + (w0 % (BX * BY)) / BX, wj1 = wj0 + 64 / BX - 1;
@@ -792,11 +807,16 @@ __global__ void __launch_bounds__(BX* BY* BZ)
 
     // brick vertex box, for plane culling
     double bb[6];
-    brick_vertex_box<MK, BX, BY, BZ>(m, brick_box, b, bx, by, bz, bb);
+    if constexpr (HASP) brick_vertex_box<MK, BX, BY, BZ>(m, brick_box, b, bx, by, bz, bb);
 
     const double dtINV = 1.0 / dt;
-    double sA = 0.0, sT = 0.0, sF[3] = {0.0, 0.0, 0.0};
-    int ct = 0;
+    double sA[CPT], sT[CPT], sF[CPT][3];
+    int ct[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        sA[q] = sT[q] = sF[q][0] = sF[q][1] = sF[q][2] = 0.0;
+        ct[q] = 0;
+    }
     constexpr int NPACC = HASP ? kMaxPlanes : 1;
     double pacc[NPACC][6];
 #pragma unroll
@@ -804,33 +824,33 @@ __global__ void __launch_bounds__(BX* BY* BZ)
 #pragma unroll
         for (int t = 0; t < 6; ++t) pacc[p][t] = 0.0;
 
-    auto contribute = [&](const double* cvw, double alpha, int ctype, int plane_slot) {
-        const d3 r = {cc.x - cvw[0], cc.y - cvw[1], cc.z - cvw[2]};
+    auto contribute = [&](int q, const double* cvw, double alpha, int ctype, int plane_slot) {
+        const d3 r = {cc[q].x - cvw[0], cc[q].y - cvw[1], cc[q].z - cvw[2]};
         const d3 us = d3{cvw[3], cvw[4], cvw[5]} + cross(d3{cvw[6], cvw[7], cvw[8]}, r);
-        const d3 f = alpha * (uf - us);
-        sA += alpha;
-        sT += alpha;
-        sF[0] += f.x * dtINV;
-        sF[1] += f.y * dtINV;
-        sF[2] += f.z * dtINV;
-        ct = ctype;
+        const d3 f = alpha * (uf[q] - us);
+        sA[q] += alpha;
+        sT[q] += alpha;
+        sF[q][0] += f.x * dtINV;
+        sF[q][1] += f.y * dtINV;
+        sF[q][2] += f.z * dtINV;
+        ct[q] = ctype;
         if (HASP && plane_slot >= 0) {
             const d3 tq = cross(r, f);
             double* a = pacc[plane_slot];
-            a[0] += (f.x * V) * dtINV;
-            a[1] += (f.y * V) * dtINV;
-            a[2] += (f.z * V) * dtINV;
-            a[3] += (tq.x * V) * dtINV;
-            a[4] += (tq.y * V) * dtINV;
-            a[5] += (tq.z * V) * dtINV;
+            a[0] += (f.x * V[q]) * dtINV;
+            a[1] += (f.y * V[q]) * dtINV;
+            a[2] += (f.z * V[q]) * dtINV;
+            a[3] += (tq.x * V[q]) * dtINV;
+            a[4] += (tq.y * V[q]) * dtINV;
+            a[5] += (tq.z * V[q]) * dtINV;
         }
     };
 
     int ia = 0, ip = 0;
     if (!HASP) nplanes = 0;
     // codes of the next kBatch listed solids, loaded together (independent loads)
-    constexpr int kBatch = 8;
-    R cb[kBatch];
+    constexpr int kBatch = 8 / CPT;
+    R cb[kBatch][CPT];
     int cb0 = -kBatch;
     while (ia < cnt || ip < nplanes) {
         const int sa = __builtin_amdgcn_readfirstlane(ia < cnt ? lst[ia] : 0x7fffffff);
@@ -839,37 +859,51 @@ __global__ void __launch_bounds__(BX* BY* BZ)
             if (ia >= cb0 + kBatch) {
                 cb0 = ia;
 #pragma unroll
-                for (int q = 0; q < kBatch; ++q) {
-                    R code = code_none<R>();
-                    const int a = ia + q;
+                for (int u = 0; u < kBatch; ++u) {
+#pragma unroll
+                    for (int q = 0; q < CPT; ++q) cb[u][q] = code_none<R>();
+                    const int a = ia + u;
                     if (a < cnt) {
                         const int sq = __builtin_amdgcn_readfirstlane(lst[a]);
                         const TileInfo T = Ts[sq];  // uniform: scalar loads
                         const bool hit = T.lo[0] <= wi1 && T.lo[0] + T.n[0] > wi0 && T.lo[1] <= wj1 &&
-                                         T.lo[1] + T.n[1] > wj0 && T.lo[2] <= wk && T.lo[2] + T.n[2] > wk &&
+                                         T.lo[1] + T.n[1] > wj0 &&
                                          T.off + (long long)T.n[0] * T.n[1] * T.n[2] <= code_cap;
-                        if (hit) {
-                            const unsigned li = (unsigned)(i - T.lo[0]), lj = (unsigned)(j - T.lo[1]);
-                            if (valid && li < (unsigned)T.n[0] && lj < (unsigned)T.n[1])
-                                code = codes[T.off + (long long)(li + T.n[0] * (lj + T.n[1] * (k - T.lo[2])))];
+                        const unsigned li = (unsigned)(i - T.lo[0]), lj = (unsigned)(j - T.lo[1]);
+                        const bool in_xy = li < (unsigned)T.n[0] && lj < (unsigned)T.n[1];
+#pragma unroll
+                        for (int q = 0; q < CPT; ++q) {
+                            const int lk = wk + q * LZ - T.lo[2];
+                            if (hit && lk >= 0 && lk < T.n[2] && valid[q] && in_xy)
+                                cb[u][q] = codes[T.off + (long long)(li + T.n[0] * (lj + T.n[1] * lk))];
                         }
                     }
-                    cb[q] = code;
                 }
             }
-            R code = cb[0];
+            R code[CPT];
 #pragma unroll
-            for (int q = 1; q < kBatch; ++q)
-                if (ia - cb0 == q) code = cb[q];
+            for (int q = 0; q < CPT; ++q) {
+                code[q] = cb[0][q];
+#pragma unroll
+                for (int u = 1; u < kBatch; ++u)
+                    if (ia - cb0 == u) code[q] = cb[u][q];
+            }
             ++ia;
-            if (!__any(code == code)) continue;  // no candidate in the wave (NaN codes)
+            bool any = false;
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) any = any || code[q] == code[q];
+            if (!__any(any)) continue;  // no candidate in the wave (NaN codes)
             const SolidK& Ka = Ks[sa];
             const double cvw[9] = {Ka.c[0], Ka.c[1], Ka.c[2], Ka.v[0], Ka.v[1], Ka.v[2], Ka.w[0], Ka.w[1], Ka.w[2]};
-            if (code != code) continue;
-            if (isinf(code))
-                contribute(cvw, 1.0, sa + 4, -1);
-            else
-                contribute(cvw, (double)fabs(code), signbit(code) ? 3 : 2, -1);
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                const R cq = code[q];
+                if (cq != cq) continue;
+                if (isinf(cq))
+                    contribute(q, cvw, 1.0, sa + 4, -1);
+                else
+                    contribute(q, cvw, (double)fabs(cq), signbit(cq) ? 3 : 2, -1);
+            }
         } else if constexpr (HASP) {
             const int slot = ip;
             ++ip;
@@ -877,38 +911,45 @@ __global__ void __launch_bounds__(BX* BY* BZ)
             // plane: only cells of the seed's component (member bitmap); cull the brick first
             const int st = plane_box_state(K, bb);
             if (st == 0) continue;  // no vertex of the brick can be inside
-            if (!valid) continue;
-            const long long c = cell_id(m, i, j, k);
-            if (!bit_get(plane_member + (long long)slot * member_words, c)) continue;
-            int nin = 8;
-            if (st == 2) {
-                d3 P[8];
-                cell_vertices<MK>(m, i, j, k, P);
-                nin = 0;
-#pragma unroll
-                for (int v = 0; v < 8; ++v) nin += inside_world(K, P[v]) ? 1 : 0;
-            }
-            if (nin == 0) continue;
-            pl_any[slot] = 1;
             const double cvw[9] = {K.c[0], K.c[1], K.c[2], K.v[0], K.v[1], K.v[2], K.w[0], K.w[1], K.w[2]};
-            if (nin == 8) {
-                contribute(cvw, 1.0, sp + 4, slot);
-            } else {
-                const R alpha = cell_fraction<R, MK>(m, K, i, j, k);
-                const bool cin = inside_world(K, cell_centre<MK>(m, i, j, k));
-                contribute(cvw, (double)alpha, cin ? 2 : 3, slot);
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                if (!valid[q]) continue;
+                const int k = kq[q];
+                const long long c = cell_id(m, i, j, k);
+                if (!bit_get(plane_member + (long long)slot * member_words, c)) continue;
+                int nin = 8;
+                if (st == 2) {
+                    d3 P[8];
+                    cell_vertices<MK>(m, i, j, k, P);
+                    nin = 0;
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) nin += inside_world(K, P[v]) ? 1 : 0;
+                }
+                if (nin == 0) continue;
+                pl_any[slot] = 1;
+                if (nin == 8) {
+                    contribute(q, cvw, 1.0, sp + 4, slot);
+                } else {
+                    const R alpha = cell_fraction<R, MK>(m, K, i, j, k);
+                    const bool cin = inside_world(K, cell_centre<MK>(m, i, j, k));
+                    contribute(q, cvw, (double)alpha, cin ? 2 : 3, slot);
+                }
             }
         }
     }
 
-    if (valid) {
-        const long long c = cell_id(m, i, j, k);
-        As[c] = R(sA < 1.0 ? sA : 1.0);
-        Fx[c] = R(sF[0]);
-        Fy[c] = R(sF[1]);
-        Fz[c] = R(sF[2]);
-        Tsf[c] = R(sT);
-        Ct[c] = ct;
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        if (!valid[q]) continue;
+        const long long c = cell_id(m, i, j, kq[q]);
+        // streaming stores: the fields are not read again in this step
+        __builtin_nontemporal_store(R(sA[q] < 1.0 ? sA[q] : 1.0), &As[c]);
+        __builtin_nontemporal_store(R(sF[q][0]), &Fx[c]);
+        __builtin_nontemporal_store(R(sF[q][1]), &Fy[c]);
+        __builtin_nontemporal_store(R(sF[q][2]), &Fz[c]);
+        __builtin_nontemporal_store(R(sT[q]), &Tsf[c]);
+        __builtin_nontemporal_store(ct[q], &Ct[c]);
     }
     if (HASP && nplanes > 0) {
         __syncthreads();
@@ -1028,7 +1069,7 @@ __global__ void k_fields_to_f64(const R* __restrict__ As, const R* __restrict__ 
 // ===========================================================================
 // host side
 // ===========================================================================
-constexpr int kBX3 = 16, kBY3 = 4, kBZ3 = 4;  // brick of a 3-D mesh
+constexpr int kBX3 = 32, kBY3 = 4, kBZ3 = 2, kCPT3 = 1;  // brick of a 3-D mesh, cells per thread
 constexpr int kBX2 = 32, kBY2 = 8, kBZ2 = 1;  // brick of a one-cell-thick (2-D) mesh
 
 struct sdfibm_ctx {
@@ -1239,10 +1280,10 @@ int launch_step(sdfibm_ctx* c, double dt) {
         k_fill_i32<<<1, 64, 0, st>>>(c->d_pscalars, 1, 0);
         k_fill_i32<<<1, 64, 0, st>>>(c->d_pscalars + 1, 1, 0x7fffffff);
         if (c->nz == 1)
-            k_plane_active<MK, kBX2, kBY2, kBZ2><<<(unsigned)c->nbricks, kBX2 * kBY2 * kBZ2, 0, st>>>(
+            k_plane_active<MK, kBX2, kBY2, kBZ2><<<dim3(c->nbx, c->nby, c->nbz), kBX2 * kBY2 * kBZ2, 0, st>>>(
                 c->m, c->nbx, c->nby, c->brick_box, K, c->d_pact, c->d_plist, c->d_pscalars, c->N, c->d_pscalars + 1);
         else
-            k_plane_active<MK, kBX3, kBY3, kBZ3><<<(unsigned)c->nbricks, kBX3 * kBY3 * kBZ3, 0, st>>>(
+            k_plane_active<MK, kBX3, kBY3, kBZ3><<<dim3(c->nbx, c->nby, c->nbz), kBX3 * kBY3 * kBZ3, 0, st>>>(
                 c->m, c->nbx, c->nby, c->brick_box, K, c->d_pact, c->d_plist, c->d_pscalars, c->N, c->d_pscalars + 1);
         int nact = 0;
         if (hipMemcpyAsync(&nact, c->d_pscalars, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1265,22 +1306,22 @@ int launch_step(sdfibm_ctx* c, double dt) {
                                           c->d_pscalars + 1, member, c->d_pseed + p);
     }
     (void)hipEventRecord(c->ev[2], st);
-#define SDFIBM_FIELDS(BX_, BY_, BZ_, HP_)                                                                       \
-    k_fields<R, MK, BX_, BY_, BZ_, HP_><<<(unsigned)c->nbricks, BX_ * BY_ * BZ_, 0, st>>>(                          \
+#define SDFIBM_FIELDS(BX_, BY_, BZ_, CPT_, HP_)                                                                 \
+    k_fields<R, MK, BX_, BY_, BZ_, CPT_, HP_><<<dim3(c->nbx, c->nby, c->nbz), BX_ * BY_ * BZ_ / CPT_, 0, st>>>(     \
         c->m, c->nbx, c->nby, c->bcount, c->d_blist, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np,          \
         c->d_pmember, c->member_words, c->brick_box, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],      \
         (R*)c->As, (R*)c->Fs[0], (R*)c->Fs[1], (R*)c->Fs[2], (R*)c->Tsf, c->Ct, dt, c->d_plane_partial, c->nbricks, \
         c->err)
     if (c->nz == 1) {
         if (np > 0)
-            SDFIBM_FIELDS(kBX2, kBY2, kBZ2, true);
+            SDFIBM_FIELDS(kBX2, kBY2, kBZ2, 1, true);
         else
-            SDFIBM_FIELDS(kBX2, kBY2, kBZ2, false);
+            SDFIBM_FIELDS(kBX2, kBY2, kBZ2, 1, false);
     } else {
         if (np > 0)
-            SDFIBM_FIELDS(kBX3, kBY3, kBZ3, true);
+            SDFIBM_FIELDS(kBX3, kBY3, kBZ3, kCPT3, true);
         else
-            SDFIBM_FIELDS(kBX3, kBY3, kBZ3, false);
+            SDFIBM_FIELDS(kBX3, kBY3, kBZ3, kCPT3, false);
     }
 #undef SDFIBM_FIELDS
     if (np > 0) k_plane_reduce<<<np, 256, 0, st>>>(c->d_plane_partial, c->nbricks, c->d_planes, c->rho_f, c->d_ft);
@@ -1556,6 +1597,8 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
     c->nby = (c->ny + c->by - 1) / c->by;
     c->nbz = (c->nz + c->bz - 1) / c->bz;
     c->nbricks = (long long)c->nbx * c->nby * c->nbz;
+    if (c->nby > 65535 || c->nbz > 65535)
+        return bail(fail(c, SDFIBM_ERR_MESH, "mesh too large along y or z for the brick grid"));
     if ((rc = dalloc(c, (void**)&c->bcount, c->nbricks * 4))) return bail(rc);
     if ((rc = dalloc(c, (void**)&c->d_blist, c->nbricks * kMaxPerBrick * 4))) return bail(rc);
     if ((rc = dalloc(c, (void**)&c->err, 64))) return bail(rc);
