@@ -136,6 +136,32 @@ int sdfibm_update_velocity(sdfibm_ctx* ctx, double dt);
  * passed to sdfibm_set_solids. */
 int sdfibm_fix_internal(sdfibm_ctx* ctx);
 
+/* Per-solid material and motion constraint of the rigid-body update.  Replaces
+ * Solid::setMaterial (solid.h:94-103: the material's density) and the
+ * Motion01Mask plugin (libmotion/motion01mask.h:15-64). */
+typedef struct {
+    double rho;           /* solid density, > 0 */
+    int32_t motion_mask;  /* bits 0..5 = vx vy vz wx wy wz free, i.e. Motion01Mask "b<vx><vy><vz><wx><wy><wz>"
+                             with bit k = character k+1; -1 = no motion constraint */
+    int32_t reserved;
+} sdfibm_dynamics;
+
+/* Set the dynamics of every solid (one record per solid of the last
+ * set_solids); resets the first-step state of the fluid force mix. */
+int sdfibm_set_dynamics(sdfibm_ctx* ctx, const sdfibm_dynamics* dyn, int64_t n);
+
+/* SolidCloud::evolve(time, dt) (solidcloud.cpp:521-562) on the device: 20
+ * sub-steps (1 for a single solid) of AB2 fluid force from the last interact
+ * (solid.h:164-174), buoyant gravity ((rho_s - rho_f)/rho_s g,
+ * solidcloud.cpp:466-475) and Solid::move (solid.h:182-202: velocity, world
+ * inverse inertia, motion mask, trapezoidal position, quaternion update and
+ * normalisation).  gravity: [3] or NULL.  The solids stay on the device; the
+ * next interact and fix_internal use the moved state. */
+int sdfibm_evolve(sdfibm_ctx* ctx, double time, double dt, const double* gravity);
+
+/* Device -> host copy of the solids' current records (after evolve). */
+int sdfibm_get_solids(sdfibm_ctx* ctx, sdfibm_solid* out, int64_t n);
+
 /* Device -> host copy of U as AoS FP64 [N*3]. */
 int sdfibm_get_velocity(sdfibm_ctx* ctx, double* U_aos);
 

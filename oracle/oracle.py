@@ -61,6 +61,8 @@ def lib():
                                          P, P, P, P, P, C.c_int]
         L.orc_update_velocity.argtypes = [i64, P, P, C.c_double]
         L.orc_fix_internal.argtypes = [P, P, P, i64, P]
+        L.orc_mass_properties.argtypes = [P, P, P]
+        L.orc_evolve.argtypes = [i64, P, P, P, C.c_double, C.c_double, P, C.c_double]
         del dp
         _lib = L
     return _lib
@@ -209,3 +211,35 @@ def world_to_local(solid, p):
     out = np.zeros(3)
     lib().orc_world_to_local(_ptr(s), _ptr(p), _ptr(out))
     return out
+
+
+# Rigid-body state per solid beyond the solid record (orc_dyn in sdfibm_oracle.h).
+DYN_DTYPE = np.dtype([("rho", "f8"), ("mask", "i4"), ("first_fluid", "i4"), ("ff_old", "f8", 3),
+                      ("ft_old", "f8", 3)])
+
+
+def make_dyn(n, rho, mask=-1):
+    d = np.zeros(n, dtype=DYN_DTYPE)
+    d["rho"] = rho
+    d["mask"] = mask
+    d["first_fluid"] = 1
+    return d
+
+
+def mass_properties(solid):
+    """(volume, principal moments per unit density) of one solid record."""
+    s = np.ascontiguousarray(solid)
+    v = np.zeros(1)
+    moi = np.zeros(3)
+    lib().orc_mass_properties(_ptr(s), _ptr(v), _ptr(moi))
+    return float(v[0]), moi
+
+
+def evolve(solids, dyn, fluid_ft, time, dt, gravity, rho_f):
+    """SolidCloud::evolve on copies: returns (solids, dyn) after one step."""
+    s = np.ascontiguousarray(solids).copy()
+    d = np.ascontiguousarray(dyn, dtype=DYN_DTYPE).copy()
+    ft = np.ascontiguousarray(fluid_ft, dtype=np.float64).reshape(-1, 6)
+    g = np.ascontiguousarray(gravity, dtype=np.float64)
+    lib().orc_evolve(s.size, _ptr(s), _ptr(d), _ptr(ft), float(time), float(dt), _ptr(g), float(rho_f))
+    return s, d

@@ -1054,4 +1054,167 @@ void orc_fix_internal(const orc_mesh* om, double* U, const double* Ct, int64_t n
     }
 }
 
+// ---------------------------------------------------------------------------
+// rigid-body update
+// ---------------------------------------------------------------------------
+void orc_mass_properties(const orc_solid* sp, double* volume, double* moi3) {
+    const orc_solid& s = *sp;
+    const double PI = 3.14159265358979323846;
+    double V = 0.0, m0 = 1.0, m1 = 1.0, m2 = 1.0;  // plane: m_volume 0, m_moi = I (ishape.h:33-36)
+    switch (s.shape) {
+    case ORC_SPHERE: {  // sphere.h:21-25
+        const double r = s.prm[0], r2 = r * r;
+        V = 4.0 / 3.0 * PI * r2 * r;
+        m0 = m1 = m2 = 0.4 * V * r2;
+        break;
+    }
+    case ORC_ELLIPSOID: {  // ellipsoid.h:25-30
+        const double a = s.prm[0], b = s.prm[1], c = s.prm[2];
+        V = 4.0 / 3.0 * PI * a * b * c;
+        m0 = 0.2 * V * (b * b + c * c);
+        m1 = 0.2 * V * (a * a + c * c);
+        m2 = 0.2 * V * (a * a + b * b);
+        break;
+    }
+    case ORC_CIRCLE:
+    case ORC_CIRCLE_TAIL:
+    case ORC_CIRCLE_TWOTAIL: {  // circle.h:22-25, circle_tail.h:28-31
+        const double r = s.prm[0], r2 = r * r;
+        V = PI * r2;
+        m0 = m1 = m2 = 0.5 * V * r2;
+        break;
+    }
+    case ORC_ELLIPSE: {  // ellipse.h:24-28
+        const double a = s.prm[0], b = s.prm[1];
+        V = PI * a * b;
+        m0 = m1 = m2 = 0.25 * V * (a * a + b * b);
+        break;
+    }
+    case ORC_BOX: {  // box.h:22-27
+        const double a = s.prm[0], b = s.prm[1], c = s.prm[2];
+        V = 8.0 * a * b * c;
+        m0 = 1.0 / 3.0 * V * (b * b + c * c);
+        m1 = 1.0 / 3.0 * V * (a * a + c * c);
+        m2 = 1.0 / 3.0 * V * (b * b + a * a);
+        break;
+    }
+    case ORC_RECTANGLE: {  // rectangle.h:25-29
+        const double a = s.prm[0], b = s.prm[1];
+        V = 4.0 * a * b;
+        m0 = m1 = m2 = 1.0 / 3.0 * V * (a * a + b * b);
+        break;
+    }
+    default: break;
+    }
+    *volume = V;
+    moi3[0] = m0;
+    moi3[1] = m1;
+    moi3[2] = m2;
+}
+
+namespace {
+struct Quat {
+    double w;
+    V3 v;
+};
+// Hamilton product, OpenFOAM's quaternion operator*: (w1 w2 - v1.v2, w1 v2 + w2 v1 + v1 x v2)
+Quat qmul(const Quat& a, const Quat& b) {
+    return {a.w * b.w - dot(a.v, b.v), (a.w * b.v + b.w * a.v) + cross(a.v, b.v)};
+}
+// quaternion::R(): rotation tensor, row-major
+void qR(const Quat& q, double R[9]) {
+    const double w2 = q.w * q.w, x2 = q.v.x * q.v.x, y2 = q.v.y * q.v.y, z2 = q.v.z * q.v.z;
+    const double txy = 2 * q.v.x * q.v.y, twz = 2 * q.w * q.v.z, txz = 2 * q.v.x * q.v.z;
+    const double twy = 2 * q.w * q.v.y, tyz = 2 * q.v.y * q.v.z, twx = 2 * q.w * q.v.x;
+    R[0] = w2 + x2 - y2 - z2;
+    R[1] = txy - twz;
+    R[2] = txz + twy;
+    R[3] = txy + twz;
+    R[4] = w2 - x2 + y2 - z2;
+    R[5] = tyz - twx;
+    R[6] = txz - twy;
+    R[7] = tyz + twx;
+    R[8] = w2 - x2 - y2 + z2;
+}
+void tmul(const double* A, const double* B, double* C) {  // C = A & B
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+}  // namespace
+
+void orc_evolve(int64_t ns, orc_solid* solids, orc_dyn* dyn, const double* fluid_ft, double time, double dt,
+                const double* gravity, double rho_f) {
+    (void)time;
+    const int n_sub = ns == 1 ? 1 : 20;  // solidcloud.cpp:524-526
+    const double dts = dt / n_sub;
+    const V3 g{gravity[0], gravity[1], gravity[2]};
+    for (int64_t id = 0; id < ns; ++id) {
+        orc_solid& s = solids[id];
+        orc_dyn& d = dyn[id];
+        double vol, moi[3];
+        orc_mass_properties(&s, &vol, moi);
+        // Solid::setMaterial (solid.h:94-103): mass = V rho, mass_inv = (1/V)/rho, moi_inv = inv(moi)/rho
+        const double volinv = vol > 0 ? 1.0 / vol : 0.0;
+        const double mass = vol * d.rho, mass_inv = volinv / d.rho;
+        double Minv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int a = 0; a < 3; ++a) Minv[4 * a] = (1.0 / moi[a]) / d.rho;
+        const V3 ff{fluid_ft[6 * id], fluid_ft[6 * id + 1], fluid_ft[6 * id + 2]};
+        const V3 ft{fluid_ft[6 * id + 3], fluid_ft[6 * id + 4], fluid_ft[6 * id + 5]};
+        for (int it = 0; it < n_sub; ++it) {
+            // clear, AB2 fluid force (first step: old = current), buoyant gravity
+            if (d.first_fluid) {
+                for (int a = 0; a < 3; ++a) {
+                    d.ff_old[a] = (&ff.x)[a];
+                    d.ft_old[a] = (&ft.x)[a];
+                }
+                d.first_fluid = 0;
+            }
+            const V3 ffo{d.ff_old[0], d.ff_old[1], d.ff_old[2]}, fto{d.ft_old[0], d.ft_old[1], d.ft_old[2]};
+            V3 force = V3{0, 0, 0} + (1.5 * ff - 0.5 * ffo);
+            const V3 torque = V3{0, 0, 0} + (1.5 * ft - 0.5 * fto);
+            const V3 gprime = ((d.rho - rho_f) / d.rho) * g;
+            force = force + mass * gprime;
+            // Solid::move
+            V3 v{s.vel[0], s.vel[1], s.vel[2]}, w{s.omega[0], s.omega[1], s.omega[2]};
+            const V3 v_old = v, w_old = w;
+            v = v + (force * mass_inv) * dts;
+            Quat q{s.q[0], {s.q[1], s.q[2], s.q[3]}};
+            double R[9], RT[9], T1[9], Mw[9];
+            qR(q, R);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) RT[3 * i + j] = R[3 * j + i];
+            tmul(R, Minv, T1);
+            tmul(T1, RT, Mw);
+            const V3 aw{Mw[0] * torque.x + Mw[1] * torque.y + Mw[2] * torque.z,
+                        Mw[3] * torque.x + Mw[4] * torque.y + Mw[5] * torque.z,
+                        Mw[6] * torque.x + Mw[7] * torque.y + Mw[8] * torque.z};
+            w = w + aw * dts;
+            if (d.mask >= 0) {  // Motion01Mask::constraint: cmptMultiply with 0/1 masks
+                v = {v.x * ((d.mask >> 0) & 1), v.y * ((d.mask >> 1) & 1), v.z * ((d.mask >> 2) & 1)};
+                w = {w.x * ((d.mask >> 3) & 1), w.y * ((d.mask >> 4) & 1), w.z * ((d.mask >> 5) & 1)};
+            }
+            const V3 c = V3{s.center[0], s.center[1], s.center[2]} + (0.5 * (v + v_old)) * dts;
+            const Quat hw{0.5 * 0.0, 0.5 * (0.5 * (w + w_old))};  // 0.5 * quaternion(0.5 (w + w_old))
+            const Quat dq = qmul(hw, q);
+            q = {q.w + dq.w * dts, q.v + dq.v * dts};
+            const double qm = std::sqrt(q.w * q.w + magSqr(q.v));
+            q = {q.w / qm, q.v / qm};
+            for (int a = 0; a < 3; ++a) {
+                s.vel[a] = (&v.x)[a];
+                s.omega[a] = (&w.x)[a];
+                s.center[a] = (&c.x)[a];
+            }
+            s.q[0] = q.w;
+            s.q[1] = q.v.x;
+            s.q[2] = q.v.y;
+            s.q[3] = q.v.z;
+        }
+        // storeOldForce (solid.h:143-149)
+        for (int a = 0; a < 3; ++a) {
+            d.ff_old[a] = (&ff.x)[a];
+            d.ft_old[a] = (&ft.x)[a];
+        }
+    }
+}
+
 }  // extern "C"

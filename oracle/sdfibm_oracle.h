@@ -113,6 +113,31 @@ int orc_interact_range(const orc_mesh* m, int two_d, double rho_f, double dt,
 void orc_update_velocity(int64_t n, double* U, const double* Fs, double dt);
 void orc_fix_internal(const orc_mesh* m, double* U, const double* Ct, int64_t ns, const orc_solid* solids);
 
+/* Rigid-body state that SolidCloud::evolve carries per solid beyond orc_solid
+ * (solid.h:28-55): density, Motion01Mask bits (bit 0..5 = vx vy vz wx wy wz
+ * free; -1 = no motion constraint), the first-fluid-step flag and the previous
+ * step's fluid force and torque. */
+typedef struct {
+    double rho;
+    int32_t mask;
+    int32_t first_fluid;
+    double ff_old[3];
+    double ft_old[3];
+} orc_dyn;
+
+/* Shape volume and principal moments of inertia per unit density (the
+ * m_volume / m_moi of each shape constructor; plane: 0 and identity). */
+void orc_mass_properties(const orc_solid* s, double* volume, double* moi3);
+
+/* SolidCloud::evolve(time, dt) (solidcloud.cpp:521-562): n_sub sub-steps of
+ * clear -> AB2 fluid force (solid.h:164-174) -> buoyant gravity
+ * (solidcloud.cpp:466-475) -> Solid::move (solid.h:182-202), then the fluid
+ * force is stored as old.  fluid_ft: [ns*6] from the last interact (x rho_f).
+ * Forcers and collisions are not part of this restatement (the reference's
+ * collision grid is built with a negative bounding radius, solidcloud.cpp:74-75). */
+void orc_evolve(int64_t ns, orc_solid* solids, orc_dyn* dyn, const double* fluid_ft, double time, double dt,
+                const double* gravity, double rho_f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,7 @@ EXPORTS = (
     "sdfibm_interact", "sdfibm_get_fields", "sdfibm_field_pointers", "sdfibm_candidates",
     "sdfibm_last_timing", "sdfibm_set_stream", "sdfibm_last_error", "sdfibm_status_string",
     "sdfibm_abi_version", "sdfibm_update_velocity", "sdfibm_fix_internal", "sdfibm_get_velocity",
+    "sdfibm_set_dynamics", "sdfibm_evolve", "sdfibm_get_solids",
 )
 
 SDFIBM_OK = 0
@@ -73,6 +74,9 @@ def load():
         "sdfibm_update_velocity": (i32, [P, d]),
         "sdfibm_fix_internal": (i32, [P]),
         "sdfibm_get_velocity": (i32, [P, P]),
+        "sdfibm_set_dynamics": (i32, [P, P, i64]),
+        "sdfibm_evolve": (i32, [P, d, d, P]),
+        "sdfibm_get_solids": (i32, [P, P, i64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

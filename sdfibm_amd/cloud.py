@@ -17,7 +17,18 @@ from . import _lib
 from .mesh import StructuredHexMesh
 from .solids import SOLID_DTYPE
 
-__all__ = ["SolidCloud"]
+__all__ = ["SolidCloud", "DYNAMICS_DTYPE", "motion_mask"]
+
+# sdfibm_dynamics (include/sdfibm_hip.h)
+DYNAMICS_DTYPE = np.dtype([("rho", "f8"), ("motion_mask", "i4"), ("reserved", "i4")])
+
+
+def motion_mask(mask: str) -> int:
+    """Motion01Mask string "b<vx><vy><vz><wx><wy><wz>" -> bits (libmotion/motion01mask.h:27-52:
+    must start with 'b', have 7 characters, each of the six 0 or 1)."""
+    if len(mask) != 7 or mask[0] != "b" or any(ch not in "01" for ch in mask[1:]):
+        raise ValueError(f"bad motion mask {mask!r}: expected b followed by six 0/1 characters")
+    return sum(1 << k for k in range(6) if mask[k + 1] == "1")
 
 
 def _p(a):
@@ -120,6 +131,23 @@ class SolidCloud:
     def fix_internal(self) -> None:
         """SolidCloud::fixInternal: U = rigid velocity in cells wholly inside a solid."""
         _lib.check(_lib.load().sdfibm_fix_internal(self._h), self._h)
+
+    # -- rigid-body update (SolidCloud::evolve) ---------------------------------------------
+    def set_dynamics(self, rho, motion_mask=-1):
+        """Solid densities and Motion01Mask bits (bit k = character k+1 of "b......"; -1 = free)."""
+        d = np.zeros(self.n_solids, dtype=DYNAMICS_DTYPE)
+        d["rho"] = rho
+        d["motion_mask"] = motion_mask
+        _lib.check(_lib.load().sdfibm_set_dynamics(self._h, _p(d), d.size), self._h)
+
+    def evolve(self, time: float, dt: float, gravity=(0.0, 0.0, 0.0)) -> None:
+        g = np.ascontiguousarray(gravity, dtype=np.float64)
+        _lib.check(_lib.load().sdfibm_evolve(self._h, float(time), float(dt), _p(g)), self._h)
+
+    def solids(self) -> np.ndarray:
+        out = np.zeros(self.n_solids, dtype=SOLID_DTYPE)
+        _lib.check(_lib.load().sdfibm_get_solids(self._h, _p(out), out.size), self._h)
+        return out
 
     def velocity(self) -> np.ndarray:
         U = np.zeros((self.n_cells, 3))

@@ -1,0 +1,146 @@
+// dynamics.h -- rigid-body update of the solids (SolidCloud::evolve,
+// src/solidcloud.cpp:521-562; Solid::move, src/solid.h:182-202), FP64, with
+// the reference's operation order: OpenFOAM vector/tensor/quaternion algebra
+// written out (quaternion product, quaternion::R(), tensor & tensor).
+#pragma once
+
+#include "shapes.h"
+
+namespace sdfibm_dev {
+
+// Per-solid rigid-body state on the device beyond the solid record.
+struct SolidDyn {
+    double rho;         // solid density (IMaterial::getRho)
+    int mask;           // Motion01Mask bits 0..5 = vx vy vz wx wy wz free; -1 = unconstrained
+    int first_fluid;    // Solid::first_fluid_step
+    double ff_old[3];   // fluid force and torque of the previous step (storeOldForce)
+    double ft_old[3];
+};
+
+// Shape volume and principal moments of inertia per unit density: the m_volume
+// and diagonal m_moi each shape constructor sets (plane: 0 and the identity).
+__host__ __device__ inline void mass_properties(const sdfibm_solid& s, double& vol, double moi[3]) {
+    const double PI = 3.14159265358979323846;
+    vol = 0.0;
+    moi[0] = moi[1] = moi[2] = 1.0;
+    switch (s.shape) {
+    case SDFIBM_SHAPE_SPHERE: {  // sphere.h:21-25
+        const double r = s.prm[0], r2 = r * r;
+        vol = 4.0 / 3.0 * PI * r2 * r;
+        moi[0] = moi[1] = moi[2] = 0.4 * vol * r2;
+        break;
+    }
+    case SDFIBM_SHAPE_ELLIPSOID: {  // ellipsoid.h:25-30
+        const double a = s.prm[0], b = s.prm[1], c = s.prm[2];
+        vol = 4.0 / 3.0 * PI * a * b * c;
+        moi[0] = 0.2 * vol * (b * b + c * c);
+        moi[1] = 0.2 * vol * (a * a + c * c);
+        moi[2] = 0.2 * vol * (a * a + b * b);
+        break;
+    }
+    case SDFIBM_SHAPE_CIRCLE:
+    case SDFIBM_SHAPE_CIRCLE_TAIL:
+    case SDFIBM_SHAPE_CIRCLE_TWOTAIL: {  // circle.h:22-25; the tails do not count
+        const double r = s.prm[0], r2 = r * r;
+        vol = PI * r2;
+        moi[0] = moi[1] = moi[2] = 0.5 * vol * r2;
+        break;
+    }
+    case SDFIBM_SHAPE_ELLIPSE: {  // ellipse.h:24-28
+        const double a = s.prm[0], b = s.prm[1];
+        vol = PI * a * b;
+        moi[0] = moi[1] = moi[2] = 0.25 * vol * (a * a + b * b);
+        break;
+    }
+    case SDFIBM_SHAPE_BOX: {  // box.h:22-27
+        const double a = s.prm[0], b = s.prm[1], c = s.prm[2];
+        vol = 8.0 * a * b * c;
+        moi[0] = 1.0 / 3.0 * vol * (b * b + c * c);
+        moi[1] = 1.0 / 3.0 * vol * (a * a + c * c);
+        moi[2] = 1.0 / 3.0 * vol * (b * b + a * a);
+        break;
+    }
+    case SDFIBM_SHAPE_RECTANGLE: {  // rectangle.h:25-29
+        const double a = s.prm[0], b = s.prm[1];
+        vol = 4.0 * a * b;
+        moi[0] = moi[1] = moi[2] = 1.0 / 3.0 * vol * (a * a + b * b);
+        break;
+    }
+    default: break;
+    }
+}
+
+// One SolidCloud::evolve of one solid: n_sub sub-steps of AB2 fluid force +
+// buoyant gravity + Solid::move, then the fluid force is stored as old.
+// ff, ft: this step's fluid force and torque (x rho_f); g: gravity.
+__host__ __device__ inline void evolve_solid(sdfibm_solid& s, SolidDyn& d, const double* fft, int n_sub, double dt,
+                                             const double* g, double rho_f) {
+    const double dts = dt / n_sub;
+    double vol, moi[3];
+    mass_properties(s, vol, moi);
+    const double volinv = vol > 0 ? 1.0 / vol : 0.0;
+    const double mass = vol * d.rho, mass_inv = volinv / d.rho;
+    const double Mi[3] = {(1.0 / moi[0]) / d.rho, (1.0 / moi[1]) / d.rho, (1.0 / moi[2]) / d.rho};
+    const d3 ff{fft[0], fft[1], fft[2]}, ft{fft[3], fft[4], fft[5]};
+    const d3 gv{g[0], g[1], g[2]};
+    for (int it = 0; it < n_sub; ++it) {
+        if (d.first_fluid) {  // Solid::addMidFluidForceAndTorque, first call
+            d.ff_old[0] = ff.x, d.ff_old[1] = ff.y, d.ff_old[2] = ff.z;
+            d.ft_old[0] = ft.x, d.ft_old[1] = ft.y, d.ft_old[2] = ft.z;
+            d.first_fluid = 0;
+        }
+        const d3 ffo{d.ff_old[0], d.ff_old[1], d.ff_old[2]}, fto{d.ft_old[0], d.ft_old[1], d.ft_old[2]};
+        d3 force = d3{0, 0, 0} + (1.5 * ff - 0.5 * ffo);
+        const d3 torque = d3{0, 0, 0} + (1.5 * ft - 0.5 * fto);
+        const d3 gp = ((d.rho - rho_f) / d.rho) * gv;  // SolidCloud::addMidEnvironment
+        force = force + mass * gp;
+        // Solid::move
+        d3 v{s.vel[0], s.vel[1], s.vel[2]}, w{s.omega[0], s.omega[1], s.omega[2]};
+        const d3 v_old = v, w_old = w;
+        v = v + dts * (mass_inv * force);
+        const double qw = s.q[0];
+        const d3 qv{s.q[1], s.q[2], s.q[3]};
+        // quaternion::R()
+        const double w2 = qw * qw, x2 = qv.x * qv.x, y2 = qv.y * qv.y, z2 = qv.z * qv.z;
+        const double txy = 2 * qv.x * qv.y, twz = 2 * qw * qv.z, txz = 2 * qv.x * qv.z;
+        const double twy = 2 * qw * qv.y, tyz = 2 * qv.y * qv.z, twx = 2 * qw * qv.x;
+        const double R[9] = {w2 + x2 - y2 - z2, txy - twz, txz + twy, txy + twz, w2 - x2 + y2 - z2,
+                             tyz - twx,         txz - twy, tyz + twx, w2 - x2 - y2 + z2};
+        // (R & diag(Mi)) & R^T, the tensor products written out in OpenFOAM's summation order
+        double T1[9], Mw[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                T1[3 * i + j] = R[3 * i] * (j == 0 ? Mi[0] : 0.0) + R[3 * i + 1] * (j == 1 ? Mi[1] : 0.0) +
+                                R[3 * i + 2] * (j == 2 ? Mi[2] : 0.0);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                Mw[3 * i + j] = T1[3 * i] * R[3 * j] + T1[3 * i + 1] * R[3 * j + 1] + T1[3 * i + 2] * R[3 * j + 2];
+        const d3 aw{Mw[0] * torque.x + Mw[1] * torque.y + Mw[2] * torque.z,
+                    Mw[3] * torque.x + Mw[4] * torque.y + Mw[5] * torque.z,
+                    Mw[6] * torque.x + Mw[7] * torque.y + Mw[8] * torque.z};
+        w = w + dts * aw;
+        if (d.mask >= 0) {  // Motion01Mask::constraint (motion01mask.h:60-64)
+            v = {v.x * ((d.mask >> 0) & 1), v.y * ((d.mask >> 1) & 1), v.z * ((d.mask >> 2) & 1)};
+            w = {w.x * ((d.mask >> 3) & 1), w.y * ((d.mask >> 4) & 1), w.z * ((d.mask >> 5) & 1)};
+        }
+        const d3 c = d3{s.center[0], s.center[1], s.center[2]} + dts * (0.5 * (v + v_old));
+        // q += (0.5 quaternion(0.5 (w + w_old)) * q) dt, then normalise
+        const d3 hv = 0.5 * (0.5 * (w + w_old));
+        const double hw = 0.5 * 0.0;
+        const double dw = hw * qw - dot(hv, qv);
+        const d3 dv = (hw * qv + qw * hv) + cross(hv, qv);
+        double nw = qw + dw * dts;
+        d3 nv = qv + dts * dv;
+        const double qm = sqrt(nw * nw + (nv.x * nv.x + nv.y * nv.y + nv.z * nv.z));
+        nw = nw / qm;
+        nv = {nv.x / qm, nv.y / qm, nv.z / qm};
+        s.vel[0] = v.x, s.vel[1] = v.y, s.vel[2] = v.z;
+        s.omega[0] = w.x, s.omega[1] = w.y, s.omega[2] = w.z;
+        s.center[0] = c.x, s.center[1] = c.y, s.center[2] = c.z;
+        s.q[0] = nw, s.q[1] = nv.x, s.q[2] = nv.y, s.q[3] = nv.z;
+    }
+    d.ff_old[0] = ff.x, d.ff_old[1] = ff.y, d.ff_old[2] = ff.z;  // Solid::storeOldForce
+    d.ft_old[0] = ft.x, d.ft_old[1] = ft.y, d.ft_old[2] = ft.z;
+}
+
+}  // namespace sdfibm_dev

@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "../../include/sdfibm_hip.h"
+#include "dynamics.h"
 #include "mesh_dev.h"
 
 using namespace sdfibm_dev;
@@ -639,9 +640,10 @@ __device__ inline bool bit_get(const unsigned* bits, long long c) { return (bits
 // Cells of the mesh with >= 1 inside vertex of a plane: bitmap, list, lowest index.
 template <int MK, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(BX* BY* BZ)
-    k_plane_active(MeshDev m, int nbx, int nby, const double* __restrict__ brick_box, SolidK K,
-                   unsigned* __restrict__ act, int* __restrict__ list, int* __restrict__ nlist, long long cap,
+    k_plane_active(MeshDev m, int nbx, int nby, const double* __restrict__ brick_box, const SolidK* __restrict__ Ks,
+                   int sp, unsigned* __restrict__ act, int* __restrict__ list, int* __restrict__ nlist, long long cap,
                    int* __restrict__ minidx) {
+    const SolidK K = Ks[sp];
     const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     const long long b = bx + (long long)nbx * (by + (long long)nby * bz);
     double bb[6];
@@ -984,7 +986,9 @@ __global__ void k_plane_reduce(const double* __restrict__ partial, long long nbr
 
 // Candidate codes of a plane over the whole mesh (parity / diagnostics only).
 template <class R, int MK>
-__global__ void k_plane_codes(MeshDev m, SolidK K, const unsigned* __restrict__ member, R* __restrict__ out) {
+__global__ void k_plane_codes(MeshDev m, const SolidK* __restrict__ Ks, int sp, const unsigned* __restrict__ member,
+                              R* __restrict__ out) {
+    const SolidK K = Ks[sp];
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long N = (long long)m.nx * m.ny * m.nz;
     if (c >= N) return;
@@ -1037,6 +1041,21 @@ __global__ void k_fix_internal(MeshDev m, const int* __restrict__ Ct, const sdfi
     Ux[c] = R(u.x);
     Uy[c] = R(u.y);
     Uz[c] = R(u.z);
+}
+
+// SolidCloud::evolve (solidcloud.cpp:521-562), one thread per solid.
+struct Vec3 {
+    double v[3];
+};
+__global__ void k_evolve(sdfibm_solid* __restrict__ S, SolidDyn* __restrict__ D, const double* __restrict__ ft,
+                         int ns, int n_sub, double dt, Vec3 g, double rho_f) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    sdfibm_solid sol = S[s];
+    SolidDyn d = D[s];
+    evolve_solid(sol, d, ft + 6LL * s, n_sub, dt, g.v, rho_f);
+    S[s] = sol;
+    D[s] = d;
 }
 
 template <class R>
@@ -1118,6 +1137,9 @@ struct sdfibm_ctx {
     int2* d_ncand = nullptr;  // per solid: ALL_INSIDE and cut candidates
     int* d_blist = nullptr;  // brick lists, kMaxPerBrick slots per brick
     double* d_ft = nullptr;
+    SolidDyn* d_dyn = nullptr;  // rigid-body state (set_dynamics)
+    long long dyn_n = -1;       // solids the dynamics were set for
+    long long dyn_cap = 0;
     std::vector<int> h_planes;
     int* d_planes = nullptr;
     double* d_plane_partial = nullptr;
@@ -1273,7 +1295,6 @@ int launch_step(sdfibm_ctx* c, double dt) {
     const int np = (int)c->h_planes.size();
     for (int p = 0; p < np; ++p) {
         const int sp = c->h_planes[p];
-        const SolidK K = make_solid_k(c->h_solids[sp]);
         unsigned* member = c->d_pmember + (long long)p * c->member_words;
         k_fill_i32<<<grid_for(c->member_words, 256), 256, 0, st>>>((int*)c->d_pact, c->member_words, 0);
         k_fill_i32<<<grid_for(c->member_words, 256), 256, 0, st>>>((int*)member, c->member_words, 0);
@@ -1281,10 +1302,12 @@ int launch_step(sdfibm_ctx* c, double dt) {
         k_fill_i32<<<1, 64, 0, st>>>(c->d_pscalars + 1, 1, 0x7fffffff);
         if (c->nz == 1)
             k_plane_active<MK, kBX2, kBY2, kBZ2><<<dim3(c->nbx, c->nby, c->nbz), kBX2 * kBY2 * kBZ2, 0, st>>>(
-                c->m, c->nbx, c->nby, c->brick_box, K, c->d_pact, c->d_plist, c->d_pscalars, c->N, c->d_pscalars + 1);
+                c->m, c->nbx, c->nby, c->brick_box, c->d_K, sp, c->d_pact, c->d_plist, c->d_pscalars, c->N,
+                c->d_pscalars + 1);
         else
             k_plane_active<MK, kBX3, kBY3, kBZ3><<<dim3(c->nbx, c->nby, c->nbz), kBX3 * kBY3 * kBZ3, 0, st>>>(
-                c->m, c->nbx, c->nby, c->brick_box, K, c->d_pact, c->d_plist, c->d_pscalars, c->N, c->d_pscalars + 1);
+                c->m, c->nbx, c->nby, c->brick_box, c->d_K, sp, c->d_pact, c->d_plist, c->d_pscalars, c->N,
+                c->d_pscalars + 1);
         int nact = 0;
         if (hipMemcpyAsync(&nact, c->d_pscalars, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
@@ -1727,6 +1750,8 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     }
     c->h_planes = planes;
     c->h_solids.assign(solids, solids + n);
+    if (n > 0) HIPCHK(c, hipMemsetAsync(c->d_ft, 0, n * 48, c->stream));  // evolve before any interact: no fluid force
+    if (c->dyn_n != n) c->dyn_n = -1;
     if (n > 0)
         HIPCHK(c, hipMemcpyAsync(c->d_solids, solids, n * sizeof(sdfibm_solid), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1767,6 +1792,24 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     if (rc) return rc;
     int flags = 0;
     HIPCHK(c, hipMemcpyAsync(&flags, c->err, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((flags & EF_CODECAP) && !(flags & EF_TILE) && c->ns > 0) {
+        // the solids moved or turned since set_solids and their tiles outgrew the code
+        // buffer: grow it to the size this step needs (with headroom) and run the step again
+        long long need = 0;
+        HIPCHK(c, hipMemcpy(&need, c->d_tile_off + c->ns, 8, hipMemcpyDeviceToHost));
+        const long long want = need + need / 4 + 1024;
+        dfree(c, c->d_codes);
+        dfree(c, c->d_cand);
+        c->d_codes = nullptr;
+        c->d_cand = nullptr;
+        c->code_cap = 0;
+        if ((rc = dalloc(c, &c->d_codes, want * rsize(c)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_cand, want * 4))) return rc;
+        c->code_cap = want;
+        if ((rc = dispatch_interact(c, dt))) return rc;
+        HIPCHK(c, hipMemcpyAsync(&flags, c->err, 4, hipMemcpyDeviceToHost, c->stream));
+    }
     if (ft && c->ns > 0) HIPCHK(c, hipMemcpyAsync(ft, c->d_ft, c->ns * 48, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms[5] = {0, 0, 0, 0, 0};
@@ -1830,6 +1873,62 @@ int sdfibm_fix_internal(sdfibm_ctx* c) {
 #undef SDFIBM_FIX
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SDFIBM_OK;
+}
+
+int sdfibm_set_dynamics(sdfibm_ctx* c, const sdfibm_dynamics* dyn, int64_t n) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "set_dynamics before set_solids");
+    if (n != c->ns || (n > 0 && !dyn)) return fail(c, SDFIBM_ERR_ARG, "need one dynamics record per solid");
+    std::vector<SolidDyn> h(n);
+    for (int64_t s = 0; s < n; ++s) {
+        if (!(dyn[s].rho > 0) || !std::isfinite(dyn[s].rho))
+            return fail(c, SDFIBM_ERR_ARG, "solid " + std::to_string(s) + ": density must be finite and > 0");
+        if (dyn[s].motion_mask < -1 || dyn[s].motion_mask > 63)
+            return fail(c, SDFIBM_ERR_ARG, "solid " + std::to_string(s) + ": motion mask must be -1 or 6 bits");
+        h[s].rho = dyn[s].rho;
+        h[s].mask = dyn[s].motion_mask;
+        h[s].first_fluid = 1;
+        for (int a = 0; a < 3; ++a) h[s].ff_old[a] = h[s].ft_old[a] = 0.0;
+    }
+    int rc;
+    if (c->d_dyn == nullptr || c->dyn_cap < n) {
+        dfree(c, c->d_dyn);
+        c->d_dyn = nullptr;
+        c->dyn_cap = 0;
+        if ((rc = dalloc(c, (void**)&c->d_dyn, std::max<long long>(n, 1) * sizeof(SolidDyn)))) return rc;
+        c->dyn_cap = n;
+    }
+    if (n > 0) HIPCHK(c, hipMemcpy(c->d_dyn, h.data(), n * sizeof(SolidDyn), hipMemcpyHostToDevice));
+    c->dyn_n = n;
+    return SDFIBM_OK;
+}
+
+int sdfibm_evolve(sdfibm_ctx* c, double time, double dt, const double* gravity) {
+    (void)time;
+    if (!c) return SDFIBM_ERR_ARG;
+    if (c->dyn_n != c->ns || c->dyn_n < 0) return fail(c, SDFIBM_ERR_STATE, "evolve needs set_dynamics for these solids");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
+    Vec3 g{{0.0, 0.0, 0.0}};
+    if (gravity)
+        for (int a = 0; a < 3; ++a) g.v[a] = gravity[a];
+    const int ns = (int)c->ns;
+    if (ns == 0) return SDFIBM_OK;
+    const int n_sub = ns == 1 ? 1 : 20;  // solidcloud.cpp:524-526
+    k_evolve<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_dyn, c->d_ft, ns, n_sub, dt, g, c->rho_f);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_step = false;  // the tiles of the last interact no longer match the solids
+    return SDFIBM_OK;
+}
+
+int sdfibm_get_solids(sdfibm_ctx* c, sdfibm_solid* out, int64_t n) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (n != c->ns || (n > 0 && !out)) return fail(c, SDFIBM_ERR_ARG, "need room for every solid");
+    if (n > 0) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(out, c->d_solids, n * sizeof(sdfibm_solid), hipMemcpyDeviceToHost));
+    }
     return SDFIBM_OK;
 }
 
@@ -1914,19 +2013,19 @@ int sdfibm_candidates(sdfibm_ctx* c, int64_t solid, int64_t* counts, int64_t* ce
         void* tmp = nullptr;
         int rc;
         if ((rc = dalloc(c, &tmp, nt * rs))) return rc;
-        const SolidK K = make_solid_k(c->h_solids[solid]);
+        const int sp = (int)solid;
         const int p = (int)(std::find(c->h_planes.begin(), c->h_planes.end(), (int)solid) - c->h_planes.begin());
         const unsigned* mem = c->d_pmember + (long long)p * c->member_words;
         if (c->prec == SDFIBM_FP64) {
             if (c->kind == SDFIBM_MESH_RECTILINEAR)
-                k_plane_codes<double, MK_RECT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, K, mem, (double*)tmp);
+                k_plane_codes<double, MK_RECT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, c->d_K, sp, mem, (double*)tmp);
             else
-                k_plane_codes<double, MK_STRUCT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, K, mem, (double*)tmp);
+                k_plane_codes<double, MK_STRUCT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, c->d_K, sp, mem, (double*)tmp);
         } else {
             if (c->kind == SDFIBM_MESH_RECTILINEAR)
-                k_plane_codes<float, MK_RECT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, K, mem, (float*)tmp);
+                k_plane_codes<float, MK_RECT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, c->d_K, sp, mem, (float*)tmp);
             else
-                k_plane_codes<float, MK_STRUCT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, K, mem, (float*)tmp);
+                k_plane_codes<float, MK_STRUCT><<<grid_for(nt, 256), 256, 0, c->stream>>>(c->m, c->d_K, sp, mem, (float*)tmp);
         }
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(raw.data(), tmp, nt * rs, hipMemcpyDeviceToHost, c->stream);
