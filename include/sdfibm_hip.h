@@ -162,6 +162,57 @@ int sdfibm_evolve(sdfibm_ctx* ctx, double time, double dt, const double* gravity
 /* Device -> host copy of the solids' current records (after evolve). */
 int sdfibm_get_solids(sdfibm_ctx* ctx, sdfibm_solid* out, int64_t n);
 
+/* ---- learned volume-fraction surrogate (no counterpart in the reference) ----
+ * An MLP 11 -> 32 -> 32 -> 1 (ReLU) over a cut cell's 8 corner signed distances
+ * / h and the unit SDF gradient at the cell centre; BF16 weights and
+ * activations, FP32 accumulation, MFMA.  Parameters: 1473 FP32 values, W1[32][11]
+ * b1[32] W2[32][32] b2[32] w3[32] b3.  Labels are the reference method's own
+ * fractions (geometrictools.cpp:47-72) on cells cut by random spheres and
+ * ellipsoids. */
+#define SDFIBM_SURROGATE_PARAMS 1473
+
+typedef struct {
+    uint64_t seed;
+    int64_t n_train;   /* training samples generated on the device */
+    int64_t n_val;     /* validation samples (another seed) */
+    int64_t batch;
+    int32_t epochs;
+    int32_t reserved;
+    double lr;         /* Adam learning rate, cosine-decayed to 0 */
+} sdfibm_surrogate_train_config;
+
+/* Train on the GPU (forward/backward kernels, Adam).  params: in = initial
+ * values, out = trained values.  report (may be NULL): [0] last batch MSE,
+ * [1] validation MSE and [2] max |error| through the deployed MFMA inference,
+ * [3] seconds. */
+int sdfibm_surrogate_train(const sdfibm_surrogate_train_config* cfg, float* params, double* report);
+
+/* n samples of the training distribution: features [n*16] (8 corner distances
+ * in cells, the centre gradient, zeros) and labels [n]. */
+int sdfibm_surrogate_samples(uint64_t seed, int64_t n, float* features, float* labels);
+
+/* The deployed MFMA inference on feature rows [n*16] -> out [n]. */
+int sdfibm_surrogate_eval(const float* params, const float* features, int64_t n, float* out);
+
+/* Gradient of 0.5 * sum (net(x) - y)^2 w.r.t. the parameters (the training
+ * kernels' forward and backward pass; straight-through for the BF16 rounding).
+ * sum_sq (may be NULL) receives sum (net(x) - y)^2. */
+int sdfibm_surrogate_grad(const float* params, const float* features, const float* labels, int64_t n,
+                          float* grad, double* sum_sq);
+
+/* Use the surrogate in the next interact calls of a FP32 context on a
+ * rectilinear 3-D mesh (params NULL: back to the exact fractions).  A cut cell
+ * falls back to the exact fraction when the prediction lies in [band_lo,
+ * band_hi] and the centre gradient is more than max_angle_deg away from the
+ * corner-difference gradient (max_angle_deg < 0: whenever the prediction lies
+ * in the band), and always for boundary cells, 2-D cases and shapes other than
+ * spheres and ellipsoids. */
+int sdfibm_set_surrogate(sdfibm_ctx* ctx, const float* params, double band_lo, double band_hi,
+                         double max_angle_deg);
+
+/* Cut cells the surrogate served in the last interact. */
+int sdfibm_surrogate_count(sdfibm_ctx* ctx, int64_t* n);
+
 /* Device -> host copy of U as AoS FP64 [N*3]. */
 int sdfibm_get_velocity(sdfibm_ctx* ctx, double* U_aos);
 

@@ -18,6 +18,8 @@ EXPORTS = (
     "sdfibm_last_timing", "sdfibm_set_stream", "sdfibm_last_error", "sdfibm_status_string",
     "sdfibm_abi_version", "sdfibm_update_velocity", "sdfibm_fix_internal", "sdfibm_get_velocity",
     "sdfibm_set_dynamics", "sdfibm_evolve", "sdfibm_get_solids",
+    "sdfibm_surrogate_train", "sdfibm_surrogate_samples", "sdfibm_surrogate_eval", "sdfibm_surrogate_grad",
+    "sdfibm_set_surrogate", "sdfibm_surrogate_count",
 )
 
 SDFIBM_OK = 0
@@ -37,6 +39,11 @@ class MeshDesc(C.Structure):
 class Config(C.Structure):
     _fields_ = [("two_d", C.c_int32), ("precision", C.c_int32), ("rho_f", C.c_double),
                 ("reserved", C.c_int64 * 4)]
+
+
+class SurrogateTrainConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_train", C.c_int64), ("n_val", C.c_int64), ("batch", C.c_int64),
+                ("epochs", C.c_int32), ("reserved", C.c_int32), ("lr", C.c_double)]
 
 
 class SdfibmError(RuntimeError):
@@ -77,6 +84,12 @@ def load():
         "sdfibm_set_dynamics": (i32, [P, P, i64]),
         "sdfibm_evolve": (i32, [P, d, d, P]),
         "sdfibm_get_solids": (i32, [P, P, i64]),
+        "sdfibm_surrogate_train": (i32, [P, P, P]),
+        "sdfibm_surrogate_samples": (i32, [C.c_uint64, i64, P, P]),
+        "sdfibm_surrogate_eval": (i32, [P, P, i64, P]),
+        "sdfibm_surrogate_grad": (i32, [P, P, P, i64, P, P]),
+        "sdfibm_set_surrogate": (i32, [P, P, d, d, d]),
+        "sdfibm_surrogate_count": (i32, [P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

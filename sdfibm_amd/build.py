@@ -18,8 +18,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsdfibm_hip.so")
-SOURCES = ["interact.hip"]
-HEADERS = ["shapes.h", "fraction.h", "mesh_dev.h"]
+SOURCES = ["interact.hip", "surrogate.hip"]
+HEADERS = ["shapes.h", "fraction.h", "mesh_dev.h", "dynamics.h", "surrogate.h"]
 
 
 def hipcc() -> str:

@@ -149,6 +149,24 @@ class SolidCloud:
         _lib.check(_lib.load().sdfibm_get_solids(self._h, _p(out), out.size), self._h)
         return out
 
+    # -- learned fraction surrogate ----------------------------------------------------------
+    def set_surrogate(self, params, band=(0.02, 0.98), max_angle_deg=30.0):
+        """Use the surrogate network (1473 FP32 parameters) for cut cells; params None: exact."""
+        L = _lib.load()
+        if params is None:
+            _lib.check(L.sdfibm_set_surrogate(self._h, None, 0.0, 0.0, 0.0), self._h)
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != 1473:
+            raise ValueError("the surrogate has 1473 parameters")
+        _lib.check(L.sdfibm_set_surrogate(self._h, _p(p), float(band[0]), float(band[1]), float(max_angle_deg)),
+                   self._h)
+
+    def surrogate_count(self) -> int:
+        n = np.zeros(1, dtype=np.int64)
+        _lib.check(_lib.load().sdfibm_surrogate_count(self._h, _p(n)), self._h)
+        return int(n[0])
+
     def velocity(self) -> np.ndarray:
         U = np.zeros((self.n_cells, 3))
         _lib.check(_lib.load().sdfibm_get_velocity(self._h, _p(U)), self._h)

@@ -41,6 +41,7 @@
 #include "../../include/sdfibm_hip.h"
 #include "dynamics.h"
 #include "mesh_dev.h"
+#include "surrogate.h"
 
 using namespace sdfibm_dev;
 
@@ -517,16 +518,25 @@ __global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK
 }
 
 // Forcing, fractions and the force/torque reduction over the candidate lists:
-// one workgroup per queued solid, largest tile class first.
-template <class R, int MK>
+// one workgroup per queued solid, largest tile class first.  SG: the learned
+// surrogate gives the fraction of a cut cell (FP32, rectilinear mesh, 3-D,
+// interior cells of quadric solids) unless its prediction lies in [sg_lo,
+// sg_hi] while the centre gradient and the corner-difference gradient differ by
+// more than the angle whose cosine is sg_cos; then, and for every other cell,
+// the exact box fraction.
+template <class R, int MK, bool SG>
 __global__ void __launch_bounds__(kBlock) k_cells(MeshDev m, int ns, const SolidK* __restrict__ Ks,
                                                   const TileInfo* __restrict__ Ts, int* __restrict__ qmeta,
                                                   const int* __restrict__ qlist, const int* __restrict__ cand,
                                                   const int2* __restrict__ ncand, const R* __restrict__ Ux,
                                                   const R* __restrict__ Uy, const R* __restrict__ Uz,
                                                   R* __restrict__ codes, long long code_cap, double dt,
-                                                  double rho_f, double* __restrict__ ft) {
+                                                  double rho_f, double* __restrict__ ft, const SgNet* __restrict__ sgnet,
+                                                  float sg_lo, float sg_hi, float sg_cos, int* __restrict__ sg_count) {
     __shared__ double red[6 * 16];
+    constexpr int NSG = SG ? 1 : 0;
+    __shared__ SgNet sn[NSG ? 1 : 1];
+    __shared__ unsigned short feat[SG ? kBlock * kSgPad : 1];
     const int tid = threadIdx.x;
     const int item = blockIdx.x;
     if (item >= qmeta[0] + qmeta[1] + qmeta[2]) return;
@@ -541,19 +551,69 @@ __global__ void __launch_bounds__(kBlock) k_cells(MeshDev m, int ns, const Solid
     const int* tl = cand + T.off;
     R* tcodes = codes + T.off;
 
+    SgRegs sgr;
+    bool sg_shape = false;
+    if constexpr (SG) {
+        for (int t = tid; t < (int)(sizeof(SgNet) / 4); t += kBlock)
+            reinterpret_cast<unsigned*>(&sn[0])[t] = reinterpret_cast<const unsigned*>(sgnet)[t];
+        __syncthreads();
+        sg_load(sn[0], sgr, tid & 63);
+        sg_shape = K.shape == SDFIBM_SHAPE_SPHERE || K.shape == SDFIBM_SHAPE_ELLIPSOID;
+    }
+    int n_net = 0;
+
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int q = tid; q < ntot; q += kBlock) {
-        const int e = tl[q];
-        const int i = e & 63, j = (e >> 6) & 8191, k = e >> 19;
+    for (int base = 0; base < ntot; base += kBlock) {  // the same trip count for every thread
+        const int q = base + tid;
+        const bool live = q < ntot;
+        int i = 0, j = 0, k = 0;
+        if (live) {
+            const int e = tl[q];
+            i = e & 63, j = (e >> 6) & 8191, k = e >> 19;
+        }
+        const bool cut = live && q >= nin;
         const int li = i + nx * (j + ny * k);
         const int gi = T.lo[0] + i, gj = T.lo[1] + j, gk = T.lo[2] + k;
         double alpha = 1.0;
-        if (q >= nin) {
-            const R a = cell_fraction<R, MK>(m, K, gi, gj, gk);
-            const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
-            tcodes[li] = cin ? a : -a;
-            alpha = (double)a;
+        if constexpr (SG) {
+            Phi8<float> h{0, 0, 0, 0, 0, 0, 0, 0};
+            d3 lc{0, 0, 0};
+            float g[3] = {0.f, 0.f, 0.f};
+            bool ok = false;
+            float ihx = 0.f, ihy = 0.f, ihz = 0.f;
+            if (cut) {
+                box_corner_phi(m, K, gi, gj, gk, h, lc);
+                ihx = (float)(1.0 / (m.xs[gi + 1] - m.xs[gi]));
+                ihy = (float)(1.0 / (m.ys[gj + 1] - m.ys[gj]));
+                ihz = (float)(1.0 / (m.zs[gk + 1] - m.zs[gk]));
+                ok = sg_shape && m.two_d == 0 && gi > 0 && gj > 0 && gk > 0 && quadric_grad(K, lc, g);
+            }
+            const float phi8[8] = {h.f0, h.f1, h.f2, h.f3, h.f4, h.f5, h.f6, h.f7};
+            const float ih = cbrtf(ihx * ihy * ihz);
+            unsigned short* fw = feat + (tid & ~63) * kSgPad;
+            sg_features(phi8, ok ? ih : 0.f, g, feat + tid * kSgPad);
+            float pred = 0.f;
+            if (__any(ok)) pred = sg_infer(sn[0], sgr, fw, tid & 63);  // whole wave
+            if (ok && pred >= sg_lo && pred <= sg_hi && corner_alignment(phi8, g, ihx, ihy, ihz) < sg_cos) ok = false;
+            if (cut) {
+                const double z0 = m.zs[gk], z1 = m.zs[gk + 1];
+                const float a = ok ? pred
+                                   : box_fraction<float>(h, gi == 0, gj == 0, gk == 0, m.two_d != 0,
+                                                         (float)((0.0 - z0) / (z1 - z0)));
+                n_net += ok ? 1 : 0;
+                const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
+                tcodes[li] = R(cin ? a : -a);
+                alpha = (double)a;
+            }
+        } else {
+            if (cut) {
+                const R a = cell_fraction<R, MK>(m, K, gi, gj, gk);
+                const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
+                tcodes[li] = cin ? a : -a;
+                alpha = (double)a;
+            }
         }
+        if (!live) continue;
         const long long c = cell_id(m, gi, gj, gk);
         const d3 cc = cell_centre<MK>(m, gi, gj, gk);
         const double V = cell_volume<MK>(m, gi, gj, gk);
@@ -568,6 +628,10 @@ __global__ void __launch_bounds__(kBlock) k_cells(MeshDev m, int ns, const Solid
         acc[3] += (tq.x * V) * dtINV;
         acc[4] += (tq.y * V) * dtINV;
         acc[5] += (tq.z * V) * dtINV;
+    }
+    if constexpr (SG) {
+        for (int off = 32; off >= 1; off >>= 1) n_net += __shfl_xor(n_net, off, 64);
+        if ((tid & 63) == 0 && n_net > 0) atomicAdd(sg_count, n_net);
     }
     block_reduce<6>(acc, red);
     if (tid == 0)
@@ -1138,6 +1202,9 @@ struct sdfibm_ctx {
     int* d_blist = nullptr;  // brick lists, kMaxPerBrick slots per brick
     double* d_ft = nullptr;
     SolidDyn* d_dyn = nullptr;  // rigid-body state (set_dynamics)
+    SgNet* d_sg = nullptr;      // learned fraction surrogate (set_surrogate), null = exact fractions
+    int* d_sg_count = nullptr;  // cut cells the surrogate served in the last interact
+    float sg_lo = 0.02f, sg_hi = 0.98f, sg_cos = 0.8660254f;
     long long dyn_n = -1;       // solids the dynamics were set for
     long long dyn_cap = 0;
     std::vector<int> h_planes;
@@ -1286,9 +1353,19 @@ int launch_step(sdfibm_ctx* c, double dt) {
                                                                c->d_cand, c->code_cap, c->d_ncand);
         if (hipStreamWaitEvent(st, c->join, 0) != hipSuccess) return -1;
         (void)hipEventRecord(c->ev[4], st);
-        k_cells<R, MK><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
-                                                   c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
-                                                   (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft);
+        constexpr bool kSgOk = MK == MK_RECT && sizeof(R) == sizeof(float);
+        if (kSgOk && c->d_sg != nullptr) {
+            k_fill_i32<<<1, 64, 0, st>>>(c->d_sg_count, 1, 0);
+            k_cells<R, MK, kSgOk><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
+                                                        c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
+                                                        (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft,
+                                                        c->d_sg, c->sg_lo, c->sg_hi, c->sg_cos, c->d_sg_count);
+        } else {
+            k_cells<R, MK, false><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
+                                                        c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
+                                                        (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft,
+                                                        nullptr, 0.f, 0.f, 0.f, nullptr);
+        }
     } else {
         (void)hipEventRecord(c->ev[4], st);
     }
@@ -1919,6 +1996,42 @@ int sdfibm_evolve(sdfibm_ctx* c, double time, double dt, const double* gravity) 
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_step = false;  // the tiles of the last interact no longer match the solids
+    return SDFIBM_OK;
+}
+
+int sdfibm_set_surrogate(sdfibm_ctx* c, const float* params, double band_lo, double band_hi, double max_angle_deg) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!params) {
+        dfree(c, c->d_sg);
+        c->d_sg = nullptr;
+        return SDFIBM_OK;
+    }
+    if (!(band_lo <= band_hi) || !std::isfinite(band_lo) || !std::isfinite(band_hi) || !(max_angle_deg <= 180))
+        return fail(c, SDFIBM_ERR_ARG, "bad surrogate band or angle");
+    for (int p = 0; p < kSgParams; ++p)
+        if (!std::isfinite(params[p])) return fail(c, SDFIBM_ERR_ARG, "non-finite surrogate parameter");
+    SgNet h;
+    sg_pack(params, h);
+    int rc;
+    if (!c->d_sg && (rc = dalloc(c, (void**)&c->d_sg, sizeof(SgNet)))) return rc;
+    if (!c->d_sg_count && (rc = dalloc(c, (void**)&c->d_sg_count, 64))) return rc;
+    HIPCHK(c, hipMemcpy(c->d_sg, &h, sizeof(SgNet), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->d_sg_count, 0, 4));
+    c->sg_lo = (float)band_lo;
+    c->sg_hi = (float)band_hi;
+    // a negative angle: every cell whose prediction lies in the band falls back
+    c->sg_cos = max_angle_deg < 0 ? 2.0f : (float)std::cos(max_angle_deg * 3.14159265358979323846 / 180.0);
+    return SDFIBM_OK;
+}
+
+int sdfibm_surrogate_count(sdfibm_ctx* c, int64_t* n) {
+    if (!c || !n) return SDFIBM_ERR_ARG;
+    int v = 0;
+    if (c->d_sg_count) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(&v, c->d_sg_count, 4, hipMemcpyDeviceToHost));
+    }
+    *n = v;
     return SDFIBM_OK;
 }
 

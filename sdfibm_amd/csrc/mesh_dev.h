@@ -157,41 +157,48 @@ __device__ inline R fraction_of_cell(const MeshDev& m, const SolidK& K, int i, i
     return fraction_with_phi<R, MK>(m, i, j, k, [&](int, int, int, d3 w) { return sdf_fast<R>(K, w); });
 }
 
+// Corner signed distances of box cell (i,j,k) of a rectilinear mesh in FP32
+// (the local coordinates of the corners share the products of sdf_fast: the
+// same values, bit for bit) and the local coordinates of the cell centre.
+__device__ inline void box_corner_phi(const MeshDev& m, const SolidK& K, int i, int j, int k, Phi8<float>& h, d3& lc) {
+    const double dx[2] = {m.xs[i] - K.c[0], m.xs[i + 1] - K.c[0]};
+    const double dy[2] = {m.ys[j] - K.c[1], m.ys[j + 1] - K.c[1]};
+    const double dz[2] = {m.zs[k] - K.c[2], m.zs[k + 1] - K.c[2]};
+    double px[3][2], py[3][2], pz[3][2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            px[r][b] = K.rt[3 * r + 0] * dx[b];
+            py[r][b] = K.rt[3 * r + 1] * dy[b];
+            pz[r][b] = K.rt[3 * r + 2] * dz[b];
+        }
+    auto loc = [&](int bx, int by, int bz) {
+        return d3{px[0][bx] + py[0][by] + pz[0][bz], px[1][bx] + py[1][by] + pz[1][bz],
+                  px[2][bx] + py[2][by] + pz[2][bz]};
+    };
+    const d3 l0 = loc(0, 0, 0), l7 = loc(1, 1, 1);
+    h.f0 = sdf_local_mixed(K, l0);
+    h.f1 = sdf_local_mixed(K, loc(1, 0, 0));
+    h.f2 = sdf_local_mixed(K, loc(0, 1, 0));
+    h.f3 = sdf_local_mixed(K, loc(1, 1, 0));
+    h.f4 = sdf_local_mixed(K, loc(0, 0, 1));
+    h.f5 = sdf_local_mixed(K, loc(1, 0, 1));
+    h.f6 = sdf_local_mixed(K, loc(0, 1, 1));
+    h.f7 = sdf_local_mixed(K, l7);
+    lc = 0.5 * (l0 + l7);
+}
+
 // Volume fraction of cell (i,j,k) inside solid K as the cut-cell kernels form
-// it: in FP32 on a rectilinear mesh the box form (corner local coordinates with
-// the products of sdf_fast shared between corners: the same values, bit for
-// bit); otherwise the general hex form (FP64 keeps the reference's operation
-// order).
+// it: in FP32 on a rectilinear mesh the box form; otherwise the general hex
+// form (FP64 keeps the reference's operation order).
 template <class R, int MK>
 __device__ inline R cell_fraction(const MeshDev& m, const SolidK& K, int i, int j, int k) {
     if constexpr (MK == MK_RECT && sizeof(R) == sizeof(float)) {
-        const double z0 = m.zs[k], z1 = m.zs[k + 1];
-        const double dx[2] = {m.xs[i] - K.c[0], m.xs[i + 1] - K.c[0]};
-        const double dy[2] = {m.ys[j] - K.c[1], m.ys[j + 1] - K.c[1]};
-        const double dz[2] = {z0 - K.c[2], z1 - K.c[2]};
-        double px[3][2], py[3][2], pz[3][2];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                px[r][b] = K.rt[3 * r + 0] * dx[b];
-                py[r][b] = K.rt[3 * r + 1] * dy[b];
-                pz[r][b] = K.rt[3 * r + 2] * dz[b];
-            }
-        auto phi = [&](int bx, int by, int bz) {
-            const d3 l{px[0][bx] + py[0][by] + pz[0][bz], px[1][bx] + py[1][by] + pz[1][bz],
-                       px[2][bx] + py[2][by] + pz[2][bz]};
-            return sdf_local_mixed(K, l);
-        };
         Phi8<float> h;
-        h.f0 = phi(0, 0, 0);
-        h.f1 = phi(1, 0, 0);
-        h.f2 = phi(0, 1, 0);
-        h.f3 = phi(1, 1, 0);
-        h.f4 = phi(0, 0, 1);
-        h.f5 = phi(1, 0, 1);
-        h.f6 = phi(0, 1, 1);
-        h.f7 = phi(1, 1, 1);
+        d3 lc;
+        box_corner_phi(m, K, i, j, k, h, lc);
+        const double z0 = m.zs[k], z1 = m.zs[k + 1];
         const float az = (float)((0.0 - z0) / (z1 - z0));
         return box_fraction<float>(h, i == 0, j == 0, k == 0, m.two_d != 0, az);
     } else {
