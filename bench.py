@@ -108,6 +108,46 @@ def cpu_baseline(scene, n_sample, budget_s):
     return pairs / elapsed, done, pairs, elapsed
 
 
+def surrogate_comparison(cloud, sc, ft0, As0, plain_ms, plain_cells_ms, steps):
+    """The same step with the learned fraction (MLP 11-32-32-1, BF16 MFMA) serving the cut cells
+    of spheres and ellipsoids, at the prescribed fallback rule (prediction in [0.02, 0.98] and a
+    gradient angle above 30 degrees -> exact fraction).  Reports the step-time ratio and the
+    deviation from the plain method's As and force/torque."""
+    import torch
+
+    from sdfibm_amd import surrogate as SG
+
+    if not os.path.exists(SG.PARAMS_PATH):
+        return {"skipped": "no trained parameter file"}
+    p = SG.load_params()
+    band, angle = (0.02, 0.98), 30.0
+    cloud.set_surrogate(p, band=band, max_angle_deg=angle)
+    for _ in range(3):
+        cloud.interact_device(0.0, sc.dt)
+    torch.cuda.synchronize()
+    k = max(5, min(steps, 20))
+    ph = np.zeros(6)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        cloud.interact_device(0.0, sc.dt)
+        ph += cloud.timing()
+    torch.cuda.synchronize()
+    sg_ms = 1e3 * (time.perf_counter() - t0) / k
+    ft1 = cloud.interact(0.0, sc.dt)
+    As1 = cloud.fields()["As"]
+    served = cloud.surrogate_count()
+    cloud.set_surrogate(None)
+    scale = np.maximum(np.abs(ft0), 1e-3 * np.abs(ft0).max())
+    dAs = float(np.abs(As1.astype(np.float64) - As0).max())
+    out = {"band": list(band), "max_angle_deg": angle, "served_cut_cells": int(served),
+           "plain_ms_per_step": plain_ms, "surrogate_ms_per_step": sg_ms, "speedup": plain_ms / sg_ms,
+           "k_cells_ms": {"plain": plain_cells_ms, "surrogate": ph[5] / k},
+           "max_abs_dAs": dAs, "force_torque_max_rel_err": float(np.max(np.abs(ft1 - ft0) / scale)),
+           "meets_max_dAs_1e-3": dAs <= 1e-3}
+    log(f"[bench] surrogate: {out}")
+    return out
+
+
 def bench_sedimentation(args):
     """The CPU-runnable case: one interact and a loop of full coupling steps, FP64."""
     import torch
@@ -185,6 +225,8 @@ def main():
     ap.add_argument("--scene-cache", default="", help="npz file to reuse the generated scene")
     ap.add_argument("--workload", default="cloud3d", choices=["cloud3d", "sedimentation2d"])
     ap.add_argument("--loop-steps", type=int, default=200, help="sedimentation2d: coupling steps timed")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="cloud3d fp32: skip the learned-fraction comparison run")
     ap.add_argument("--copy-test", type=float, default=0.0,
                     help="only run a plain device copy of this many GB (HBM counter calibration)")
     args = ap.parse_args()
@@ -299,6 +341,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes": kernels[dom]["bytes"]},
     }
+    if args.precision == "fp32" and not args.no_surrogate:
+        line["surrogate"] = surrogate_comparison(cloud, sc, ft, f["As"], ms_per_step, phases[5], args.steps)
     if not args.no_cpu:
         rate, ns_t, pairs_t, el = cpu_baseline(sc, args.cpu_sample, args.cpu_budget)
         line["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",

@@ -94,3 +94,14 @@ def test_interact_surrogate_switches():
         c.set_surrogate(None)
         c.interact(0.0, sc.dt)
         np.testing.assert_array_equal(c.fields()["As"], f0["As"])
+
+
+def test_shipped_parameters_accuracy():
+    """The parameter file shipped with the package (trained on the GPU on 10M cells, 10 epochs):
+    error on fresh cells of the training distribution, and it beats the untrained network."""
+    p = SG.load_params()
+    X, y = SG.samples(2024, 100000)
+    e = np.abs(SG.evaluate(p, X) - y)
+    e0 = np.abs(SG.evaluate(SG.init_params(0), X) - y)
+    assert e.mean() < 0.015 and np.quantile(e, 0.99) < 0.06 and e.max() < 0.3, (e.mean(), e.max())
+    assert e.mean() < 0.1 * e0.mean()
