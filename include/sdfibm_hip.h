@@ -159,6 +159,25 @@ int sdfibm_set_dynamics(sdfibm_ctx* ctx, const sdfibm_dynamics* dyn, int64_t n);
  * next interact and fix_internal use the moved state. */
 int sdfibm_evolve(sdfibm_ctx* ctx, double time, double dt, const double* gravity);
 
+/* Contacts between solids inside evolve (SolidCloud::solidSolidInteract and
+ * solidSolidCollision, solidcloud.cpp:477-519; UGrid pair search,
+ * libcollision/ugrid.cpp:5-77; geometry of collision.cpp:7-32).  Every finite
+ * solid is its bounding sphere (IShape::m_radiusB of its shape), a plane solid
+ * the half-space local y < 0.  Force on each partner along the line of centres
+ * (plane: along the plane normal), no torque:
+ *     f = max(0, stiffness * overlap - damping * v_n),
+ * v_n = the normal relative velocity of the contact point (> 0 separating).
+ * The reference's law is stiffness 1e4, damping 0; at its HEAD the search grid
+ * is built with a bounding radius of -1 (solidcloud.cpp:74-75, 245) and finds no
+ * pairs, which is stiffness = damping = 0 here (the default: no contacts).
+ * Evaluated every sub-step from the state at its start; a solid's contacts are
+ * summed in ascending partner index, then the planes.  Unlike the reference,
+ * planes are not put in the search grid (it inserts a plane by its reference
+ * point only) but tested against every finite solid, and shapes other than
+ * sphere/circle use their bounding sphere instead of the reference's table
+ * lookup (which maps unknown shape names to Plane, collision.h:11-15). */
+int sdfibm_set_contact(sdfibm_ctx* ctx, double stiffness, double damping);
+
 /* Device -> host copy of the solids' current records (after evolve). */
 int sdfibm_get_solids(sdfibm_ctx* ctx, sdfibm_solid* out, int64_t n);
 

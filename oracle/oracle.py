@@ -63,6 +63,11 @@ def lib():
         L.orc_fix_internal.argtypes = [P, P, P, i64, P]
         L.orc_mass_properties.argtypes = [P, P, P]
         L.orc_evolve.argtypes = [i64, P, P, P, C.c_double, C.c_double, P, C.c_double]
+        L.orc_bounding_radius.restype = C.c_double
+        L.orc_bounding_radius.argtypes = [P]
+        L.orc_contact_forces.argtypes = [i64, P, C.c_double, C.c_double, P]
+        L.orc_evolve_contact.argtypes = [i64, P, P, P, C.c_double, C.c_double, P, C.c_double, C.c_double,
+                                         C.c_double]
         del dp
         _lib = L
     return _lib
@@ -242,4 +247,29 @@ def evolve(solids, dyn, fluid_ft, time, dt, gravity, rho_f):
     ft = np.ascontiguousarray(fluid_ft, dtype=np.float64).reshape(-1, 6)
     g = np.ascontiguousarray(gravity, dtype=np.float64)
     lib().orc_evolve(s.size, _ptr(s), _ptr(d), _ptr(ft), float(time), float(dt), _ptr(g), float(rho_f))
+    return s, d
+
+
+def bounding_radius(solid) -> float:
+    """IShape::m_radiusB of one solid record (plane: 0)."""
+    a = np.ascontiguousarray(np.atleast_1d(solid)[:1]).copy()
+    return float(lib().orc_bounding_radius(_ptr(a)))
+
+
+def contact_forces(solids, k, damping=0.0):
+    """Contact force [ns, 3] on every solid by brute force over all pairs (orc_contact_forces)."""
+    s = np.ascontiguousarray(solids).copy()
+    out = np.zeros((s.size, 3))
+    lib().orc_contact_forces(s.size, _ptr(s), float(k), float(damping), _ptr(out))
+    return out
+
+
+def evolve_contact(solids, dyn, fluid_ft, time, dt, gravity, rho_f, k, damping=0.0):
+    """SolidCloud::evolve with contacts on copies: returns (solids, dyn) after one step."""
+    s = np.ascontiguousarray(solids).copy()
+    d = np.ascontiguousarray(dyn, dtype=DYN_DTYPE).copy()
+    ft = np.ascontiguousarray(fluid_ft, dtype=np.float64).reshape(-1, 6)
+    g = np.ascontiguousarray(gravity, dtype=np.float64)
+    lib().orc_evolve_contact(s.size, _ptr(s), _ptr(d), _ptr(ft), float(time), float(dt), _ptr(g), float(rho_f),
+                             float(k), float(damping))
     return s, d

@@ -1142,6 +1142,129 @@ void tmul(const double* A, const double* B, double* C) {  // C = A & B
 }
 }  // namespace
 
+}  // extern "C"
+
+namespace {
+using namespace orc;
+
+struct Body {
+    double mass, mass_inv, Minv[9];
+};
+
+// Solid::setMaterial (solid.h:94-103): mass = V rho, mass_inv = (1/V)/rho, moi_inv = inv(moi)/rho
+Body body_of(const orc_solid& s, const orc_dyn& d) {
+    double vol, moi[3];
+    orc_mass_properties(&s, &vol, moi);
+    const double volinv = vol > 0 ? 1.0 / vol : 0.0;
+    Body b;
+    b.mass = vol * d.rho;
+    b.mass_inv = volinv / d.rho;
+    for (int a = 0; a < 9; ++a) b.Minv[a] = 0.0;
+    for (int a = 0; a < 3; ++a) b.Minv[4 * a] = (1.0 / moi[a]) / d.rho;
+    return b;
+}
+
+// One sub-step of one solid: clear, AB2 fluid force (first step: old = current),
+// buoyant gravity, the contact force fc (if any), Solid::move.
+void substep(orc_solid& s, orc_dyn& d, const Body& b, V3 ff, V3 ft, V3 g, double rho_f, const V3* fc, double dts) {
+    if (d.first_fluid) {
+        for (int a = 0; a < 3; ++a) {
+            d.ff_old[a] = (&ff.x)[a];
+            d.ft_old[a] = (&ft.x)[a];
+        }
+        d.first_fluid = 0;
+    }
+    const V3 ffo{d.ff_old[0], d.ff_old[1], d.ff_old[2]}, fto{d.ft_old[0], d.ft_old[1], d.ft_old[2]};
+    V3 force = V3{0, 0, 0} + (1.5 * ff - 0.5 * ffo);
+    const V3 torque = V3{0, 0, 0} + (1.5 * ft - 0.5 * fto);
+    const V3 gprime = ((d.rho - rho_f) / d.rho) * g;
+    force = force + b.mass * gprime;
+    if (fc) force = force + *fc;
+    // Solid::move
+    V3 v{s.vel[0], s.vel[1], s.vel[2]}, w{s.omega[0], s.omega[1], s.omega[2]};
+    const V3 v_old = v, w_old = w;
+    v = v + (force * b.mass_inv) * dts;
+    Quat q{s.q[0], {s.q[1], s.q[2], s.q[3]}};
+    double R[9], RT[9], T1[9], Mw[9];
+    qR(q, R);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) RT[3 * i + j] = R[3 * j + i];
+    tmul(R, b.Minv, T1);
+    tmul(T1, RT, Mw);
+    const V3 aw{Mw[0] * torque.x + Mw[1] * torque.y + Mw[2] * torque.z,
+                Mw[3] * torque.x + Mw[4] * torque.y + Mw[5] * torque.z,
+                Mw[6] * torque.x + Mw[7] * torque.y + Mw[8] * torque.z};
+    w = w + aw * dts;
+    if (d.mask >= 0) {  // Motion01Mask::constraint: cmptMultiply with 0/1 masks
+        v = {v.x * ((d.mask >> 0) & 1), v.y * ((d.mask >> 1) & 1), v.z * ((d.mask >> 2) & 1)};
+        w = {w.x * ((d.mask >> 3) & 1), w.y * ((d.mask >> 4) & 1), w.z * ((d.mask >> 5) & 1)};
+    }
+    const V3 c = V3{s.center[0], s.center[1], s.center[2]} + (0.5 * (v + v_old)) * dts;
+    const Quat hw{0.5 * 0.0, 0.5 * (0.5 * (w + w_old))};  // 0.5 * quaternion(0.5 (w + w_old))
+    const Quat dq = qmul(hw, q);
+    q = {q.w + dq.w * dts, q.v + dq.v * dts};
+    const double qm = std::sqrt(q.w * q.w + magSqr(q.v));
+    q = {q.w / qm, q.v / qm};
+    for (int a = 0; a < 3; ++a) {
+        s.vel[a] = (&v.x)[a];
+        s.omega[a] = (&w.x)[a];
+        s.center[a] = (&c.x)[a];
+    }
+    s.q[0] = q.w;
+    s.q[1] = q.v.x;
+    s.q[2] = q.v.y;
+    s.q[3] = q.v.z;
+}
+
+// storeOldForce (solid.h:143-149)
+void store_old(orc_dyn& d, V3 ff, V3 ft) {
+    for (int a = 0; a < 3; ++a) {
+        d.ff_old[a] = (&ff.x)[a];
+        d.ft_old[a] = (&ft.x)[a];
+    }
+}
+
+V3 cvec(const double* p) { return {p[0], p[1], p[2]}; }
+
+// Contact force on a from finite solid b (sphereSphereCollision geometry,
+// collision.cpp:7-17: normal from a to b, contact point 0.5 (ca + cb + (ra - rb) n)).
+V3 pair_force(const orc_solid& a, double ra, const orc_solid& b, double rb, double k, double cdamp) {
+    const V3 ca = cvec(a.center), cb = cvec(b.center);
+    const V3 ab = cb - ca;
+    const double dist = mag(ab);
+    const double delta = ra + rb - dist;
+    const V3 n = ab / (dist + 1e-300);  // Foam::normalised
+    const V3 cp = 0.5 * ((ca + cb) + (ra - rb) * n);
+    const V3 va = cvec(a.vel) + cross(cvec(a.omega), cp - ca);
+    const V3 vb = cvec(b.vel) + cross(cvec(b.omega), cp - cb);
+    const double vn = dot(vb - va, n);
+    const double fm = k * delta - cdamp * vn;
+    if (!(delta > 0) || !(fm > 0)) return {0, 0, 0};
+    return (-fm) * n;
+}
+
+// Contact force on finite solid s from plane p (planeSphereCollision geometry,
+// collision.cpp:19-28: normal = the plane's local +y axis, overlap r - local y
+// of the centre, contact point centre - r n).
+V3 plane_force(const orc_solid& s, double r, const orc_solid& p, double k, double cdamp) {
+    double R[9];
+    qR(Quat{p.q[0], {p.q[1], p.q[2], p.q[3]}}, R);
+    const V3 n{R[1], R[4], R[7]};
+    const V3 cs = cvec(s.center), cpl = cvec(p.center);
+    const double delta = r - dot(n, cs - cpl);
+    const V3 cp = cs - r * n;
+    const V3 vs = cvec(s.vel) + cross(cvec(s.omega), cp - cs);
+    const V3 vp = cvec(p.vel) + cross(cvec(p.omega), cp - cpl);
+    const double vn = dot(vs - vp, n);
+    const double fm = k * delta - cdamp * vn;
+    if (!(delta > 0) || !(fm > 0)) return {0, 0, 0};
+    return fm * n;
+}
+
+}  // namespace
+
+extern "C" {
+
 void orc_evolve(int64_t ns, orc_solid* solids, orc_dyn* dyn, const double* fluid_ft, double time, double dt,
                 const double* gravity, double rho_f) {
     (void)time;
@@ -1149,72 +1272,72 @@ void orc_evolve(int64_t ns, orc_solid* solids, orc_dyn* dyn, const double* fluid
     const double dts = dt / n_sub;
     const V3 g{gravity[0], gravity[1], gravity[2]};
     for (int64_t id = 0; id < ns; ++id) {
-        orc_solid& s = solids[id];
-        orc_dyn& d = dyn[id];
-        double vol, moi[3];
-        orc_mass_properties(&s, &vol, moi);
-        // Solid::setMaterial (solid.h:94-103): mass = V rho, mass_inv = (1/V)/rho, moi_inv = inv(moi)/rho
-        const double volinv = vol > 0 ? 1.0 / vol : 0.0;
-        const double mass = vol * d.rho, mass_inv = volinv / d.rho;
-        double Minv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int a = 0; a < 3; ++a) Minv[4 * a] = (1.0 / moi[a]) / d.rho;
-        const V3 ff{fluid_ft[6 * id], fluid_ft[6 * id + 1], fluid_ft[6 * id + 2]};
-        const V3 ft{fluid_ft[6 * id + 3], fluid_ft[6 * id + 4], fluid_ft[6 * id + 5]};
-        for (int it = 0; it < n_sub; ++it) {
-            // clear, AB2 fluid force (first step: old = current), buoyant gravity
-            if (d.first_fluid) {
-                for (int a = 0; a < 3; ++a) {
-                    d.ff_old[a] = (&ff.x)[a];
-                    d.ft_old[a] = (&ft.x)[a];
-                }
-                d.first_fluid = 0;
+        const Body b = body_of(solids[id], dyn[id]);
+        const V3 ff = cvec(fluid_ft + 6 * id), ft = cvec(fluid_ft + 6 * id + 3);
+        for (int it = 0; it < n_sub; ++it) substep(solids[id], dyn[id], b, ff, ft, g, rho_f, nullptr, dts);
+        store_old(dyn[id], ff, ft);
+    }
+}
+
+double orc_bounding_radius(const orc_solid* s) {
+    switch (s->shape) {
+    case 0:  // Sphere, sphere.h:28
+    case 2:  // Circle, circle.h:27
+        return s->prm[0];
+    case 1:  // Ellipsoid, ellipsoid.h:32
+    case 4:  // Box, box.h:29
+        return std::max(std::max(s->prm[0], s->prm[1]), s->prm[2]);
+    case 3:  // Ellipse, ellipse.h:30
+    case 5:  // Rectangle, rectangle.h:31
+        return std::max(s->prm[0], s->prm[1]);
+    case 7:  // Circle_Tail, circle_tail.h:28,33: m_radiusB = 2 m_radiusa, m_radiusa = (ratio + 1) r / 2
+    case 8:  // Circle_TwoTail, circle_twotail.h:26,31
+        return 2 * ((s->prm[1] + 1) * 0.5 * s->prm[0]);
+    default:
+        return 0.0;
+    }
+}
+
+void orc_contact_forces(int64_t ns, const orc_solid* solids, double k, double cdamp, double* out) {
+    std::vector<double> rb(ns);
+    for (int64_t s = 0; s < ns; ++s) rb[s] = orc_bounding_radius(&solids[s]);
+    for (int64_t s = 0; s < ns; ++s) {
+        V3 fc{0, 0, 0};
+        if (rb[s] > 0) {
+            for (int64_t j = 0; j < ns; ++j) {  // every finite partner, ascending
+                if (j == s || !(rb[j] > 0)) continue;
+                const V3 ab = cvec(solids[j].center) - cvec(solids[s].center);
+                if (!(rb[s] + rb[j] - mag(ab) > 0)) continue;
+                fc = fc + pair_force(solids[s], rb[s], solids[j], rb[j], k, cdamp);
             }
-            const V3 ffo{d.ff_old[0], d.ff_old[1], d.ff_old[2]}, fto{d.ft_old[0], d.ft_old[1], d.ft_old[2]};
-            V3 force = V3{0, 0, 0} + (1.5 * ff - 0.5 * ffo);
-            const V3 torque = V3{0, 0, 0} + (1.5 * ft - 0.5 * fto);
-            const V3 gprime = ((d.rho - rho_f) / d.rho) * g;
-            force = force + mass * gprime;
-            // Solid::move
-            V3 v{s.vel[0], s.vel[1], s.vel[2]}, w{s.omega[0], s.omega[1], s.omega[2]};
-            const V3 v_old = v, w_old = w;
-            v = v + (force * mass_inv) * dts;
-            Quat q{s.q[0], {s.q[1], s.q[2], s.q[3]}};
-            double R[9], RT[9], T1[9], Mw[9];
-            qR(q, R);
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) RT[3 * i + j] = R[3 * j + i];
-            tmul(R, Minv, T1);
-            tmul(T1, RT, Mw);
-            const V3 aw{Mw[0] * torque.x + Mw[1] * torque.y + Mw[2] * torque.z,
-                        Mw[3] * torque.x + Mw[4] * torque.y + Mw[5] * torque.z,
-                        Mw[6] * torque.x + Mw[7] * torque.y + Mw[8] * torque.z};
-            w = w + aw * dts;
-            if (d.mask >= 0) {  // Motion01Mask::constraint: cmptMultiply with 0/1 masks
-                v = {v.x * ((d.mask >> 0) & 1), v.y * ((d.mask >> 1) & 1), v.z * ((d.mask >> 2) & 1)};
-                w = {w.x * ((d.mask >> 3) & 1), w.y * ((d.mask >> 4) & 1), w.z * ((d.mask >> 5) & 1)};
-            }
-            const V3 c = V3{s.center[0], s.center[1], s.center[2]} + (0.5 * (v + v_old)) * dts;
-            const Quat hw{0.5 * 0.0, 0.5 * (0.5 * (w + w_old))};  // 0.5 * quaternion(0.5 (w + w_old))
-            const Quat dq = qmul(hw, q);
-            q = {q.w + dq.w * dts, q.v + dq.v * dts};
-            const double qm = std::sqrt(q.w * q.w + magSqr(q.v));
-            q = {q.w / qm, q.v / qm};
-            for (int a = 0; a < 3; ++a) {
-                s.vel[a] = (&v.x)[a];
-                s.omega[a] = (&w.x)[a];
-                s.center[a] = (&c.x)[a];
-            }
-            s.q[0] = q.w;
-            s.q[1] = q.v.x;
-            s.q[2] = q.v.y;
-            s.q[3] = q.v.z;
+            for (int64_t p = 0; p < ns; ++p)  // then the planes, ascending
+                if (solids[p].shape == 6) fc = fc + plane_force(solids[s], rb[s], solids[p], k, cdamp);
         }
-        // storeOldForce (solid.h:143-149)
-        for (int a = 0; a < 3; ++a) {
-            d.ff_old[a] = (&ff.x)[a];
-            d.ft_old[a] = (&ft.x)[a];
+        out[3 * s] = fc.x;
+        out[3 * s + 1] = fc.y;
+        out[3 * s + 2] = fc.z;
+    }
+}
+
+void orc_evolve_contact(int64_t ns, orc_solid* solids, orc_dyn* dyn, const double* fluid_ft, double time, double dt,
+                        const double* gravity, double rho_f, double k, double cdamp) {
+    (void)time;
+    const int n_sub = ns == 1 ? 1 : 20;
+    const double dts = dt / n_sub;
+    const V3 g{gravity[0], gravity[1], gravity[2]};
+    std::vector<double> fc(3 * ns);
+    std::vector<Body> body(ns);
+    for (int64_t id = 0; id < ns; ++id) body[id] = body_of(solids[id], dyn[id]);
+    for (int it = 0; it < n_sub; ++it) {
+        // contact forces from the state at the start of the sub-step, then every solid moves
+        orc_contact_forces(ns, solids, k, cdamp, fc.data());
+        for (int64_t id = 0; id < ns; ++id) {
+            const V3 f = cvec(&fc[3 * id]);
+            substep(solids[id], dyn[id], body[id], cvec(fluid_ft + 6 * id), cvec(fluid_ft + 6 * id + 3), g, rho_f,
+                    orc_bounding_radius(&solids[id]) > 0 ? &f : nullptr, dts);
         }
     }
+    for (int64_t id = 0; id < ns; ++id) store_old(dyn[id], cvec(fluid_ft + 6 * id), cvec(fluid_ft + 6 * id + 3));
 }
 
 }  // extern "C"

@@ -138,6 +138,18 @@ void orc_mass_properties(const orc_solid* s, double* volume, double* moi3);
 void orc_evolve(int64_t ns, orc_solid* solids, orc_dyn* dyn, const double* fluid_ft, double time, double dt,
                 const double* gravity, double rho_f);
 
+/* Contacts (this library's law; the reference's geometry): IShape::m_radiusB of
+ * a shape (plane: 0); the contact force on every solid by brute force over all
+ * pairs -- f = max(0, k overlap - cdamp v_n) along the line of centres for two
+ * finite solids (bounding spheres), along the plane normal for a plane; summed
+ * over finite partners in ascending index, then the planes; out [3*ns]; and
+ * evolve with those forces added after gravity in every sub-step.  k = cdamp =
+ * 0 gives orc_evolve. */
+double orc_bounding_radius(const orc_solid* s);
+void orc_contact_forces(int64_t ns, const orc_solid* solids, double k, double cdamp, double* out);
+void orc_evolve_contact(int64_t ns, orc_solid* solids, orc_dyn* dyn, const double* fluid_ft, double time, double dt,
+                        const double* gravity, double rho_f, double k, double cdamp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,6 +140,11 @@ class SolidCloud:
         d["motion_mask"] = motion_mask
         _lib.check(_lib.load().sdfibm_set_dynamics(self._h, _p(d), d.size), self._h)
 
+    def set_contact(self, stiffness: float, damping: float = 0.0):
+        """Contact spring-dashpot between solids during evolve (0, 0: no contacts, the
+        reference's behaviour at its HEAD; 1e4, 0: the reference's linear law)."""
+        _lib.check(_lib.load().sdfibm_set_contact(self._h, float(stiffness), float(damping)), self._h)
+
     def evolve(self, time: float, dt: float, gravity=(0.0, 0.0, 0.0)) -> None:
         g = np.ascontiguousarray(gravity, dtype=np.float64)
         _lib.check(_lib.load().sdfibm_evolve(self._h, float(time), float(dt), _p(g)), self._h)

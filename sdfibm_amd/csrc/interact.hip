@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1122,6 +1123,86 @@ __global__ void k_evolve(sdfibm_solid* __restrict__ S, SolidDyn* __restrict__ D,
     D[s] = d;
 }
 
+// Uniform grid of the contact search: cubic cells of twice the largest
+// bounding radius over the mesh's bounding box (the grid the reference sizes
+// UGrid(bbox, 2 m_radiusB) for, ugrid.cpp:5-22); centres outside the box fall
+// into the border cells, which keeps every pair closer than one cell size in
+// neighbouring cells.
+struct ContactGrid {
+    double lo[3];
+    double inv_cell;
+    int n[3];
+};
+
+__device__ inline int contact_cell(const ContactGrid& g, const double* c, int* ijk) {
+    for (int a = 0; a < 3; ++a) {
+        const double t = floor((c[a] - g.lo[a]) * g.inv_cell);
+        ijk[a] = t >= 0.0 ? (t < (double)(g.n[a] - 1) ? (int)t : g.n[a] - 1) : 0;  // NaN -> 0
+    }
+    return ijk[0] + g.n[0] * (ijk[1] + g.n[1] * ijk[2]);
+}
+
+// Cell lists of the finite solids (bounding radius > 0): head per cell, next per solid.
+__global__ void k_contact_insert(const sdfibm_solid* __restrict__ S, const double* __restrict__ rb, int ns,
+                                 ContactGrid g, int* __restrict__ head, int* __restrict__ next) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns || !(rb[s] > 0)) return;
+    int ijk[3];
+    const int cell = contact_cell(g, S[s].center, ijk);
+    next[s] = atomicExch(head + cell, s);
+}
+
+// One sub-step of SolidCloud::evolve with contacts, one thread per solid:
+// loads, contact forces from the state at the start of the sub-step (Sin),
+// Solid::move into Sout.  The contacts of a solid are summed in ascending
+// partner index (finite solids), then the planes in list order -- the sum does
+// not depend on the order the cell lists were built in.  Also resets the
+// other sub-step's cell heads (nobody reads them during this launch).
+__global__ void k_contact_step(const sdfibm_solid* __restrict__ Sin, sdfibm_solid* __restrict__ Sout,
+                               SolidDyn* __restrict__ D, const double* __restrict__ ft, const double* __restrict__ rb,
+                               int ns, ContactGrid g, const int* __restrict__ head, const int* __restrict__ next,
+                               int* __restrict__ head_reset, int ncells, const int* __restrict__ planes, int nplanes,
+                               ContactLaw law, double dts, Vec3 gv, double rho_f, int last) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int t = tid; t < ncells; t += gridDim.x * blockDim.x) head_reset[t] = -1;
+    if (tid >= ns) return;
+    const int s = tid;
+    sdfibm_solid sol = Sin[s];
+    SolidDyn d = D[s];
+    const BodyConst b = body_const(sol, d);
+    const double* f6 = ft + 6LL * s;
+    const d3 ff{f6[0], f6[1], f6[2]}, fq{f6[3], f6[4], f6[5]};
+    d3 force, torque;
+    substep_loads(d, ff, fq, d3{gv.v[0], gv.v[1], gv.v[2]}, rho_f, b, force, torque);
+    const double r = rb[s];
+    if (r > 0) {
+        d3 fc{0, 0, 0};
+        int ijk[3];
+        contact_cell(g, sol.center, ijk);
+        const int i0 = max(ijk[0] - 1, 0), i1 = min(ijk[0] + 1, g.n[0] - 1);
+        const int j0 = max(ijk[1] - 1, 0), j1 = min(ijk[1] + 1, g.n[1] - 1);
+        const int k0 = max(ijk[2] - 1, 0), k1 = min(ijk[2] + 1, g.n[2] - 1);
+        int prev = -1;
+        for (;;) {  // next partner in contact: the smallest index above prev
+            int best = INT_MAX;
+            for (int kk = k0; kk <= k1; ++kk)
+                for (int jj = j0; jj <= j1; ++jj)
+                    for (int ii = i0; ii <= i1; ++ii)
+                        for (int j = head[ii + g.n[0] * (jj + g.n[1] * kk)]; j >= 0; j = next[j])
+                            if (j > prev && j < best && j != s && pair_overlap(sol, r, Sin[j], rb[j]) > 0) best = j;
+            if (best == INT_MAX) break;
+            fc = fc + pair_force(sol, r, Sin[best], rb[best], law);
+            prev = best;
+        }
+        for (int p = 0; p < nplanes; ++p) fc = fc + plane_force(sol, r, Sin[planes[p]], law);
+        force = force + fc;
+    }
+    move_solid(sol, d, b, force, torque, dts);
+    if (last) store_old_force(d, ff, fq);
+    Sout[s] = sol;
+    D[s] = d;
+}
+
 template <class R>
 __global__ void k_aos_to_soa(const double* __restrict__ in, long long n, R* __restrict__ x, R* __restrict__ y,
                              R* __restrict__ z) {
@@ -1169,6 +1250,7 @@ struct sdfibm_ctx {
     MeshDev m{};   // device pointers
     MeshDev hm{};  // host pointers (tile estimates)
     std::vector<double> h_xs, h_ys, h_zs;
+    double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // the mesh's bounding box (contact grid)
     // device buffers
     std::vector<void*> allocs;
     void* U[3] = {nullptr, nullptr, nullptr};
@@ -1207,6 +1289,15 @@ struct sdfibm_ctx {
     float sg_lo = 0.02f, sg_hi = 0.98f, sg_cos = 0.8660254f;
     long long dyn_n = -1;       // solids the dynamics were set for
     long long dyn_cap = 0;
+    // contacts (set_contact): law, bounding radii, sub-step solid buffer, cell lists
+    ContactLaw law{0.0, 0.0};
+    double* d_rb = nullptr;
+    sdfibm_solid* d_solids_alt = nullptr;
+    int* d_chead = nullptr;  // two head arrays of ncells each
+    int* d_cnext = nullptr;
+    long long contact_cap = 0, chead_cap = 0;
+    ContactGrid cgrid{};
+    int ncells_c = 0;
     std::vector<int> h_planes;
     int* d_planes = nullptr;
     double* d_plane_partial = nullptr;
@@ -1467,6 +1558,10 @@ int setup_mesh(sdfibm_ctx* c, const sdfibm_mesh_desc* d) {
         m.xs = p[0];
         m.ys = p[1];
         m.zs = p[2];
+        for (int a = 0; a < 3; ++a) {
+            c->box_lo[a] = src[a]->front();
+            c->box_hi[a] = src[a]->back();
+        }
         c->hm = m;
         c->hm.xs = c->h_xs.data();
         c->hm.ys = c->h_ys.data();
@@ -1487,6 +1582,8 @@ int setup_mesh(sdfibm_ctx* c, const sdfibm_mesh_desc* d) {
             o[a] = P[a];
             h[a] = (P[3 * pe + a] - P[a]) / (double)nn[a];
             if (!(h[a] > 0)) return fail(c, SDFIBM_ERR_MESH, "structured mesh: corner points do not span a box");
+            c->box_lo[a] = o[a];
+            c->box_hi[a] = P[3 * pe + a];
         }
         double jit = 0.0;
         std::vector<double> soa(NP);
@@ -1744,6 +1841,11 @@ int sdfibm_set_stream(sdfibm_ctx* c, void* s) {
     return SDFIBM_OK;
 }
 
+namespace {
+bool contact_on(const sdfibm_ctx* c);
+int contact_prepare(sdfibm_ctx* c);
+}  // namespace
+
 int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if (!c) return SDFIBM_ERR_ARG;
     if (n < 0 || (n > 0 && !solids)) return fail(c, SDFIBM_ERR_ARG, "bad solid array");
@@ -1835,6 +1937,7 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     c->ns = n;
     c->have_solids = true;
     c->have_step = false;
+    if (contact_on(c)) return contact_prepare(c);
     return SDFIBM_OK;
 }
 
@@ -1981,6 +2084,71 @@ int sdfibm_set_dynamics(sdfibm_ctx* c, const sdfibm_dynamics* dyn, int64_t n) {
     return SDFIBM_OK;
 }
 
+namespace {
+
+bool contact_on(const sdfibm_ctx* c) { return c->law.k > 0 || c->law.c > 0; }
+
+// Bounding radii, the contact grid and its buffers for the current solids.
+int contact_prepare(sdfibm_ctx* c) {
+    const long long n = c->ns;
+    std::vector<double> rb(std::max<long long>(n, 1), 0.0);
+    double rmax = 0.0;
+    for (long long s = 0; s < n; ++s) {
+        rb[s] = bounding_radius(c->h_solids[s]);
+        rmax = std::max(rmax, rb[s]);
+    }
+    double cell = 2.0 * rmax, ext[3];
+    for (int a = 0; a < 3; ++a) ext[a] = c->box_hi[a] - c->box_lo[a];
+    if (!(cell > 0)) cell = std::max(ext[0], std::max(ext[1], ext[2]));
+    ContactGrid g{};
+    long long total = 0;
+    for (;;) {
+        total = 1;
+        for (int a = 0; a < 3; ++a) {
+            g.n[a] = (int)std::max(1.0, std::min(1.0e6, std::ceil(ext[a] / cell)));
+            total *= g.n[a];
+        }
+        if (total <= (1LL << 22)) break;
+        cell *= 1.25;  // coarser cells: more candidates per solid, never a missed pair
+    }
+    for (int a = 0; a < 3; ++a) g.lo[a] = c->box_lo[a];
+    g.inv_cell = 1.0 / cell;
+    int rc;
+    if (n > c->contact_cap) {
+        for (void* p : {(void*)c->d_rb, (void*)c->d_solids_alt, (void*)c->d_cnext}) dfree(c, p);
+        c->d_rb = nullptr;
+        c->d_solids_alt = nullptr;
+        c->d_cnext = nullptr;
+        c->contact_cap = 0;
+        if ((rc = dalloc(c, (void**)&c->d_rb, n * 8))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_solids_alt, n * sizeof(sdfibm_solid)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_cnext, n * 4))) return rc;
+        c->contact_cap = n;
+    }
+    if (total > c->chead_cap) {
+        dfree(c, c->d_chead);
+        c->d_chead = nullptr;
+        c->chead_cap = 0;
+        if ((rc = dalloc(c, (void**)&c->d_chead, 2 * total * 4))) return rc;
+        c->chead_cap = total;
+    }
+    if (n > 0) HIPCHK(c, hipMemcpy(c->d_rb, rb.data(), n * 8, hipMemcpyHostToDevice));
+    c->cgrid = g;
+    c->ncells_c = (int)total;
+    return SDFIBM_OK;
+}
+
+}  // namespace
+
+int sdfibm_set_contact(sdfibm_ctx* c, double stiffness, double damping) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!(stiffness >= 0) || !(damping >= 0) || !std::isfinite(stiffness) || !std::isfinite(damping))
+        return fail(c, SDFIBM_ERR_ARG, "contact stiffness and damping must be finite and >= 0");
+    c->law = ContactLaw{stiffness, damping};
+    if (contact_on(c) && c->have_solids) return contact_prepare(c);
+    return SDFIBM_OK;
+}
+
 int sdfibm_evolve(sdfibm_ctx* c, double time, double dt, const double* gravity) {
     (void)time;
     if (!c) return SDFIBM_ERR_ARG;
@@ -1992,7 +2160,24 @@ int sdfibm_evolve(sdfibm_ctx* c, double time, double dt, const double* gravity) 
     const int ns = (int)c->ns;
     if (ns == 0) return SDFIBM_OK;
     const int n_sub = ns == 1 ? 1 : 20;  // solidcloud.cpp:524-526
-    k_evolve<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_dyn, c->d_ft, ns, n_sub, dt, g, c->rho_f);
+    if (contact_on(c) && ns > 1) {
+        // sub-steps alternate between the two solid buffers (20: the result ends in d_solids)
+        k_fill_i32<<<grid_for(2LL * c->ncells_c, 256), 256, 0, c->stream>>>(c->d_chead, 2LL * c->ncells_c, -1);
+        sdfibm_solid* buf[2] = {c->d_solids, c->d_solids_alt};
+        const double dts = dt / n_sub;
+        const int nplanes = (int)c->h_planes.size();
+        for (int it = 0; it < n_sub; ++it) {
+            int* head = c->d_chead + (long long)(it & 1) * c->ncells_c;
+            int* other = c->d_chead + (long long)((it + 1) & 1) * c->ncells_c;
+            k_contact_insert<<<grid_for(ns, 64), 64, 0, c->stream>>>(buf[it & 1], c->d_rb, ns, c->cgrid, head,
+                                                                     c->d_cnext);
+            k_contact_step<<<grid_for(ns, 64), 64, 0, c->stream>>>(
+                buf[it & 1], buf[(it + 1) & 1], c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, head, c->d_cnext, other,
+                c->ncells_c, c->d_planes, nplanes, c->law, dts, g, c->rho_f, it == n_sub - 1);
+        }
+    } else {
+        k_evolve<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_dyn, c->d_ft, ns, n_sub, dt, g, c->rho_f);
+    }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_step = false;  // the tiles of the last interact no longer match the solids
