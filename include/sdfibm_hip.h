@@ -178,6 +178,20 @@ int sdfibm_evolve(sdfibm_ctx* ctx, double time, double dt, const double* gravity
  * lookup (which maps unknown shape names to Plane, collision.h:11-15). */
 int sdfibm_set_contact(sdfibm_ctx* ctx, double stiffness, double damping);
 
+/* n_steps coupled steps of the reference's solver loop (src/main.cpp:66-84
+ * without the PISO solve): interact -> U -= Fs dt -> evolve (with contacts if
+ * set) -> fixInternal, each step a replay of one HIP graph captured at the
+ * first call (re-captured when dt, gravity or any setting changes); no host
+ * synchronisation between the steps.  Before the replays one interact of the
+ * current state computes the plane solids' cell sets, which the replays keep:
+ * plane solids must have motion mask b000000 and zero velocity and spin
+ * (SDFIBM_ERR_STATE otherwise, and if a plane's pose changes during the
+ * replays).  Capacity errors of any step are reported after the last one; the
+ * state from the failing step on is then not valid.  ft: [ns*6] force/torque
+ * of the last step's interact, or NULL.  time: unused (no time-dependent
+ * motions or forcers).  sdfibm_timing's total is the replays' device time. */
+int sdfibm_run_steps(sdfibm_ctx* ctx, double time, double dt, int64_t n_steps, const double* gravity, double* ft);
+
 /* Device -> host copy of the solids' current records (after evolve). */
 int sdfibm_get_solids(sdfibm_ctx* ctx, sdfibm_solid* out, int64_t n);
 

@@ -149,6 +149,15 @@ class SolidCloud:
         g = np.ascontiguousarray(gravity, dtype=np.float64)
         _lib.check(_lib.load().sdfibm_evolve(self._h, float(time), float(dt), _p(g)), self._h)
 
+    def run_steps(self, time: float, dt: float, n_steps: int, gravity=(0.0, 0.0, 0.0)) -> np.ndarray:
+        """n_steps coupled steps (interact, U -= Fs dt, evolve, fixInternal) as replays of one
+        captured HIP graph; returns the last step's force/torque [ns, 6]."""
+        g = np.ascontiguousarray(gravity, dtype=np.float64)
+        ft = np.zeros((self.n_solids, 6))
+        _lib.check(_lib.load().sdfibm_run_steps(self._h, float(time), float(dt), int(n_steps), _p(g), _p(ft)),
+                   self._h)
+        return ft
+
     def solids(self) -> np.ndarray:
         out = np.zeros(self.n_solids, dtype=SOLID_DTYPE)
         _lib.check(_lib.load().sdfibm_get_solids(self._h, _p(out), out.size), self._h)

@@ -52,7 +52,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxPerBrick = 128;  // solids per brick handled by k_fields
 constexpr int kMaxPlanes = 8;
 
-enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_BRICK = 8 };
+enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_BRICK = 8, EF_PLANE_MOVED = 16 };
 
 // ---------------------------------------------------------------------------
 // small device helpers
@@ -1203,6 +1203,25 @@ __global__ void k_contact_step(const sdfibm_solid* __restrict__ Sin, sdfibm_soli
     D[s] = d;
 }
 
+// Graph replays: keep every step's error flags, and flag a plane solid whose
+// pose moved away from the one its membership bitmaps were computed for (the
+// quaternion normalisation of Solid::move may change a fixed plane's pose by
+// rounding; that much is accepted).
+__global__ void k_flag_or(const int* __restrict__ err, int* __restrict__ sticky) {
+    if (threadIdx.x == 0 && err[0] != 0) sticky[0] |= err[0];
+}
+
+__global__ void k_plane_static(const sdfibm_solid* __restrict__ S, const int* __restrict__ planes, int np,
+                               const double* __restrict__ pose, int* __restrict__ sticky) {
+    const int p = threadIdx.x;
+    if (p >= np) return;
+    const sdfibm_solid& s = S[planes[p]];
+    bool same = true;
+    for (int a = 0; a < 3; ++a) same = same && fabs(s.center[a] - pose[7 * p + a]) <= 1e-12 * (1.0 + fabs(pose[7 * p + a]));
+    for (int a = 0; a < 4; ++a) same = same && fabs(s.q[a] - pose[7 * p + 3 + a]) <= 1e-12;
+    if (!same) atomicOr(sticky, (int)EF_PLANE_MOVED);
+}
+
 template <class R>
 __global__ void k_aos_to_soa(const double* __restrict__ in, long long n, R* __restrict__ x, R* __restrict__ y,
                              R* __restrict__ z) {
@@ -1298,6 +1317,12 @@ struct sdfibm_ctx {
     long long contact_cap = 0, chead_cap = 0;
     ContactGrid cgrid{};
     int ncells_c = 0;
+    // the coupled step captured as a HIP graph (run_steps)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double graph_dt = 0.0, graph_g[3] = {0, 0, 0};
+    int* d_sticky = nullptr;
+    double* d_plane_pose = nullptr;  // center + q of each plane at capture
     std::vector<int> h_planes;
     int* d_planes = nullptr;
     double* d_plane_partial = nullptr;
@@ -1403,11 +1428,14 @@ int validate_solid(const sdfibm_solid& s, long long idx, std::string& why) {
     return SDFIBM_OK;
 }
 
+// graph: the launches are being captured into a HIP graph -- no timing events
+// and no plane labelling (it needs host round trips); the plane membership
+// bitmaps of the last eager pass are used, the planes being fixed.
 template <class R, int MK>
-int launch_step(sdfibm_ctx* c, double dt) {
+int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     const int ns = (int)c->ns;
     hipStream_t st = c->stream;
-    (void)hipEventRecord(c->ev[0], st);
+    if (!graph) (void)hipEventRecord(c->ev[0], st);
     k_fill_i32<<<1, 64, 0, st>>>(c->err, 1, 0);
     k_fill_i32<<<1, 64, 0, st>>>(c->qmeta, 8, 0);
     if (ns > 0) {
@@ -1424,7 +1452,7 @@ int launch_step(sdfibm_ctx* c, double dt) {
         else
             k_bin<kBX3, kBY3, kBZ3><<<gb, 256, 0, st>>>(ns, c->d_T, c->nbx, c->nby, c->bcount, c->d_blist, c->err);
     }
-    (void)hipEventRecord(c->ev[1], st);
+    if (!graph) (void)hipEventRecord(c->ev[1], st);
     R* codes = (R*)c->d_codes;
     if (ns > 0) {
         // persistent launches: a few workgroups per CU take solids from the class queues
@@ -1443,7 +1471,7 @@ int launch_step(sdfibm_ctx* c, double dt) {
         k_classify<R, MK, kRowsSmall, 0, kBlock><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
                                                                c->d_cand, c->code_cap, c->d_ncand);
         if (hipStreamWaitEvent(st, c->join, 0) != hipSuccess) return -1;
-        (void)hipEventRecord(c->ev[4], st);
+        if (!graph) (void)hipEventRecord(c->ev[4], st);
         constexpr bool kSgOk = MK == MK_RECT && sizeof(R) == sizeof(float);
         if (kSgOk && c->d_sg != nullptr) {
             k_fill_i32<<<1, 64, 0, st>>>(c->d_sg_count, 1, 0);
@@ -1458,10 +1486,10 @@ int launch_step(sdfibm_ctx* c, double dt) {
                                                         nullptr, 0.f, 0.f, 0.f, nullptr);
         }
     } else {
-        (void)hipEventRecord(c->ev[4], st);
+        if (!graph) (void)hipEventRecord(c->ev[4], st);
     }
     const int np = (int)c->h_planes.size();
-    for (int p = 0; p < np; ++p) {
+    for (int p = 0; p < (graph ? 0 : np); ++p) {
         const int sp = c->h_planes[p];
         unsigned* member = c->d_pmember + (long long)p * c->member_words;
         k_fill_i32<<<grid_for(c->member_words, 256), 256, 0, st>>>((int*)c->d_pact, c->member_words, 0);
@@ -1496,7 +1524,7 @@ int launch_step(sdfibm_ctx* c, double dt) {
         k_plane_member<<<g, 256, 0, st>>>(c->d_plist, c->d_pscalars, c->d_plabel, c->d_pact, c->d_T, sp,
                                           c->d_pscalars + 1, member, c->d_pseed + p);
     }
-    (void)hipEventRecord(c->ev[2], st);
+    if (!graph) (void)hipEventRecord(c->ev[2], st);
 #define SDFIBM_FIELDS(BX_, BY_, BZ_, CPT_, HP_)                                                                 \
     k_fields<R, MK, BX_, BY_, BZ_, CPT_, HP_><<<dim3(c->nbx, c->nby, c->nbz), BX_ * BY_ * BZ_ / CPT_, 0, st>>>(     \
         c->m, c->nbx, c->nby, c->bcount, c->d_blist, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np,          \
@@ -1516,7 +1544,7 @@ int launch_step(sdfibm_ctx* c, double dt) {
     }
 #undef SDFIBM_FIELDS
     if (np > 0) k_plane_reduce<<<np, 256, 0, st>>>(c->d_plane_partial, c->nbricks, c->d_planes, c->rho_f, c->d_ft);
-    (void)hipEventRecord(c->ev[3], st);
+    if (!graph) (void)hipEventRecord(c->ev[3], st);
     return 0;
 }
 
@@ -1674,20 +1702,89 @@ int setup_mesh(sdfibm_ctx* c, const sdfibm_mesh_desc* d) {
 }
 
 template <class R, int MK>
-int interact_impl(sdfibm_ctx* c, double dt) {
-    const int r = launch_step<R, MK>(c, dt);
+int interact_impl(sdfibm_ctx* c, double dt, bool graph) {
+    const int r = launch_step<R, MK>(c, dt, graph);
     HIPCHK(c, hipGetLastError());
     if (r == -1) return fail(c, SDFIBM_ERR_HIP, "plane component pass: HIP copy/synchronize failed");
     if (r == -2) return fail(c, SDFIBM_ERR_CAPACITY, "plane component labelling did not converge");
     return SDFIBM_OK;
 }
 
-int dispatch_interact(sdfibm_ctx* c, double dt) {
+int dispatch_interact(sdfibm_ctx* c, double dt, bool graph = false) {
     if (c->prec == SDFIBM_FP64)
-        return c->kind == SDFIBM_MESH_RECTILINEAR ? interact_impl<double, MK_RECT>(c, dt)
-                                                  : interact_impl<double, MK_STRUCT>(c, dt);
-    return c->kind == SDFIBM_MESH_RECTILINEAR ? interact_impl<float, MK_RECT>(c, dt)
-                                              : interact_impl<float, MK_STRUCT>(c, dt);
+        return c->kind == SDFIBM_MESH_RECTILINEAR ? interact_impl<double, MK_RECT>(c, dt, graph)
+                                                  : interact_impl<double, MK_STRUCT>(c, dt, graph);
+    return c->kind == SDFIBM_MESH_RECTILINEAR ? interact_impl<float, MK_RECT>(c, dt, graph)
+                                              : interact_impl<float, MK_STRUCT>(c, dt, graph);
+}
+
+bool contact_on(const sdfibm_ctx* c);
+int contact_prepare(sdfibm_ctx* c);
+
+// Drop the captured step (its launches hold the buffers and settings of capture time).
+void graph_reset(sdfibm_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+// Launch-only parts of the steps around interact (no synchronisation; also
+// used inside graph capture).
+void launch_update_velocity(sdfibm_ctx* c, double dt) {
+    const unsigned g = grid_for(c->N, 256);
+    if (c->prec == SDFIBM_FP64)
+        k_update_velocity<double><<<g, 256, 0, c->stream>>>(c->N, (double*)c->U[0], (double*)c->U[1], (double*)c->U[2],
+                                                            (const double*)c->Fs[0], (const double*)c->Fs[1],
+                                                            (const double*)c->Fs[2], dt);
+    else
+        k_update_velocity<float><<<g, 256, 0, c->stream>>>(c->N, (float*)c->U[0], (float*)c->U[1], (float*)c->U[2],
+                                                           (const float*)c->Fs[0], (const float*)c->Fs[1],
+                                                           (const float*)c->Fs[2], dt);
+}
+
+void launch_fix_internal(sdfibm_ctx* c) {
+    const unsigned g = grid_for(c->N, 256);
+    const int ns = (int)c->ns;
+#define SDFIBM_FIX(R_, MK_)                                                                                  \
+    k_fix_internal<R_, MK_><<<g, 256, 0, c->stream>>>(c->m, c->Ct, c->d_solids, ns, (R_*)c->U[0], (R_*)c->U[1], \
+                                                     (R_*)c->U[2])
+    if (c->prec == SDFIBM_FP64) {
+        if (c->kind == SDFIBM_MESH_RECTILINEAR)
+            SDFIBM_FIX(double, MK_RECT);
+        else
+            SDFIBM_FIX(double, MK_STRUCT);
+    } else {
+        if (c->kind == SDFIBM_MESH_RECTILINEAR)
+            SDFIBM_FIX(float, MK_RECT);
+        else
+            SDFIBM_FIX(float, MK_STRUCT);
+    }
+#undef SDFIBM_FIX
+}
+
+void launch_evolve(sdfibm_ctx* c, double dt, Vec3 g) {
+    const int ns = (int)c->ns;
+    if (ns == 0) return;
+    const int n_sub = ns == 1 ? 1 : 20;  // solidcloud.cpp:524-526
+    if (contact_on(c) && ns > 1) {
+        // sub-steps alternate between the two solid buffers (20: the result ends in d_solids)
+        k_fill_i32<<<grid_for(2LL * c->ncells_c, 256), 256, 0, c->stream>>>(c->d_chead, 2LL * c->ncells_c, -1);
+        sdfibm_solid* buf[2] = {c->d_solids, c->d_solids_alt};
+        const double dts = dt / n_sub;
+        const int nplanes = (int)c->h_planes.size();
+        for (int it = 0; it < n_sub; ++it) {
+            int* head = c->d_chead + (long long)(it & 1) * c->ncells_c;
+            int* other = c->d_chead + (long long)((it + 1) & 1) * c->ncells_c;
+            k_contact_insert<<<grid_for(ns, 64), 64, 0, c->stream>>>(buf[it & 1], c->d_rb, ns, c->cgrid, head,
+                                                                     c->d_cnext);
+            k_contact_step<<<grid_for(ns, 64), 64, 0, c->stream>>>(
+                buf[it & 1], buf[(it + 1) & 1], c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, head, c->d_cnext, other,
+                c->ncells_c, c->d_planes, nplanes, c->law, dts, g, c->rho_f, it == n_sub - 1);
+        }
+    } else {
+        k_evolve<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_dyn, c->d_ft, ns, n_sub, dt, g, c->rho_f);
+    }
 }
 
 std::string err_text(int flags) {
@@ -1695,6 +1792,7 @@ std::string err_text(int flags) {
     if (flags & EF_TILE) s += "a solid's cell tile exceeds 63 cells along x or 4096 vertex rows; ";
     if (flags & EF_CODECAP) s += "tile code buffer too small; ";
     if (flags & EF_BRICK) s += "more than 128 solids overlap one brick; ";
+    if (flags & EF_PLANE_MOVED) s += "a plane solid moved during a graph replay; ";
     return s;
 }
 
@@ -1814,6 +1912,8 @@ int sdfibm_destroy(sdfibm_ctx* c) {
     if (!c) return SDFIBM_ERR_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     for (auto& e : c->ev)
@@ -1828,6 +1928,7 @@ int sdfibm_destroy(sdfibm_ctx* c) {
 
 int sdfibm_set_stream(sdfibm_ctx* c, void* s) {
     if (!c) return SDFIBM_ERR_ARG;
+    graph_reset(c);
     if (c->own_stream && c->stream) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamDestroy(c->stream);
@@ -1841,13 +1942,9 @@ int sdfibm_set_stream(sdfibm_ctx* c, void* s) {
     return SDFIBM_OK;
 }
 
-namespace {
-bool contact_on(const sdfibm_ctx* c);
-int contact_prepare(sdfibm_ctx* c);
-}  // namespace
-
 int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if (!c) return SDFIBM_ERR_ARG;
+    graph_reset(c);
     if (n < 0 || (n > 0 && !solids)) return fail(c, SDFIBM_ERR_ARG, "bad solid array");
     if (n > (1LL << 30)) return fail(c, SDFIBM_ERR_ARG, "too many solids");
     std::vector<int> planes;
@@ -1979,6 +2076,7 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
         long long need = 0;
         HIPCHK(c, hipMemcpy(&need, c->d_tile_off + c->ns, 8, hipMemcpyDeviceToHost));
         const long long want = need + need / 4 + 1024;
+        graph_reset(c);
         dfree(c, c->d_codes);
         dfree(c, c->d_cand);
         c->d_codes = nullptr;
@@ -2017,15 +2115,7 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
 int sdfibm_update_velocity(sdfibm_ctx* c, double dt) {
     if (!c) return SDFIBM_ERR_ARG;
     if (!std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite");
-    const unsigned g = grid_for(c->N, 256);
-    if (c->prec == SDFIBM_FP64)
-        k_update_velocity<double><<<g, 256, 0, c->stream>>>(c->N, (double*)c->U[0], (double*)c->U[1], (double*)c->U[2],
-                                                            (const double*)c->Fs[0], (const double*)c->Fs[1],
-                                                            (const double*)c->Fs[2], dt);
-    else
-        k_update_velocity<float><<<g, 256, 0, c->stream>>>(c->N, (float*)c->U[0], (float*)c->U[1], (float*)c->U[2],
-                                                           (const float*)c->Fs[0], (const float*)c->Fs[1],
-                                                           (const float*)c->Fs[2], dt);
+    launch_update_velocity(c, dt);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SDFIBM_OK;
@@ -2034,23 +2124,7 @@ int sdfibm_update_velocity(sdfibm_ctx* c, double dt) {
 int sdfibm_fix_internal(sdfibm_ctx* c) {
     if (!c) return SDFIBM_ERR_ARG;
     if (!c->have_ct) return fail(c, SDFIBM_ERR_STATE, "fix_internal needs the Ct field of a successful interact");
-    const unsigned g = grid_for(c->N, 256);
-    const int ns = (int)c->ns;
-#define SDFIBM_FIX(R_, MK_)                                                                                  \
-    k_fix_internal<R_, MK_><<<g, 256, 0, c->stream>>>(c->m, c->Ct, c->d_solids, ns, (R_*)c->U[0], (R_*)c->U[1], \
-                                                     (R_*)c->U[2])
-    if (c->prec == SDFIBM_FP64) {
-        if (c->kind == SDFIBM_MESH_RECTILINEAR)
-            SDFIBM_FIX(double, MK_RECT);
-        else
-            SDFIBM_FIX(double, MK_STRUCT);
-    } else {
-        if (c->kind == SDFIBM_MESH_RECTILINEAR)
-            SDFIBM_FIX(float, MK_RECT);
-        else
-            SDFIBM_FIX(float, MK_STRUCT);
-    }
-#undef SDFIBM_FIX
+    launch_fix_internal(c);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SDFIBM_OK;
@@ -2058,6 +2132,7 @@ int sdfibm_fix_internal(sdfibm_ctx* c) {
 
 int sdfibm_set_dynamics(sdfibm_ctx* c, const sdfibm_dynamics* dyn, int64_t n) {
     if (!c) return SDFIBM_ERR_ARG;
+    graph_reset(c);
     if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "set_dynamics before set_solids");
     if (n != c->ns || (n > 0 && !dyn)) return fail(c, SDFIBM_ERR_ARG, "need one dynamics record per solid");
     std::vector<SolidDyn> h(n);
@@ -2142,6 +2217,7 @@ int contact_prepare(sdfibm_ctx* c) {
 
 int sdfibm_set_contact(sdfibm_ctx* c, double stiffness, double damping) {
     if (!c) return SDFIBM_ERR_ARG;
+    graph_reset(c);
     if (!(stiffness >= 0) || !(damping >= 0) || !std::isfinite(stiffness) || !std::isfinite(damping))
         return fail(c, SDFIBM_ERR_ARG, "contact stiffness and damping must be finite and >= 0");
     c->law = ContactLaw{stiffness, damping};
@@ -2157,35 +2233,122 @@ int sdfibm_evolve(sdfibm_ctx* c, double time, double dt, const double* gravity) 
     Vec3 g{{0.0, 0.0, 0.0}};
     if (gravity)
         for (int a = 0; a < 3; ++a) g.v[a] = gravity[a];
-    const int ns = (int)c->ns;
-    if (ns == 0) return SDFIBM_OK;
-    const int n_sub = ns == 1 ? 1 : 20;  // solidcloud.cpp:524-526
-    if (contact_on(c) && ns > 1) {
-        // sub-steps alternate between the two solid buffers (20: the result ends in d_solids)
-        k_fill_i32<<<grid_for(2LL * c->ncells_c, 256), 256, 0, c->stream>>>(c->d_chead, 2LL * c->ncells_c, -1);
-        sdfibm_solid* buf[2] = {c->d_solids, c->d_solids_alt};
-        const double dts = dt / n_sub;
-        const int nplanes = (int)c->h_planes.size();
-        for (int it = 0; it < n_sub; ++it) {
-            int* head = c->d_chead + (long long)(it & 1) * c->ncells_c;
-            int* other = c->d_chead + (long long)((it + 1) & 1) * c->ncells_c;
-            k_contact_insert<<<grid_for(ns, 64), 64, 0, c->stream>>>(buf[it & 1], c->d_rb, ns, c->cgrid, head,
-                                                                     c->d_cnext);
-            k_contact_step<<<grid_for(ns, 64), 64, 0, c->stream>>>(
-                buf[it & 1], buf[(it + 1) & 1], c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, head, c->d_cnext, other,
-                c->ncells_c, c->d_planes, nplanes, c->law, dts, g, c->rho_f, it == n_sub - 1);
-        }
-    } else {
-        k_evolve<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_dyn, c->d_ft, ns, n_sub, dt, g, c->rho_f);
-    }
+    if (c->ns == 0) return SDFIBM_OK;
+    launch_evolve(c, dt, g);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_step = false;  // the tiles of the last interact no longer match the solids
     return SDFIBM_OK;
 }
 
+int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, const double* gravity, double* ft) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "run_steps before set_solids");
+    if (c->dyn_n != c->ns || c->dyn_n < 0)
+        return fail(c, SDFIBM_ERR_STATE, "run_steps needs set_dynamics for these solids");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
+    if (n_steps < 0) return fail(c, SDFIBM_ERR_ARG, "n_steps must be >= 0");
+    Vec3 g{{0.0, 0.0, 0.0}};
+    if (gravity)
+        for (int a = 0; a < 3; ++a) g.v[a] = gravity[a];
+    if (!std::isfinite(g.v[0]) || !std::isfinite(g.v[1]) || !std::isfinite(g.v[2]))
+        return fail(c, SDFIBM_ERR_ARG, "gravity must be finite");
+    const int ns = (int)c->ns;
+    const int np = (int)c->h_planes.size();
+    // the plane memberships are computed once, before the replays: the planes must stay put
+    std::vector<double> pose(7 * std::max(np, 1));
+    for (int p = 0; p < np; ++p) {
+        sdfibm_solid ps;
+        SolidDyn pd;
+        HIPCHK(c, hipMemcpy(&ps, c->d_solids + c->h_planes[p], sizeof(ps), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(&pd, c->d_dyn + c->h_planes[p], sizeof(pd), hipMemcpyDeviceToHost));
+        bool still = pd.mask == 0;
+        for (int a = 0; a < 3; ++a) still = still && ps.vel[a] == 0.0 && ps.omega[a] == 0.0;
+        if (!still)
+            return fail(c, SDFIBM_ERR_STATE,
+                        "run_steps needs fixed plane solids (motion mask b000000, zero velocity and spin): plane " +
+                            std::to_string(c->h_planes[p]));
+        for (int a = 0; a < 3; ++a) pose[7 * p + a] = ps.center[a];
+        for (int a = 0; a < 4; ++a) pose[7 * p + 3 + a] = ps.q[a];
+    }
+    if (n_steps == 0) return SDFIBM_OK;
+    // an eager pass over the current state: plane memberships and the code buffer size
+    int rc = sdfibm_interact(c, time, dt, nullptr);
+    if (rc) return rc;
+    long long need = 0;
+    if (ns > 0) HIPCHK(c, hipMemcpy(&need, c->d_tile_off + ns, 8, hipMemcpyDeviceToHost));
+    const long long want = need + need / 2 + 4096;  // headroom: tiles grow as the solids move and turn
+    if (want > c->code_cap) {
+        graph_reset(c);
+        dfree(c, c->d_codes);
+        dfree(c, c->d_cand);
+        c->d_codes = nullptr;
+        c->d_cand = nullptr;
+        c->code_cap = 0;
+        if ((rc = dalloc(c, &c->d_codes, want * rsize(c)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_cand, want * 4))) return rc;
+        c->code_cap = want;
+    }
+    if (!c->d_sticky && (rc = dalloc(c, (void**)&c->d_sticky, 64))) return rc;
+    if (!c->d_plane_pose && (rc = dalloc(c, (void**)&c->d_plane_pose, 7 * kMaxPlanes * 8))) return rc;
+    if (np > 0) HIPCHK(c, hipMemcpy(c->d_plane_pose, pose.data(), 7 * np * 8, hipMemcpyHostToDevice));
+    if (c->gexec && (c->graph_dt != dt || c->graph_g[0] != g.v[0] || c->graph_g[1] != g.v[1] || c->graph_g[2] != g.v[2]))
+        graph_reset(c);
+    if (!c->gexec) {
+        // one coupled step (main.cpp:66-84 without the PISO solve), captured once
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int r = dispatch_interact(c, dt, true);
+        if (r == SDFIBM_OK) {
+            launch_update_velocity(c, dt);
+            launch_evolve(c, dt, g);
+            launch_fix_internal(c);
+            k_flag_or<<<1, 64, 0, c->stream>>>(c->err, c->d_sticky);
+            if (np > 0)
+                k_plane_static<<<1, 64, 0, c->stream>>>(c->d_solids, c->d_planes, np, c->d_plane_pose, c->d_sticky);
+        }
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        if (r != SDFIBM_OK) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return r;
+        }
+        if (e != hipSuccess) return fail(c, SDFIBM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(gr);
+            c->gexec = nullptr;
+            return fail(c, SDFIBM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+        c->graph = gr;
+        c->graph_dt = dt;
+        for (int a = 0; a < 3; ++a) c->graph_g[a] = g.v[a];
+    }
+    k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_sticky, 1, 0);
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int64_t t = 0; t < n_steps; ++t) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    int flags = 0;
+    HIPCHK(c, hipMemcpyAsync(&flags, c->d_sticky, 4, hipMemcpyDeviceToHost, c->stream));
+    if (ft && ns > 0) HIPCHK(c, hipMemcpyAsync(ft, c->d_ft, ns * 48, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[3]);
+    for (double& t : c->timing) t = 0.0;
+    c->timing[3] = ms;
+    c->have_step = false;  // the solids moved after the last interact
+    c->have_ct = flags == 0;
+    if (flags)
+        return fail(c,
+                    (flags & EF_TILE)          ? SDFIBM_ERR_TILE
+                    : (flags & EF_PLANE_MOVED) ? SDFIBM_ERR_STATE
+                                               : SDFIBM_ERR_CAPACITY,
+                    err_text(flags) + "the state after the first failing step is not valid");
+    return SDFIBM_OK;
+}
+
 int sdfibm_set_surrogate(sdfibm_ctx* c, const float* params, double band_lo, double band_hi, double max_angle_deg) {
     if (!c) return SDFIBM_ERR_ARG;
+    graph_reset(c);
     if (!params) {
         dfree(c, c->d_sg);
         c->d_sg = nullptr;
