@@ -148,6 +148,106 @@ def surrogate_comparison(cloud, sc, ft0, As0, plain_ms, plain_cells_ms, steps):
     return out
 
 
+def bench_ellipsoids_loop(args):
+    """Jittered-hex ellipsoid cloud with six walls: the coupled step loop (interact, U -= Fs dt,
+    evolve, fixInternal) as replays of one captured HIP graph, against the same steps issued one
+    call at a time."""
+    import torch
+
+    from sdfibm_amd import scenes
+    from sdfibm_amd.cloud import SolidCloud, motion_mask
+
+    n, nsol = args.n or 200, args.solids or 2000
+    t0 = time.time()
+    sc = scenes.ellipsoids_jittered(n, nsol)
+    ns = sc.solids.size
+    log(f"[bench] scene {sc.name} ready in {time.time() - t0:.1f}s")
+    dt, g = sc.dt, np.zeros(3)
+    steps = args.loop_steps or 500
+    rho = np.full(ns, 1.5)
+    masks = np.full(ns, -1, dtype=np.int32)
+    masks[nsol:] = motion_mask("b000000")
+    cloud = SolidCloud(sc.mesh, rho_f=sc.rho_f, precision=args.precision)
+    cloud.set_solids(sc.solids)
+    cloud.set_velocity(sc.U)
+    cloud.set_dynamics(rho, masks)
+    if args.contact:
+        cloud.set_contact(*args.contact)
+    log(f"[bench] uploaded in {time.time() - t0:.1f}s")
+    # per-kernel phases of one interact of the initial state
+    cloud.interact(0.0, dt)
+    ph = np.zeros(6)
+    for _ in range(5):
+        cloud.interact_device(0.0, dt)
+        ph += cloud.timing()
+    ph /= 5
+    f = cloud.fields()
+    nc, nv = touched_stats(sc.mesh, f["Ct"])
+    cloud.run_steps(0.0, dt, max(args.warmup, 1), g)
+    batches = 5
+    per = max(1, steps // batches)
+    pairs = [sum(cloud.pair_counts())]
+    dev_ms = 0.0
+    t = time.perf_counter()
+    for b in range(batches):
+        cloud.run_steps(b * per * dt, dt, per, g)
+        dev_ms += cloud.timing()[3]
+        pairs.append(sum(cloud.pair_counts()))
+    wall_ms = 1e3 * (time.perf_counter() - t)
+    steps = batches * per
+    ms_per_step = dev_ms / steps
+    n_pairs = float(np.mean(pairs))
+    # the same step issued call by call (each call synchronises)
+    k_eager = min(50, steps)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(k_eager):
+        cloud.interact_device(k * dt, dt)
+        cloud.update_velocity(dt)
+        cloud.evolve(k * dt, dt, g)
+        cloud.fix_internal()
+    torch.cuda.synchronize()
+    eager_ms = 1e3 * (time.perf_counter() - t) / k_eager
+    fin = cloud.solids()
+    cloud.close()
+    rb = 4 if args.precision == "fp32" else 8
+    N = sc.mesh.n_cells
+    kernels = {"k_fields": {"ms": ph[2], "bytes": (5 * rb + 4) * N + 3 * rb * nc},
+               "k_cells": {"ms": ph[5], "bytes": 3 * rb * nc + 3 * rb * nv}}
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    achieved = kernels[dom]["bytes"] / (kernels[dom]["ms"] * 1e-3)
+    line = {
+        "metric": METRIC, "value": n_pairs / (ms_per_step * 1e-3), "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic (seeded generator)",
+        "config": {"workload": f"ellipsoids3d_jittered {n}^3 hex (interior vertices jittered U(+-0.2h)), {nsol} "
+                               f"ellipsoids (semi-axes U[3,10] cells, at rest, omega ~ N(0,0.5)) + 6 fixed plane walls, "
+                               f"coupled step loop (interact, U -= Fs dt, evolve, fixInternal) as HIP graph "
+                               f"replays, dt {dt}, {args.precision} fields",
+                   "cells": N, "solids": int(ns), "candidate_pairs_mean": n_pairs,
+                   "candidate_pairs_samples": pairs, "contact": list(args.contact) if args.contact else None,
+                   "parallelism": "single GPU"},
+        "ms_per_step_value_basis": "device time of the graph replays (HIP events), per coupled step",
+        "wall_ms_per_step": wall_ms / steps,
+        "eager_ms_per_step": eager_ms, "graph_speedup_vs_eager": eager_ms / ms_per_step,
+        "phases_ms_initial_interact": {"setup_bin": ph[0], "k_classify": ph[4], "k_cells": ph[5], "k_fields": ph[2]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(dom, f"ellipsoids3d_jittered_{n}_{nsol}_{args.precision}"),
+                     "algorithmic_bytes": kernels[dom]["bytes"]},
+        "surrogate": "not applicable: the learned fraction serves cubic cells of rectilinear meshes only",
+        "mean_displacement": float(np.mean(np.linalg.norm(fin["center"][:nsol] - sc.solids["center"][:nsol], axis=1))),
+    }
+    if not args.no_cpu:
+        rate, ns_t, pairs_t, el = cpu_baseline(sc, min(args.cpu_sample, 200), args.cpu_budget)
+        line["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                                "sample": f"interact of the first {ns_t} of {ns} solids on the full mesh "
+                                          f"({pairs_t} cell-updates, {el:.1f}s), reference-cost bookkeeping, "
+                                          f"FP64, 1 thread",
+                                "speedup": line["value"] / rate}
+    return line
+
+
 def bench_sedimentation(args):
     """The CPU-runnable case: one interact and a loop of full coupling steps, FP64."""
     import torch
@@ -185,6 +285,12 @@ def bench_sedimentation(args):
     torch.cuda.synchronize()
     ms_step = 1e3 * (time.perf_counter() - t) / args.loop_steps
     fin = cloud.solids()
+    # the same loop continued as replays of one captured HIP graph
+    cloud.run_steps(args.loop_steps * dt, dt, 2, g)  # capture + warm-up
+    t = time.perf_counter()
+    cloud.run_steps((args.loop_steps + 2) * dt, dt, args.loop_steps, g)
+    ms_graph = 1e3 * (time.perf_counter() - t) / args.loop_steps
+    dev_graph = cloud.timing()[3] / args.loop_steps
     cloud.close()
     line = {
         "metric": METRIC, "value": n_pairs / (ms_interact * 1e-3), "unit": "cell-updates/s", "n_gpus": 1,
@@ -194,7 +300,9 @@ def bench_sedimentation(args):
                                "g = -10 y, dt 5e-4, fp64", "cells": sc.mesh.n_cells, "solids": int(ns),
                    "candidate_pairs": n_pairs, "parallelism": "single GPU"},
         "loop": {"steps": args.loop_steps, "ms_per_coupling_step": ms_step,
-                 "mean_disc_drop": float(np.mean(sc.solids["center"][:100, 1] - fin["center"][:100, 1]))},
+                 "mean_disc_drop": float(np.mean(sc.solids["center"][:100, 1] - fin["center"][:100, 1])),
+                 "graph_ms_per_coupling_step": ms_graph, "graph_device_ms_per_step": dev_graph,
+                 "graph_speedup": ms_step / ms_graph},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -223,8 +331,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scene-cache", default="", help="npz file to reuse the generated scene")
-    ap.add_argument("--workload", default="cloud3d", choices=["cloud3d", "sedimentation2d"])
-    ap.add_argument("--loop-steps", type=int, default=200, help="sedimentation2d: coupling steps timed")
+    ap.add_argument("--workload", default="cloud3d", choices=["cloud3d", "sedimentation2d", "ellipsoids3d_jittered"])
+    ap.add_argument("--loop-steps", type=int, default=None,
+                    help="coupled steps timed (sedimentation2d: 200, ellipsoids3d_jittered: 500)")
+    ap.add_argument("--contact", type=float, nargs=2, default=None, metavar=("K", "C"),
+                    help="ellipsoids3d_jittered: contact stiffness and damping (default: no contacts)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="cloud3d fp32: skip the learned-fraction comparison run")
     ap.add_argument("--copy-test", type=float, default=0.0,
@@ -256,7 +367,13 @@ def main():
         return
 
     if args.workload == "sedimentation2d":
+        args.loop_steps = args.loop_steps or 200
         print(json.dumps(bench_sedimentation(args)), flush=True)
+        return
+    if args.workload == "ellipsoids3d_jittered":
+        if args.n == 256 and args.solids == 10000:  # the cloud3d defaults
+            args.n, args.solids = 200, 2000
+        print(json.dumps(bench_ellipsoids_loop(args)), flush=True)
         return
 
     t0 = time.time()

@@ -266,6 +266,13 @@ int sdfibm_field_pointers(sdfibm_ctx* ctx, void** U3 /* [3]: Ux,Uy,Uz */, void**
 int sdfibm_candidates(sdfibm_ctx* ctx, int64_t solid, int64_t* counts, int64_t* cells,
                       double* alpha, int64_t capacity);
 
+/* Totals over all solids of the candidate sets of the last interact (also
+ * after run_steps: its last step): out[0] = ALL_INSIDE pairs, out[1] = cut
+ * pairs (CENTER_INSIDE + CENTER_OUTSIDE); their sum is the number of
+ * (solid, cell) updates, sum_s |cellids_s| of solidcloud.cpp:403-422.  For
+ * benchmarks. */
+int sdfibm_pair_counts(sdfibm_ctx* ctx, int64_t* out);
+
 /* Timing of the phases of the last interact (milliseconds, HIP events on the
  * context's stream): [0] setup+binning, [1] per-solid tiles (classification,
  * cut cells and forcing, plane labelling), [2] field pass, [3] total,

@@ -17,7 +17,7 @@ EXPORTS = (
     "sdfibm_interact", "sdfibm_get_fields", "sdfibm_field_pointers", "sdfibm_candidates",
     "sdfibm_last_timing", "sdfibm_set_stream", "sdfibm_last_error", "sdfibm_status_string",
     "sdfibm_abi_version", "sdfibm_update_velocity", "sdfibm_fix_internal", "sdfibm_get_velocity",
-    "sdfibm_set_dynamics", "sdfibm_evolve", "sdfibm_get_solids", "sdfibm_set_contact", "sdfibm_run_steps",
+    "sdfibm_set_dynamics", "sdfibm_evolve", "sdfibm_get_solids", "sdfibm_set_contact", "sdfibm_run_steps", "sdfibm_pair_counts",
     "sdfibm_surrogate_train", "sdfibm_surrogate_samples", "sdfibm_surrogate_eval", "sdfibm_surrogate_grad",
     "sdfibm_set_surrogate", "sdfibm_surrogate_count",
 )
@@ -86,6 +86,7 @@ def load():
         "sdfibm_get_solids": (i32, [P, P, i64]),
         "sdfibm_set_contact": (i32, [P, d, d]),
         "sdfibm_run_steps": (i32, [P, d, d, i64, P, P]),
+        "sdfibm_pair_counts": (i32, [P, P]),
         "sdfibm_surrogate_train": (i32, [P, P, P]),
         "sdfibm_surrogate_samples": (i32, [C.c_uint64, i64, P, P]),
         "sdfibm_surrogate_eval": (i32, [P, P, i64, P]),

@@ -205,6 +205,12 @@ class SolidCloud:
                    self._h)
         return counts, cells, alpha
 
+    def pair_counts(self) -> tuple:
+        """(ALL_INSIDE pairs, cut pairs) of the last interact, summed over all solids."""
+        out = np.zeros(2, dtype=np.int64)
+        _lib.check(_lib.load().sdfibm_pair_counts(self._h, _p(out)), self._h)
+        return int(out[0]), int(out[1])
+
     def timing(self) -> np.ndarray:
         """Phase times of the last interact in ms: setup+binning, tile pass, field pass, total,
         tile classification, cut cells + forcing (+ plane labelling)."""

@@ -2556,4 +2556,36 @@ int sdfibm_candidates(sdfibm_ctx* c, int64_t solid, int64_t* counts, int64_t* ce
     return SDFIBM_OK;
 }
 
+int sdfibm_pair_counts(sdfibm_ctx* c, int64_t* out) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!out) return fail(c, SDFIBM_ERR_ARG, "out is NULL");
+    if (!c->have_ct) return fail(c, SDFIBM_ERR_STATE, "pair_counts before a successful interact");
+    out[0] = out[1] = 0;
+    const long long ns = c->ns;
+    if (ns == 0) return SDFIBM_OK;
+    std::vector<int2> nc(ns);
+    HIPCHK(c, hipMemcpy(nc.data(), c->d_ncand, ns * sizeof(int2), hipMemcpyDeviceToHost));
+    std::vector<char> is_plane(ns, 0);
+    for (int p : c->h_planes) is_plane[p] = 1;
+    for (long long s = 0; s < ns; ++s)
+        if (!is_plane[s]) {
+            out[0] += nc[s].x;
+            out[1] += nc[s].y;
+        }
+    const bool had = c->have_step;
+    c->have_step = true;  // the plane cell sets of the last interact are still in place
+    for (int p : c->h_planes) {
+        int64_t cnt[4] = {0, 0, 0, 0};
+        const int rc = sdfibm_candidates(c, p, cnt, nullptr, nullptr, 0);
+        if (rc) {
+            c->have_step = had;
+            return rc;
+        }
+        out[0] += cnt[0];
+        out[1] += cnt[1] + cnt[2];
+    }
+    c->have_step = had;
+    return SDFIBM_OK;
+}
+
 }  // extern "C"

@@ -218,21 +218,23 @@ def _hex_cell_geometry(fx_c, fx_s, fy_c, fy_s, fz_c, fz_s, nx, ny, nz):
     fam = {"-x": (xm_c, xm_s), "+x": (xp_c, xp_s), "-y": (ym_c, ym_s), "+y": (yp_c, yp_s),
            "-z": (zm_c, zm_s), "+z": (zp_c, zp_s)}
     # cEst: mean of face centres, summed in the cell's face order
+    faces = [[(name, mask) for name, mask in slot if mask.any()] for slot in faces]
     cest = np.zeros(shape + (3,))
     for slot in faces:
         for name, mask in slot:
-            c = fam[name][0]
-            cest = np.where(mask[..., None], cest + c, cest)
+            np.add(cest, fam[name][0], out=cest, where=mask[..., None])
     cest = cest / 6.0
     ctr = np.zeros(shape + (3,))
     vol = np.zeros(shape)
+    for name in fam:  # each face of a cell once: pyramid volume x3 and weighted centre
+        c, s = fam[name]
+        pyr3 = _dot(s, c - cest)
+        fam[name] = (pyr3, pyr3[..., None] * ((3.0 / 4.0) * c + (1.0 / 4.0) * cest))
     for slot in faces:
         for name, mask in slot:
-            c, s = fam[name]
-            pyr3 = _dot(s, c - cest)
-            pc = (3.0 / 4.0) * c + (1.0 / 4.0) * cest
-            ctr = np.where(mask[..., None], ctr + pyr3[..., None] * pc, ctr)
-            vol = np.where(mask, vol + pyr3, vol)
+            pyr3, wc = fam[name]
+            np.add(ctr, wc, out=ctr, where=mask[..., None])
+            np.add(vol, pyr3, out=vol, where=mask)
     ctr = ctr / vol[..., None]
     vol = vol * (1.0 / 3.0)
     return ctr, vol

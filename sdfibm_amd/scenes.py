@@ -8,6 +8,10 @@ cloud_3d           uniform n^3 hex mesh on the unit cube, spheres and ellipsoids
                    diameters (cells), random non-overlapping placement (largest first), random
                    unit quaternions, v ~ N(0, 0.1), omega ~ N(0, 1), and a divergence-free
                    Fourier velocity field (16 modes, rms 1).
+ellipsoids_jittered n^3 hex mesh of the unit cube with interior vertices jittered by U(+-0.2 h)
+                   per axis, non-overlapping ellipsoids with semi-axes U[3, 10] cells,
+                   omega ~ N(0, 5), six fixed plane walls (floor and ceiling 1.5 h inside the
+                   faces, the side walls on them), Fourier velocity field (rms 0.05).
 """
 from __future__ import annotations
 
@@ -58,33 +62,24 @@ def fourier_velocity(points: np.ndarray, seed: int, n_modes: int = 16, kmax: int
     return U / rms
 
 
-def cloud_3d(n: int = 256, n_solids: int = 10000, frac_sphere: float = 0.7, d_median: float = 8.0,
-             d_sigma: float = 0.45, d_min: float = 4.0, d_max: float = 24.0, seed: int = DEFAULT_SEED,
-             with_velocity: bool = True, gap_cells: float = 0.0, max_tries: int = 4000) -> Scene:
-    """Spheres + ellipsoids in the unit cube (sizes in cells of h = 1/n)."""
-    rng = _rng(seed)
-    h = 1.0 / n
-    mesh = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
-    d = np.exp(rng.normal(math.log(d_median), d_sigma, size=n_solids))
-    while np.any((d < d_min) | (d > d_max)):  # truncate by resampling
-        bad = (d < d_min) | (d > d_max)
-        d[bad] = np.exp(rng.normal(math.log(d_median), d_sigma, size=int(bad.sum())))
-    is_sphere = rng.random(n_solids) < frac_sphere
-    ratios = rng.uniform(0.5, 1.0, size=(n_solids, 2))
-    order = np.argsort(-d, kind="stable")
-    radius_b = 0.5 * d * h  # bounding radius (max semi-axis)
+def place_nonoverlapping(rng, radius_b, lo, hi, gap: float = 0.0, max_tries: int = 4000) -> np.ndarray:
+    """Random sequential placement of bounding spheres inside the box lo..hi, largest first,
+    rejecting overlaps (plus `gap`) through a uniform-grid hash of the placed centres."""
+    n_solids = radius_b.size
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    order = np.argsort(-radius_b, kind="stable")
     pos = np.full((n_solids, 3), np.nan)
-    # uniform-grid hashing of placed centres
-    cell = 2.0 * radius_b.max() + gap_cells * h
-    ng = max(1, int(1.0 / cell))
+    cell = 2.0 * radius_b.max() + gap
+    ng = max(1, int(float((hi - lo).min()) / cell))
     grid: dict = {}
     placed_r = np.zeros(n_solids)
     for idx in order:
         r = radius_b[idx]
         ok = False
         for _attempt in range(max_tries // 64):
-            cand = rng.uniform(r, 1.0 - r, size=(64, 3))
-            gi = np.minimum((cand * ng).astype(np.int64), ng - 1)
+            cand = rng.uniform(lo + r, hi - r, size=(64, 3))
+            gi = np.minimum(((cand - lo) / (hi - lo) * ng).astype(np.int64), ng - 1)
             good = np.ones(64, dtype=bool)
             for c in range(64):
                 key0 = gi[c]
@@ -95,7 +90,7 @@ def cloud_3d(n: int = 256, n_solids: int = 10000, frac_sphere: float = 0.7, d_me
                             if not lst:
                                 continue
                             P = pos[lst]
-                            rr = placed_r[lst] + r + gap_cells * h
+                            rr = placed_r[lst] + r + gap
                             if np.any(np.sum((P - cand[c]) ** 2, axis=1) < rr * rr):
                                 good[c] = False
                                 break
@@ -112,7 +107,25 @@ def cloud_3d(n: int = 256, n_solids: int = 10000, frac_sphere: float = 0.7, d_me
             if ok:
                 break
         if not ok:
-            raise RuntimeError(f"could not place solid {idx} (d={d[idx]:.2f} cells) without overlap")
+            raise RuntimeError(f"could not place solid {idx} (radius {r:.4g}) without overlap")
+    return pos
+
+
+def cloud_3d(n: int = 256, n_solids: int = 10000, frac_sphere: float = 0.7, d_median: float = 8.0,
+             d_sigma: float = 0.45, d_min: float = 4.0, d_max: float = 24.0, seed: int = DEFAULT_SEED,
+             with_velocity: bool = True, gap_cells: float = 0.0, max_tries: int = 4000) -> Scene:
+    """Spheres + ellipsoids in the unit cube (sizes in cells of h = 1/n)."""
+    rng = _rng(seed)
+    h = 1.0 / n
+    mesh = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
+    d = np.exp(rng.normal(math.log(d_median), d_sigma, size=n_solids))
+    while np.any((d < d_min) | (d > d_max)):  # truncate by resampling
+        bad = (d < d_min) | (d > d_max)
+        d[bad] = np.exp(rng.normal(math.log(d_median), d_sigma, size=int(bad.sum())))
+    is_sphere = rng.random(n_solids) < frac_sphere
+    ratios = rng.uniform(0.5, 1.0, size=(n_solids, 2))
+    radius_b = 0.5 * d * h  # bounding radius (max semi-axis)
+    pos = place_nonoverlapping(rng, radius_b, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), gap_cells * h, max_tries)
     sol = S.make_solids(n_solids)
     sol["center"] = pos
     sol["q"] = S.random_unit_quaternions(rng, n_solids)
@@ -226,3 +239,116 @@ def small_mixed_3d(n: int = 24, seed: int = 7, n_each: int = 2, jitter: float = 
         sol[t]["omega"] = rng.normal(0, 1.0, 3)
     U = rng.normal(0, 1.0, size=(mesh.n_cells, 3))
     return Scene(f"mixed3d_{n}", mesh, sol, U, False, 1.3, 1e-3, dict(seed=seed))
+
+
+def walls_3d(inset_y: float, inset_side: float = 0.0) -> np.ndarray:
+    """Six plane solids of the unit cube, normals (local +y) into the fluid, the solid side of each
+    the half-space beyond it: floor and ceiling `inset_y` inside the y faces, the four side walls
+    `inset_side` inside theirs.  (A cell inside two walls is forced by both every step, which
+    the explicit loop without a fluid solve does not survive where three walls meet: keep the
+    side walls on the faces, where they own no cell, when the floor and ceiling are inset.)"""
+    w = S.make_solids(6)
+    w["shape"] = S.PLANE
+    a, b = inset_y, inset_side
+    spec = (((0.5, a, 0.5), (0.0, 0.0, 0.0)), ((0.5, 1 - a, 0.5), (0.0, 0.0, np.pi)),
+            ((b, 0.5, 0.5), (0.0, 0.0, -np.pi / 2)), ((1 - b, 0.5, 0.5), (0.0, 0.0, np.pi / 2)),
+            ((0.5, 0.5, b), (np.pi / 2, 0.0, 0.0)), ((0.5, 0.5, 1 - b), (-np.pi / 2, 0.0, 0.0)))
+    for t, (c, eul) in enumerate(spec):
+        w["center"][t] = c
+        w["q"][t] = S.quaternion_xyz(eul)
+    return w
+
+
+def _quat_R(q):
+    """Rotation matrices [n, 3, 3] of unit quaternions (w, x, y, z)."""
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    return np.stack([np.stack([w * w + x * x - y * y - z * z, 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                     np.stack([2 * (x * y + w * z), w * w - x * x + y * y - z * z, 2 * (y * z - w * x)], -1),
+                     np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), w * w - x * x - y * y + z * z], -1)], -2)
+
+
+def place_ellipsoids(rng, axes, R, lo, hi, gap: float, max_tries: int = 20000) -> np.ndarray:
+    """Random sequential placement of oriented ellipsoids (semi-axes `axes`, rotations `R`) in the
+    box lo..hi, largest first, each grown by gap/2 per axis for the test.  Overlap test: the
+    Perram-Wertheim contact function F(l) = l (1 - l) r^T [(1 - l) A + l B]^-1 r, A = R diag(a^2)
+    R^T; two ellipsoids are apart iff max over l in [0, 1] of F >= 1 (sampled at 33 values of
+    l, which can only under-estimate the maximum: a sampled pass is never an overlap)."""
+    n = axes.shape[0]
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    ax = axes + 0.5 * gap
+    M = np.einsum("nij,nj,nkj->nik", R, ax ** 2, R)
+    rb = ax.max(axis=1)
+    lam = np.linspace(0.0, 1.0, 33)[1:-1]
+    order = np.argsort(-rb, kind="stable")
+    cell = 2.0 * rb.max()
+    ng = max(1, int(float((hi - lo).min()) / cell))
+    grid: dict = {}
+    pos = np.full((n, 3), np.nan)
+    for idx in order:
+        r = axes[idx].max()
+        ok = False
+        for _attempt in range(max_tries // 64):
+            cand = rng.uniform(lo + r, hi - r, size=(64, 3))
+            gi = np.minimum(((cand - lo) / (hi - lo) * ng).astype(np.int64), ng - 1)
+            for c in range(64):
+                nb = []
+                for dz in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dx in (-1, 0, 1):
+                            lst = grid.get((gi[c][0] + dx, gi[c][1] + dy, gi[c][2] + dz))
+                            if lst:
+                                nb.extend(lst)
+                if nb:
+                    nb = np.asarray(nb)
+                    dvec = pos[nb] - cand[c]
+                    near = np.linalg.norm(dvec, axis=1) < rb[nb] + rb[idx]
+                    if np.any(near):
+                        nb, dvec = nb[near], dvec[near]
+                        S_ = (1 - lam)[None, :, None, None] * M[idx][None, None] + \
+                            lam[None, :, None, None] * M[nb][:, None]  # [m, L, 3, 3]
+                        x = np.linalg.solve(S_, np.broadcast_to(dvec[:, None, :, None], S_.shape[:2] + (3, 1)))
+                        F = lam * (1 - lam) * np.einsum("mi,mli->ml", dvec, x[..., 0])
+                        if np.any(F.max(axis=1) < 1.0):
+                            continue
+                pos[idx] = cand[c]
+                grid.setdefault(tuple(gi[c]), []).append(idx)
+                ok = True
+                break
+            if ok:
+                break
+        if not ok:
+            raise RuntimeError(f"could not place ellipsoid {idx} without overlap")
+    return pos
+
+
+def ellipsoids_jittered(n: int = 200, n_solids: int = 2000, jitter: float = 0.2, a_min: float = 3.0,
+                        a_max: float = 10.0, omega_sigma: float = 0.5, u_rms: float = 0.02,
+                        seed: int = DEFAULT_SEED) -> Scene:
+    """Non-overlapping ellipsoids (semi-axes U[a_min, a_max] cells, at least h/2 apart) among six
+    fixed walls on a jittered hex mesh; the walls are the last six solids.  Overlapping solids
+    force their shared cells twice per step (U -= (Fs_A + Fs_B) dt), which the explicit loop
+    without a fluid solve does not survive; so the packing starts apart and the spin and flow
+    are gentle (omega ~ N(0, 0.5), rms |U| 0.02): at omega ~ N(0, 5) the packed ellipsoids
+    turn into each other within ~100 steps."""
+    rng = _rng(seed)
+    h = 1.0 / n
+    mesh = M.jittered_box((n, n, n), (0, 0, 0), (1, 1, 1), jitter, seed + 11)
+    axes = rng.uniform(a_min, a_max, size=(n_solids, 3)) * h
+    inset = 1.5 * h
+    q = S.random_unit_quaternions(rng, n_solids)
+    pos = place_ellipsoids(rng, axes, _quat_R(q), (inset + 0.5 * h,) * 3, (1 - inset - 0.5 * h,) * 3, 0.5 * h,
+                           max_tries=200000)
+    sol = S.make_solids(n_solids)
+    sol["shape"] = S.ELLIPSOID
+    sol["prm"][:, :3] = axes
+    sol["center"] = pos
+    sol["q"] = q
+    sol["vel"] = 0.0  # the solids start at rest and are carried by the flow
+    sol["omega"] = rng.normal(0.0, omega_sigma, size=(n_solids, 3))
+    sol = np.concatenate([sol, walls_3d(inset, 0.0)])
+    U = u_rms * fourier_velocity(mesh_cell_centres(mesh), seed + 1)
+    vol = float(np.sum(4.0 / 3.0 * np.pi * np.prod(axes, axis=1)))
+    info = dict(n=n, n_solids=n_solids, n_walls=6, seed=seed, jitter=jitter, solid_volume_fraction=vol,
+                semi_axes_cells=(a_min, a_max))
+    return Scene(f"ellipsoids_jittered_{n}_{n_solids}", mesh, sol, U, False, 1.0, 1e-3, info)

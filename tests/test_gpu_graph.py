@@ -94,3 +94,19 @@ def test_moving_plane_is_refused():
         c.set_dynamics(2.0, masks)
         with pytest.raises(SdfibmError, match="fixed plane"):  # still has a velocity
             c.run_steps(0.0, 1e-3, 2, (0, -9.81, 0))
+
+
+def test_pair_counts_total_the_candidate_sets():
+    sc = scenes.small_mixed_3d(24, seed=6, n_each=2)
+    sol = np.concatenate([sc.solids, _walls(0.08)])
+    with SolidCloud(sc.mesh, rho_f=1.0, precision="fp64") as c:
+        c.set_solids(sol)
+        c.set_velocity(sc.U)
+        c.interact(0.0, 1e-3)
+        inside = cut = 0
+        for s in range(sol.size):
+            counts, _, _ = c.candidates(s)
+            inside += int(counts[0])
+            cut += int(counts[1] + counts[2])
+        assert c.pair_counts() == (inside, cut)
+        assert inside > 0 and cut > 0
