@@ -159,7 +159,17 @@ def bench_ellipsoids_loop(args):
 
     n, nsol = args.n or 200, args.solids or 2000
     t0 = time.time()
-    sc = scenes.ellipsoids_jittered(n, nsol)
+    sc = None
+    if args.scene_cache and os.path.exists(args.scene_cache):
+        z = np.load(args.scene_cache, allow_pickle=False)
+        if int(z["n"]) == n and int(z["ns"]) == nsol:
+            from sdfibm_amd import mesh as M
+            mesh = M.jittered_box((n,) * 3, (0, 0, 0), (1, 1, 1), 0.2, scenes.DEFAULT_SEED + 11)
+            sc = scenes.Scene(f"ellipsoids_jittered_{n}_{nsol}", mesh, z["solids"], z["U"], False, 1.0, 1e-3, {})
+    if sc is None:
+        sc = scenes.ellipsoids_jittered(n, nsol)
+        if args.scene_cache:
+            np.savez(args.scene_cache, n=n, ns=nsol, solids=sc.solids, U=sc.U)
     ns = sc.solids.size
     log(f"[bench] scene {sc.name} ready in {time.time() - t0:.1f}s")
     dt, g = sc.dt, np.zeros(3)

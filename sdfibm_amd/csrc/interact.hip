@@ -138,7 +138,8 @@ __global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, 
     const SolidK k = make_solid_k(in[s]);
     K[s] = k;
     TileInfo t = make_tile<MK>(m, k);
-    t.seed = t.cls == TC_EMPTY ? -1 : nearest_cell<MK>(m, k.c);
+    // structured meshes: the seed search (a box of cell centres) is spread over a wave by k_seed
+    t.seed = t.cls == TC_EMPTY ? -1 : (MK == MK_RECT ? nearest_cell<MK>(m, k.c) : -2);
     if (t.cls == TC_TOOBIG) atomicOr(err, EF_TILE);
     tile_cells[s] = tile_is_finite(t.cls) ? (long long)t.n[0] * t.n[1] * t.n[2] : 0;
     T[s] = t;
@@ -156,6 +157,52 @@ __global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, 
         base = __shfl(base, leader, 64);
         if (q == qq) qlist[(long long)qq * ns + base + __popcll(b & ((1ull << lane) - 1ull))] = s;
     }
+}
+
+// The seed cell of every non-empty solid on a structured mesh (nearest_cell's
+// rule, one wave per solid): the smallest cell index among the cells whose
+// centre lies within a relative 1e-10 of the smallest squared distance, over
+// the box of cells the jitter allows around the nominal cell.
+__global__ void __launch_bounds__(256) k_seed(MeshDev m, int ns, const SolidK* __restrict__ Ks,
+                                              TileInfo* __restrict__ Ts) {
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= ns || Ts[s].seed != -2) return;  // wave-uniform
+    const double* p = Ks[s].c;
+    const int nn[3] = {m.nx, m.ny, m.nz};
+    int c0[3];
+    for (int d = 0; d < 3; ++d) {
+        double f = floor((p[d] - m.o[d]) / m.h[d]);
+        if (f < 0) f = 0;
+        if (f > nn[d] - 1) f = nn[d] - 1;
+        c0[d] = (int)f;
+    }
+    const int r = (int)ceil(2.0 * m.jit) + 1, w = 2 * r + 1, box = w * w * w;
+    double best = __builtin_huge_val();
+    for (int t = lane; t < box; t += 64) {
+        const int i = c0[0] - r + t % w, j = c0[1] - r + (t / w) % w, k = c0[2] - r + t / (w * w);
+        if (i < 0 || i >= m.nx || j < 0 || j >= m.ny || k < 0 || k >= m.nz) continue;
+        const long long c = cell_id(m, i, j, k);
+        const double dx = m.ccx[c] - p[0], dy = m.ccy[c] - p[1], dz = m.ccz[c] - p[2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < best) best = d2;
+    }
+    for (int off = 32; off >= 1; off >>= 1) best = fmin(best, __shfl_xor(best, off, 64));
+    const double lim = best * (1.0 + kNearTie) + 1e-300;
+    long long bc = LLONG_MAX;
+    for (int t = lane; t < box; t += 64) {
+        const int i = c0[0] - r + t % w, j = c0[1] - r + (t / w) % w, k = c0[2] - r + t / (w * w);
+        if (i < 0 || i >= m.nx || j < 0 || j >= m.ny || k < 0 || k >= m.nz) continue;
+        const long long c = cell_id(m, i, j, k);
+        const double dx = m.ccx[c] - p[0], dy = m.ccy[c] - p[1], dz = m.ccz[c] - p[2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 <= lim && c < bc) bc = c;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const long long o = __shfl_xor(bc, off, 64);
+        bc = o < bc ? o : bc;
+    }
+    if (lane == 0) Ts[s].seed = bc == LLONG_MAX ? -1 : (int)bc;
 }
 
 // Exclusive scan of n values in three passes: tile sums, a scan of the tile
@@ -798,6 +845,34 @@ __global__ void k_plane_member(const int* __restrict__ list, const int* __restri
     if (t == 0) *seed_used = s;
 }
 
+// Codes of a plane's member cells (ALL_INSIDE: +inf; cut: the fraction, negative
+// when the centre is outside), computed once per plane labelling so that the
+// field pass only reads them.  Cells that are not members are left as they are.
+template <class R, int MK>
+__global__ void k_plane_alpha(MeshDev m, const int* __restrict__ list, const int* __restrict__ nlist,
+                              const SolidK* __restrict__ Ks, int solid, const unsigned* __restrict__ member,
+                              R* __restrict__ pcode) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *nlist) return;
+    const int c = list[t];
+    if (!bit_get(member, c)) return;
+    const int i = c % m.nx, j = (c / m.nx) % m.ny, k = c / (m.nx * m.ny);
+    const SolidK K = Ks[solid];
+    d3 P[8];
+    cell_vertices<MK>(m, i, j, k, P);
+    int nin = 0;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) nin += inside_world(K, P[v]) ? 1 : 0;
+    R code = code_none<R>();
+    if (nin == 8) {
+        code = R(__builtin_huge_val());
+    } else if (nin > 0) {
+        const R a = cell_fraction<R, MK>(m, K, i, j, k);
+        code = inside_world(K, cell_centre<MK>(m, i, j, k)) ? a : -a;
+    }
+    pcode[c] = code;
+}
+
 // Field pass: one workgroup per brick of BX*BY*BZ cells, CPT cells per thread
 // (the thread's cells are CPT layers apart along z: lane (ti, tj) of layer tk
 // takes layers tk, tk + BZ/CPT, ...).  Each wave covers 64/BX whole x-rows of
@@ -808,7 +883,7 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
     k_fields(MeshDev m, int nbx, int nby, const int* __restrict__ bcount, const int* __restrict__ blist,
              const SolidK* __restrict__ Ks, const TileInfo* __restrict__ Ts,
              const R* __restrict__ codes, long long code_cap, const int* __restrict__ planes, int nplanes,
-             const unsigned* __restrict__ plane_member, long long member_words,
+             const unsigned* __restrict__ plane_member, long long member_words, const R* __restrict__ pcodes,
              const double* __restrict__ brick_box, const R* __restrict__ Ux, const R* __restrict__ Uy,
              const R* __restrict__ Uz, R* __restrict__ As, R* __restrict__ Fx, R* __restrict__ Fy,
              R* __restrict__ Fz, R* __restrict__ Tsf, int* __restrict__ Ct, double dt,
@@ -820,7 +895,6 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
     __shared__ int raw[kMaxPerBrick];
     __shared__ int lst[kMaxPerBrick];
     __shared__ double red[6 * 16];
-    __shared__ int pl_any[kMaxPlanes];
 
     // grid (nbx, nby, nbz): no integer division for the brick coordinates
     const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
@@ -857,7 +931,6 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
     }
     if (tid < min(cnt, kSpec)) raw[tid] = spec;
     for (int t = kSpec + tid; t < cnt; t += NT) raw[t] = bl[t];
-    if (tid < kMaxPlanes) pl_any[tid] = 0;
     __syncthreads();
     for (int t = tid; t < cnt; t += NT) {  // rank sort (solid ids are distinct)
         const int v = raw[t];
@@ -884,14 +957,8 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
         sA[q] = sT[q] = sF[q][0] = sF[q][1] = sF[q][2] = 0.0;
         ct[q] = 0;
     }
-    constexpr int NPACC = HASP ? kMaxPlanes : 1;
-    double pacc[NPACC][6];
-#pragma unroll
-    for (int p = 0; p < NPACC; ++p)
-#pragma unroll
-        for (int t = 0; t < 6; ++t) pacc[p][t] = 0.0;
-
-    auto contribute = [&](int q, const double* cvw, double alpha, int ctype, int plane_slot) {
+    // contribute: a solid's share in cell q; a plane also adds its force and torque to pa
+    auto contribute = [&](int q, const double* cvw, double alpha, int ctype, double* pa) {
         const d3 r = {cc[q].x - cvw[0], cc[q].y - cvw[1], cc[q].z - cvw[2]};
         const d3 us = d3{cvw[3], cvw[4], cvw[5]} + cross(d3{cvw[6], cvw[7], cvw[8]}, r);
         const d3 f = alpha * (uf[q] - us);
@@ -901,9 +968,9 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
         sF[q][1] += f.y * dtINV;
         sF[q][2] += f.z * dtINV;
         ct[q] = ctype;
-        if (HASP && plane_slot >= 0) {
+        if (HASP && pa != nullptr) {
             const d3 tq = cross(r, f);
-            double* a = pacc[plane_slot];
+            double* a = pa;
             a[0] += (f.x * V[q]) * dtINV;
             a[1] += (f.y * V[q]) * dtINV;
             a[2] += (f.z * V[q]) * dtINV;
@@ -967,17 +1034,24 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
                 const R cq = code[q];
                 if (cq != cq) continue;
                 if (isinf(cq))
-                    contribute(q, cvw, 1.0, sa + 4, -1);
+                    contribute(q, cvw, 1.0, sa + 4, nullptr);
                 else
-                    contribute(q, cvw, (double)fabs(cq), signbit(cq) ? 3 : 2, -1);
+                    contribute(q, cvw, (double)fabs(cq), signbit(cq) ? 3 : 2, nullptr);
             }
         } else if constexpr (HASP) {
             const int slot = ip;
             ++ip;
             const SolidK K = Ks[sp];
             // plane: only cells of the seed's component (member bitmap); cull the brick first
+            // (uniform over the workgroup: the brick's vertex box)
             const int st = plane_box_state(K, bb);
-            if (st == 0) continue;  // no vertex of the brick can be inside
+            double* part = plane_partial + ((long long)slot * nbricks + b) * 6;
+            if (st == 0) {
+                if (tid == 0)
+                    for (int t = 0; t < 6; ++t) part[t] = 0.0;
+                continue;  // no vertex of the brick can be inside
+            }
+            double pa[6] = {0, 0, 0, 0, 0, 0};
             const double cvw[9] = {K.c[0], K.c[1], K.c[2], K.v[0], K.v[1], K.v[2], K.w[0], K.w[1], K.w[2]};
 #pragma unroll
             for (int q = 0; q < CPT; ++q) {
@@ -985,24 +1059,16 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
                 const int k = kq[q];
                 const long long c = cell_id(m, i, j, k);
                 if (!bit_get(plane_member + (long long)slot * member_words, c)) continue;
-                int nin = 8;
-                if (st == 2) {
-                    d3 P[8];
-                    cell_vertices<MK>(m, i, j, k, P);
-                    nin = 0;
-#pragma unroll
-                    for (int v = 0; v < 8; ++v) nin += inside_world(K, P[v]) ? 1 : 0;
-                }
-                if (nin == 0) continue;
-                pl_any[slot] = 1;
-                if (nin == 8) {
-                    contribute(q, cvw, 1.0, sp + 4, slot);
-                } else {
-                    const R alpha = cell_fraction<R, MK>(m, K, i, j, k);
-                    const bool cin = inside_world(K, cell_centre<MK>(m, i, j, k));
-                    contribute(q, cvw, (double)alpha, cin ? 2 : 3, slot);
-                }
+                const R pc = pcodes[(long long)slot * m.nx * m.ny * m.nz + c];  // k_plane_alpha
+                if (pc != pc) continue;
+                if (isinf(pc))
+                    contribute(q, cvw, 1.0, sp + 4, pa);
+                else
+                    contribute(q, cvw, (double)fabs(pc), signbit(pc) ? 3 : 2, pa);
             }
+            block_reduce<6>(pa, red);  // the brick's share of the plane's force and torque
+            if (tid == 0)
+                for (int t = 0; t < 6; ++t) part[t] = pa[t];
         }
     }
 
@@ -1018,21 +1084,6 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
         __builtin_nontemporal_store(R(sT[q]), &Tsf[c]);
         __builtin_nontemporal_store(ct[q], &Ct[c]);
     }
-    if (HASP && nplanes > 0) {
-        __syncthreads();
-        for (int p = 0; p < nplanes; ++p) {
-            if (!pl_any[p]) {
-                if (tid == 0)
-                    for (int t = 0; t < 6; ++t) plane_partial[((long long)p * nbricks + b) * 6 + t] = 0.0;
-                continue;
-            }
-            double v6[6];
-            for (int t = 0; t < 6; ++t) v6[t] = pacc[p][t];
-            block_reduce<6>(v6, red);
-            if (tid == 0)
-                for (int t = 0; t < 6; ++t) plane_partial[((long long)p * nbricks + b) * 6 + t] = v6[t];
-        }
-    }
 }
 
 __global__ void k_plane_reduce(const double* __restrict__ partial, long long nbricks, const int* __restrict__ planes,
@@ -1040,10 +1091,10 @@ __global__ void k_plane_reduce(const double* __restrict__ partial, long long nbr
     __shared__ double red[6 * 16];
     const int p = blockIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    const long long chunk = (nbricks + blockDim.x - 1) / blockDim.x;
-    const long long b0 = threadIdx.x * chunk, b1 = b0 + chunk < nbricks ? b0 + chunk : nbricks;
-    for (long long b = b0; b < b1; ++b)
-        for (int t = 0; t < 6; ++t) acc[t] += partial[((long long)p * nbricks + b) * 6 + t];
+    // thread t sums bricks t, t + blockDim, ... (neighbouring threads read neighbouring bricks)
+    const double* base = partial + (long long)p * nbricks * 6;
+    for (long long b = threadIdx.x; b < nbricks; b += blockDim.x)
+        for (int t = 0; t < 6; ++t) acc[t] += base[b * 6 + t];
     block_reduce<6>(acc, red);
     if (threadIdx.x == 0)
         for (int t = 0; t < 6; ++t) ft[6 * planes[p] + t] = acc[t] * rho_f;
@@ -1332,6 +1383,7 @@ struct sdfibm_ctx {
     int* d_plabel = nullptr;
     int* d_pscalars = nullptr;  // [0] list size, [1] lowest active index, [2] changed flag
     unsigned* d_pmember = nullptr;
+    void* d_pcode = nullptr;  // per plane: codes of its member cells (N each)
     int* d_pseed = nullptr;
     long long member_words = 0;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -1441,6 +1493,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     if (ns > 0) {
         k_setup<MK><<<grid_for(ns, 64), 64, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_cells,
                                                        c->d_ft, c->qmeta, c->qlist, c->err);
+        if (MK != MK_RECT) k_seed<<<grid_for(ns, 4), 256, 0, st>>>(c->m, ns, c->d_K, c->d_T);
         scan<long long>(c->d_tile_cells, ns, c->d_tile_off, c->scan_tmp, st);
         k_tile_offsets<<<grid_for(ns, 128), 128, 0, st>>>(c->d_T, c->d_tile_off, ns, c->code_cap, c->err);
     }
@@ -1523,12 +1576,15 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
         if (changed) return -2;
         k_plane_member<<<g, 256, 0, st>>>(c->d_plist, c->d_pscalars, c->d_plabel, c->d_pact, c->d_T, sp,
                                           c->d_pscalars + 1, member, c->d_pseed + p);
+        k_plane_alpha<R, MK><<<g, 256, 0, st>>>(c->m, c->d_plist, c->d_pscalars, c->d_K, sp, member,
+                                                (R*)c->d_pcode + (long long)p * c->N);
     }
     if (!graph) (void)hipEventRecord(c->ev[2], st);
 #define SDFIBM_FIELDS(BX_, BY_, BZ_, CPT_, HP_)                                                                 \
     k_fields<R, MK, BX_, BY_, BZ_, CPT_, HP_><<<dim3(c->nbx, c->nby, c->nbz), BX_ * BY_ * BZ_ / CPT_, 0, st>>>(     \
         c->m, c->nbx, c->nby, c->bcount, c->d_blist, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np,          \
-        c->d_pmember, c->member_words, c->brick_box, (const R*)c->U[0], (const R*)c->U[1], (const R*)c->U[2],      \
+        c->d_pmember, c->member_words, (const R*)c->d_pcode, c->brick_box, (const R*)c->U[0], (const R*)c->U[1],   \
+        (const R*)c->U[2],                                                                                         \
         (R*)c->As, (R*)c->Fs[0], (R*)c->Fs[1], (R*)c->Fs[2], (R*)c->Tsf, c->Ct, dt, c->d_plane_partial, c->nbricks, \
         c->err)
     if (c->nz == 1) {
@@ -1543,7 +1599,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
             SDFIBM_FIELDS(kBX3, kBY3, kBZ3, kCPT3, false);
     }
 #undef SDFIBM_FIELDS
-    if (np > 0) k_plane_reduce<<<np, 256, 0, st>>>(c->d_plane_partial, c->nbricks, c->d_planes, c->rho_f, c->d_ft);
+    if (np > 0) k_plane_reduce<<<np, 1024, 0, st>>>(c->d_plane_partial, c->nbricks, c->d_planes, c->rho_f, c->d_ft);
     if (!graph) (void)hipEventRecord(c->ev[3], st);
     return 0;
 }
@@ -2018,11 +2074,14 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     }
     dfree(c, c->d_pmember);
     dfree(c, c->d_pseed);
+    dfree(c, c->d_pcode);
     c->d_pmember = nullptr;
     c->d_pseed = nullptr;
+    c->d_pcode = nullptr;
     if (!planes.empty()) {
         if ((rc = dalloc(c, (void**)&c->d_pmember, planes.size() * c->member_words * 4))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_pseed, planes.size() * 4))) return rc;
+        if ((rc = dalloc(c, &c->d_pcode, (long long)planes.size() * c->N * rsize(c)))) return rc;
     }
     c->h_planes = planes;
     c->h_solids.assign(solids, solids + n);
