@@ -346,6 +346,8 @@ def main():
                     help="coupled steps timed (sedimentation2d: 200, ellipsoids3d_jittered: 500)")
     ap.add_argument("--contact", type=float, nargs=2, default=None, metavar=("K", "C"),
                     help="ellipsoids3d_jittered: contact stiffness and damping (default: no contacts)")
+    ap.add_argument("--small-spheres", action="store_true",
+                    help="cloud3d: spheres of uniform diameter 2-4 cells instead of the lognormal mix")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="cloud3d fp32: skip the learned-fraction comparison run")
     ap.add_argument("--copy-test", type=float, default=0.0,
@@ -390,15 +392,19 @@ def main():
     sc = None
     if args.scene_cache and os.path.exists(args.scene_cache):
         z = np.load(args.scene_cache, allow_pickle=False)
-        if int(z["n"]) == args.n and int(z["ns"]) == args.solids:
+        small = int(z["small"]) if "small" in z.files else 0
+        if int(z["n"]) == args.n and int(z["ns"]) == args.solids and small == int(args.small_spheres):
             from sdfibm_amd import mesh as M
             sc = scenes.Scene(f"cloud3d_{args.n}_{args.solids}",
                               M.uniform_box((args.n,) * 3, (0, 0, 0), (1, 1, 1)), z["solids"], z["U"],
                               False, 1.0, 1e-3, json.loads(str(z["info"])))
     if sc is None:
-        sc = scenes.cloud_3d(args.n, args.solids)
+        if args.small_spheres:  # 2-4 cell spheres, uniform diameters
+            sc = scenes.cloud_3d(args.n, args.solids, frac_sphere=1.0, d_min=2.0, d_max=4.0, d_uniform=True)
+        else:
+            sc = scenes.cloud_3d(args.n, args.solids)
         if args.scene_cache:
-            np.savez(args.scene_cache, n=args.n, ns=args.solids, solids=sc.solids, U=sc.U,
+            np.savez(args.scene_cache, n=args.n, ns=args.solids, small=int(args.small_spheres), solids=sc.solids, U=sc.U,
                      info=json.dumps({k: (float(v) if isinstance(v, (np.floating,)) else v)
                                       for k, v in sc.info.items()}))
     log(f"[bench] scene {sc.name} ready in {time.time() - t0:.1f}s: {sc.info}")
@@ -454,8 +460,10 @@ def main():
         "vs_baseline": None,
         "dtype": args.precision,
         "data": "synthetic (seeded generator)",
-        "config": {"workload": f"cloud3d {args.n}^3 hex, {args.solids} solids (70% sphere / 30% ellipsoid), "
-                               f"plain method, {args.precision} fields, fp64 reductions",
+        "config": {"workload": f"cloud3d {args.n}^3 hex, {args.solids} solids "
+                               + ("(spheres, d ~ U[2,4] cells), " if args.small_spheres else
+                                  "(70% sphere / 30% ellipsoid), ")
+                               + f"plain method, {args.precision} fields, fp64 reductions",
                    "cells": N, "solids": int(sc.solids.size), "candidate_pairs": n_pairs,
                    "inside_pairs": n_inside, "cut_pairs": n_cut, "touched_cells": nc,
                    "touched_vertices": nv, "solid_volume_fraction": float(sc.info["solid_volume_fraction"]),

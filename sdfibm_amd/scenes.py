@@ -113,12 +113,17 @@ def place_nonoverlapping(rng, radius_b, lo, hi, gap: float = 0.0, max_tries: int
 
 def cloud_3d(n: int = 256, n_solids: int = 10000, frac_sphere: float = 0.7, d_median: float = 8.0,
              d_sigma: float = 0.45, d_min: float = 4.0, d_max: float = 24.0, seed: int = DEFAULT_SEED,
-             with_velocity: bool = True, gap_cells: float = 0.0, max_tries: int = 4000) -> Scene:
-    """Spheres + ellipsoids in the unit cube (sizes in cells of h = 1/n)."""
+             with_velocity: bool = True, gap_cells: float = 0.0, max_tries: int = 4000,
+             d_uniform: bool = False) -> Scene:
+    """Spheres + ellipsoids in the unit cube (sizes in cells of h = 1/n).  Diameters lognormal
+    (median, sigma) truncated to [d_min, d_max], or uniform on [d_min, d_max] (d_uniform)."""
     rng = _rng(seed)
     h = 1.0 / n
     mesh = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
-    d = np.exp(rng.normal(math.log(d_median), d_sigma, size=n_solids))
+    if d_uniform:
+        d = rng.uniform(d_min, d_max, size=n_solids)
+    else:
+        d = np.exp(rng.normal(math.log(d_median), d_sigma, size=n_solids))
     while np.any((d < d_min) | (d > d_max)):  # truncate by resampling
         bad = (d < d_min) | (d > d_max)
         d[bad] = np.exp(rng.normal(math.log(d_median), d_sigma, size=int(bad.sum())))
