@@ -123,8 +123,9 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
     if (i < n) p[i] = v;
 }
 
-// Work queues of the per-solid tile pass, one per tile class (small, medium,
-// large): qmeta[q] = number of solids queued, qlist[q * ns + t] = the t-th of them.
+// Work queues of the per-solid tile pass, one per tile class (0 small, 1 medium,
+// 2 large; k_setup moves small tiles of at most kRowsTiny vertex rows to 3, tiny):
+// qmeta[q] = number of solids queued, qlist[q * ns + t] = the t-th of them.
 __host__ __device__ inline int tile_queue(int cls) {
     return cls == TC_SMALL ? 0 : (cls == TC_MEDIUM ? 1 : (cls == TC_LARGE ? 2 : -1));
 }
@@ -146,9 +147,10 @@ __global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, 
     for (int q = 0; q < 6; ++q) ft[6 * s + q] = 0.0;  // solids that touch no cell exert nothing
     // queue the solid: one atomic per wave and queue (the lanes of a wave are
     // consecutive solids, queued in index order)
-    const int q = tile_queue(t.cls);
+    int q = tile_queue(t.cls);
+    if (q == 0 && (t.n[1] + 1) * (t.n[2] + 1) <= kRowsTiny) q = 3;  // tiny tiles: a wave each
     const int lane = threadIdx.x & 63;
-    for (int qq = 0; qq < 3; ++qq) {
+    for (int qq = 0; qq < 4; ++qq) {
         const unsigned long long b = __ballot(q == qq);
         if (!b) continue;
         const int leader = __builtin_ctzll(b);
@@ -361,10 +363,13 @@ __device__ inline void block_exscan2(int a, int b, int& pa, int& pb, int& ta, in
 
 // Solid of item t of the tile pass, largest tile class first.
 __device__ inline int queue_item(const int* qmeta, const int* qlist, int ns, int t) {
-    const int n2 = qmeta[2], n1 = qmeta[1];
+    const int n2 = qmeta[2], n1 = qmeta[1], n0 = qmeta[0];
     if (t < n2) return qlist[2LL * ns + t];
-    if (t < n2 + n1) return qlist[(long long)ns + (t - n2)];
-    return qlist[t - n2 - n1];
+    t -= n2;
+    if (t < n1) return qlist[1LL * ns + t];
+    t -= n1;
+    if (t < n0) return qlist[t];
+    return qlist[3LL * ns + (t - n0)];
 }
 
 // Candidate entry: the cell's tile-local (i, j, k) packed as i | j << 6 | k << 19
@@ -572,8 +577,8 @@ __global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK
 // sg_hi] while the centre gradient and the corner-difference gradient differ by
 // more than the angle whose cosine is sg_cos; then, and for every other cell,
 // the exact box fraction.
-template <class R, int MK, bool SG>
-__global__ void __launch_bounds__(kBlock) k_cells(MeshDev m, int ns, const SolidK* __restrict__ Ks,
+template <class R, int MK, bool SG, int NT>
+__global__ void __launch_bounds__(NT) k_cells(MeshDev m, int ns, const SolidK* __restrict__ Ks,
                                                   const TileInfo* __restrict__ Ts, int* __restrict__ qmeta,
                                                   const int* __restrict__ qlist, const int* __restrict__ cand,
                                                   const int2* __restrict__ ncand, const R* __restrict__ Ux,
@@ -584,10 +589,10 @@ __global__ void __launch_bounds__(kBlock) k_cells(MeshDev m, int ns, const Solid
     __shared__ double red[6 * 16];
     constexpr int NSG = SG ? 1 : 0;
     __shared__ SgNet sn[NSG ? 1 : 1];
-    __shared__ unsigned short feat[SG ? kBlock * kSgPad : 1];
+    __shared__ unsigned short feat[SG ? NT * kSgPad : 1];
     const int tid = threadIdx.x;
     const int item = blockIdx.x;
-    if (item >= qmeta[0] + qmeta[1] + qmeta[2]) return;
+    if (item >= qmeta[0] + qmeta[1] + qmeta[2] + qmeta[3]) return;
     const double dtINV = 1.0 / dt;
     const int s = __builtin_amdgcn_readfirstlane(queue_item(qmeta, qlist, ns, item));
     const TileInfo T = Ts[s];
@@ -602,7 +607,7 @@ __global__ void __launch_bounds__(kBlock) k_cells(MeshDev m, int ns, const Solid
     SgRegs sgr;
     bool sg_shape = false;
     if constexpr (SG) {
-        for (int t = tid; t < (int)(sizeof(SgNet) / 4); t += kBlock)
+        for (int t = tid; t < (int)(sizeof(SgNet) / 4); t += NT)
             reinterpret_cast<unsigned*>(&sn[0])[t] = reinterpret_cast<const unsigned*>(sgnet)[t];
         __syncthreads();
         sg_load(sn[0], sgr, tid & 63);
@@ -611,7 +616,7 @@ __global__ void __launch_bounds__(kBlock) k_cells(MeshDev m, int ns, const Solid
     int n_net = 0;
 
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int base = 0; base < ntot; base += kBlock) {  // the same trip count for every thread
+    for (int base = 0; base < ntot; base += NT) {  // the same trip count for every thread
         const int q = base + tid;
         const bool live = q < ntot;
         int i = 0, j = 0, k = 0;
@@ -1366,6 +1371,7 @@ struct sdfibm_ctx {
     int* d_chead = nullptr;  // two head arrays of ncells each
     int* d_cnext = nullptr;
     long long contact_cap = 0, chead_cap = 0;
+    bool small_tiles = false;  // mean tile of the finite solids under 512 cells (set_solids)
     ContactGrid cgrid{};
     int ncells_c = 0;
     // the coupled step captured as a HIP graph (run_steps)
@@ -1523,20 +1529,34 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
         if (hipEventRecord(c->join, c->aux) != hipSuccess) return -1;
         k_classify<R, MK, kRowsSmall, 0, kBlock><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
                                                                c->d_cand, c->code_cap, c->d_ncand);
+        k_classify<R, MK, kRowsTiny, 3, 64><<<g, 64, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
+                                                              c->d_cand, c->code_cap, c->d_ncand);
         if (hipStreamWaitEvent(st, c->join, 0) != hipSuccess) return -1;
         if (!graph) (void)hipEventRecord(c->ev[4], st);
         constexpr bool kSgOk = MK == MK_RECT && sizeof(R) == sizeof(float);
         if (kSgOk && c->d_sg != nullptr) {
+            // same workgroup size as the plain pass: identical reduction order when every cell falls back
             k_fill_i32<<<1, 64, 0, st>>>(c->d_sg_count, 1, 0);
-            k_cells<R, MK, kSgOk><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
-                                                        c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
-                                                        (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft,
-                                                        c->d_sg, c->sg_lo, c->sg_hi, c->sg_cos, c->d_sg_count);
-        } else {
-            k_cells<R, MK, false><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
+            if (c->small_tiles)
+                k_cells<R, MK, kSgOk, 64><<<g, 64, 0, st>>>(
+                    c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand, c->d_ncand, (const R*)c->U[0],
+                    (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft, c->d_sg,
+                    c->sg_lo, c->sg_hi, c->sg_cos, c->d_sg_count);
+            else
+                k_cells<R, MK, kSgOk, kBlock><<<g, kBlock, 0, st>>>(
+                    c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand, c->d_ncand, (const R*)c->U[0],
+                    (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft, c->d_sg,
+                    c->sg_lo, c->sg_hi, c->sg_cos, c->d_sg_count);
+        } else if (c->small_tiles) {  // a few dozen candidates per solid: one wave each
+            k_cells<R, MK, false, 64><<<g, 64, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
                                                         c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
                                                         (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft,
                                                         nullptr, 0.f, 0.f, 0.f, nullptr);
+        } else {
+            k_cells<R, MK, false, kBlock><<<g, kBlock, 0, st>>>(
+                c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand, c->d_ncand, (const R*)c->U[0],
+                (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft, nullptr, 0.f, 0.f,
+                0.f, nullptr);
         }
     } else {
         if (!graph) (void)hipEventRecord(c->ev[4], st);
@@ -2004,7 +2024,7 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if (n < 0 || (n > 0 && !solids)) return fail(c, SDFIBM_ERR_ARG, "bad solid array");
     if (n > (1LL << 30)) return fail(c, SDFIBM_ERR_ARG, "too many solids");
     std::vector<int> planes;
-    long long cells = 0;
+    long long cells = 0, tile_sum = 0, n_tiles = 0;
     for (int64_t s = 0; s < n; ++s) {
         std::string why;
         const int v = validate_solid(solids[s], s, why);
@@ -2018,6 +2038,8 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         if (t.cls == TC_TOOBIG)
             return fail(c, SDFIBM_ERR_TILE, "solid " + std::to_string(s) + ": " + err_text(EF_TILE));
         if (t.cls == TC_EMPTY) continue;
+        tile_sum += (long long)t.n[0] * t.n[1] * t.n[2];
+        ++n_tiles;
         // allow one extra cell layer per axis for motion between set_solids calls
         const long long ex = std::min(t.n[0] + 1, c->nx), ey = std::min(t.n[1] + 1, c->ny),
                         ez = std::min(t.n[2] + 1, c->nz);
@@ -2040,7 +2062,7 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         dfree(c, c->d_ncand);
         if ((rc = dalloc(c, (void**)&c->d_ncand, cap * sizeof(int2)))) return rc;
         dfree(c, c->qlist);
-        if ((rc = dalloc(c, (void**)&c->qlist, cap * 3 * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->qlist, cap * 4 * 4))) return rc;
         dfree(c, c->scan_tmp);
         if ((rc = dalloc(c, (void**)&c->scan_tmp, ((cap + kScanTile - 1) / kScanTile + 2) * 8))) return rc;
         c->solid_cap = cap;
@@ -2084,6 +2106,7 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         if ((rc = dalloc(c, &c->d_pcode, (long long)planes.size() * c->N * rsize(c)))) return rc;
     }
     c->h_planes = planes;
+    c->small_tiles = n_tiles > 0 && tile_sum < 512 * n_tiles;
     c->h_solids.assign(solids, solids + n);
     if (n > 0) HIPCHK(c, hipMemsetAsync(c->d_ft, 0, n * 48, c->stream));  // evolve before any interact: no fluid force
     if (c->dyn_n != n) c->dyn_n = -1;

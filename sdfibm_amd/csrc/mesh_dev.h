@@ -222,6 +222,7 @@ struct TileInfo {
 };
 
 constexpr int kMaxTileX = 63;      // cells along x per tile (vertex row fits one 64-bit word)
+constexpr int kRowsTiny = 64;      // vertex rows of a tiny tile (classified by one wave)
 constexpr int kRowsSmall = 400;    // vertex rows (ny+1)(nz+1) of a small tile
 constexpr int kRowsMedium = 1024;  // ... of a medium tile
 constexpr int kRowsLarge = 4096;   // ... of a large tile

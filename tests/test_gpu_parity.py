@@ -65,6 +65,16 @@ def test_mixed_shapes_3d(prec, structured):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_small_spheres_one_wave_per_solid(prec):
+    """2-4-cell spheres: mean tiles under 512 cells select the one-wave cut-cell pass."""
+    sc = scenes.cloud_3d(32, 150, frac_sphere=1.0, d_min=2.0, d_max=4.0, d_uniform=True)
+    om = oracle.OracleMesh(sc.mesh.to_polymesh())
+    ref = om.interact(sc.solids, sc.U, rho_f=sc.rho_f, dt=sc.dt, capture=True)
+    for c, ft in _run(sc.mesh, sc.solids, sc.U, False, sc.rho_f, sc.dt, prec):
+        _compare(c, ref, ft, sc.solids.size, prec, 1e-4 * np.abs(ref["ft"]).max())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_jittered_mesh(prec):
     sc = scenes.small_mixed_3d(20, seed=3, jitter=0.2)
     om = oracle.OracleMesh(sc.mesh.to_polymesh())
