@@ -344,6 +344,12 @@ def bench_sedimentation(args):
     cloud.run_steps((args.loop_steps + 2) * dt, dt, args.loop_steps, g)
     ms_graph = 1e3 * (time.perf_counter() - t) / args.loop_steps
     dev_graph = cloud.timing()[3] / args.loop_steps
+    # and with the contact stage on (discs and walls; the reference's linear law, k = 1e4, plus
+    # damping): its cost per step inside the graph
+    cloud.set_contact(1e4, 2.0)
+    cloud.run_steps((2 * args.loop_steps + 2) * dt, dt, 2, g)
+    cloud.run_steps((2 * args.loop_steps + 4) * dt, dt, args.loop_steps, g)
+    dev_graph_contact = cloud.timing()[3] / args.loop_steps
     cloud.close()
     line = {
         "metric": METRIC, "value": n_pairs / (ms_interact * 1e-3), "unit": "cell-updates/s", "n_gpus": 1,
@@ -355,7 +361,8 @@ def bench_sedimentation(args):
         "loop": {"steps": args.loop_steps, "ms_per_coupling_step": ms_step,
                  "mean_disc_drop": float(np.mean(sc.solids["center"][:100, 1] - fin["center"][:100, 1])),
                  "graph_ms_per_coupling_step": ms_graph, "graph_device_ms_per_step": dev_graph,
-                 "graph_speedup": ms_step / ms_graph},
+                 "graph_speedup": ms_step / ms_graph,
+                 "graph_device_ms_per_step_with_contacts": dev_graph_contact},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -51,6 +51,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxPerBrick = 128;  // solids per brick handled by k_fields
 constexpr int kMaxPlanes = 8;
+constexpr int kContactBlockSolids = 1024;   // evolve with contacts in one workgroup up to this many solids
+constexpr int kContactBlockCells = 1 << 16;  // ... and contact grid cells
 
 enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_BRICK = 8, EF_PLANE_MOVED = 16 };
 
@@ -1209,22 +1211,21 @@ __global__ void k_contact_insert(const sdfibm_solid* __restrict__ S, const doubl
 }
 
 // One sub-step of SolidCloud::evolve with contacts, one thread per solid:
-// loads, contact forces from the state at the start of the sub-step (Sin),
-// Solid::move into Sout.  The contacts of a solid are summed in ascending
-// partner index (finite solids), then the planes in list order -- the sum does
-// not depend on the order the cell lists were built in.  Also resets the
-// other sub-step's cell heads (nobody reads them during this launch).
-__global__ void k_contact_step(const sdfibm_solid* __restrict__ Sin, sdfibm_solid* __restrict__ Sout,
-                               SolidDyn* __restrict__ D, const double* __restrict__ ft, const double* __restrict__ rb,
-                               int ns, ContactGrid g, const int* __restrict__ head, const int* __restrict__ next,
-                               int* __restrict__ head_reset, int ncells, const int* __restrict__ planes, int nplanes,
-                               ContactLaw law, double dts, Vec3 gv, double rho_f, int last) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int t = tid; t < ncells; t += gridDim.x * blockDim.x) head_reset[t] = -1;
-    if (tid >= ns) return;
-    const int s = tid;
-    sdfibm_solid sol = Sin[s];
-    SolidDyn d = D[s];
+// loads, contact forces from the state at the start of the sub-step (Sin, cell
+// lists head/next), Solid::move into Sout, then (unless it is the last
+// sub-step) the moved solid goes into the next sub-step's lists (head_ins,
+// next_ins).  The contacts of a solid are summed in ascending partner index
+// (finite solids), then the planes in list order -- the sum does not depend on
+// the order the cell lists were built in.  Three head arrays and two next
+// arrays rotate over the sub-steps; this launch also resets the head array the
+// sub-step after the next one fills (the previous sub-step read it).
+// One sub-step of one solid with contacts (see k_contact_step); returns the
+// moved solid in sol and the updated dynamics in d.
+__device__ inline void contact_substep(const sdfibm_solid* __restrict__ Sin, int s, sdfibm_solid& sol, SolidDyn& d,
+                                       const double* __restrict__ ft, const double* __restrict__ rb, ContactGrid g,
+                                       const int* __restrict__ head, const int* __restrict__ next,
+                                       const int* __restrict__ planes, int nplanes, ContactLaw law, double dts,
+                                       Vec3 gv, double rho_f, int last) {
     const BodyConst b = body_const(sol, d);
     const double* f6 = ft + 6LL * s;
     const d3 ff{f6[0], f6[1], f6[2]}, fq{f6[3], f6[4], f6[5]};
@@ -1235,17 +1236,21 @@ __global__ void k_contact_step(const sdfibm_solid* __restrict__ Sin, sdfibm_soli
         d3 fc{0, 0, 0};
         int ijk[3];
         contact_cell(g, sol.center, ijk);
-        const int i0 = max(ijk[0] - 1, 0), i1 = min(ijk[0] + 1, g.n[0] - 1);
-        const int j0 = max(ijk[1] - 1, 0), j1 = min(ijk[1] + 1, g.n[1] - 1);
-        const int k0 = max(ijk[2] - 1, 0), k1 = min(ijk[2] + 1, g.n[2] - 1);
+        // the heads of the 27 neighbour cells, loaded together (independent loads)
+        int h27[27];
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const int ii = ijk[0] + t % 3 - 1, jj = ijk[1] + (t / 3) % 3 - 1, kk = ijk[2] + t / 9 - 1;
+            const bool in = ii >= 0 && ii < g.n[0] && jj >= 0 && jj < g.n[1] && kk >= 0 && kk < g.n[2];
+            h27[t] = in ? head[ii + g.n[0] * (jj + g.n[1] * kk)] : -1;
+        }
         int prev = -1;
         for (;;) {  // next partner in contact: the smallest index above prev
             int best = INT_MAX;
-            for (int kk = k0; kk <= k1; ++kk)
-                for (int jj = j0; jj <= j1; ++jj)
-                    for (int ii = i0; ii <= i1; ++ii)
-                        for (int j = head[ii + g.n[0] * (jj + g.n[1] * kk)]; j >= 0; j = next[j])
-                            if (j > prev && j < best && j != s && pair_overlap(sol, r, Sin[j], rb[j]) > 0) best = j;
+#pragma unroll
+            for (int t = 0; t < 27; ++t)
+                for (int j = h27[t]; j >= 0; j = next[j])
+                    if (j > prev && j < best && j != s && pair_overlap(sol, r, Sin[j], rb[j]) > 0) best = j;
             if (best == INT_MAX) break;
             fc = fc + pair_force(sol, r, Sin[best], rb[best], law);
             prev = best;
@@ -1255,8 +1260,73 @@ __global__ void k_contact_step(const sdfibm_solid* __restrict__ Sin, sdfibm_soli
     }
     move_solid(sol, d, b, force, torque, dts);
     if (last) store_old_force(d, ff, fq);
+}
+
+// One sub-step of SolidCloud::evolve with contacts, one thread per solid:
+// loads, contact forces from the state at the start of the sub-step (Sin, cell
+// lists head/next), Solid::move into Sout, then (unless it is the last
+// sub-step) the moved solid goes into the next sub-step's lists (head_ins,
+// next_ins).  The contacts of a solid are summed in ascending partner index
+// (finite solids), then the planes in list order -- the sum does not depend on
+// the order the cell lists were built in.  Three head arrays and two next
+// arrays rotate over the sub-steps; this launch also resets the head array the
+// sub-step after the next one fills (the previous sub-step read it).
+__global__ void k_contact_step(const sdfibm_solid* __restrict__ Sin, sdfibm_solid* __restrict__ Sout,
+                               SolidDyn* __restrict__ D, const double* __restrict__ ft, const double* __restrict__ rb,
+                               int ns, ContactGrid g, const int* __restrict__ head, const int* __restrict__ next,
+                               int* __restrict__ head_ins, int* __restrict__ next_ins, int* __restrict__ head_reset,
+                               int ncells, const int* __restrict__ planes, int nplanes, ContactLaw law, double dts,
+                               Vec3 gv, double rho_f, int last) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int t = tid; t < ncells; t += gridDim.x * blockDim.x) head_reset[t] = -1;
+    if (tid >= ns) return;
+    const int s = tid;
+    sdfibm_solid sol = Sin[s];
+    SolidDyn d = D[s];
+    contact_substep(Sin, s, sol, d, ft, rb, g, head, next, planes, nplanes, law, dts, gv, rho_f, last);
     Sout[s] = sol;
     D[s] = d;
+    if (!last && rb[s] > 0) {
+        int ijk[3];
+        const int cell = contact_cell(g, sol.center, ijk);
+        next_ins[s] = atomicExch(head_ins + cell, s);
+    }
+}
+
+// All sub-steps of an evolve with contacts in one workgroup (few solids, small
+// grid): per sub-step build the cell lists, move every solid, clear the lists,
+// with workgroup barriers between the phases instead of kernel boundaries.
+// The same per-solid arithmetic and partner order as k_contact_step.
+__global__ void __launch_bounds__(1024) k_contact_evolve_block(
+    sdfibm_solid* __restrict__ S0, sdfibm_solid* __restrict__ S1, SolidDyn* __restrict__ D,
+    const double* __restrict__ ft, const double* __restrict__ rb, int ns, ContactGrid g, int* __restrict__ head,
+    int* __restrict__ next, int ncells, const int* __restrict__ planes, int nplanes, ContactLaw law, double dts,
+    Vec3 gv, double rho_f, int n_sub) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int t = tid; t < ncells; t += nt) head[t] = -1;
+    __syncthreads();
+    for (int it = 0; it < n_sub; ++it) {
+        const sdfibm_solid* Sin = (it & 1) ? S1 : S0;
+        sdfibm_solid* Sout = (it & 1) ? S0 : S1;
+        for (int s = tid; s < ns; s += nt)
+            if (rb[s] > 0) {
+                int ijk[3];
+                const int cell = contact_cell(g, Sin[s].center, ijk);
+                next[s] = atomicExch(head + cell, s);
+            }
+        __syncthreads();
+        for (int s = tid; s < ns; s += nt) {
+            sdfibm_solid sol = Sin[s];
+            SolidDyn d = D[s];
+            contact_substep(Sin, s, sol, d, ft, rb, g, head, next, planes, nplanes, law, dts, gv, rho_f,
+                            it == n_sub - 1);
+            Sout[s] = sol;
+            D[s] = d;
+        }
+        __syncthreads();
+        for (int t = tid; t < ncells; t += nt) head[t] = -1;
+        __syncthreads();
+    }
 }
 
 // Graph replays: keep every step's error flags, and flag a plane solid whose
@@ -1368,7 +1438,8 @@ struct sdfibm_ctx {
     ContactLaw law{0.0, 0.0};
     double* d_rb = nullptr;
     sdfibm_solid* d_solids_alt = nullptr;
-    int* d_chead = nullptr;  // two head arrays of ncells each
+    int* d_chead = nullptr;  // three head arrays of ncells each
+    // d_cnext: two next arrays of contact_cap each
     int* d_cnext = nullptr;
     long long contact_cap = 0, chead_cap = 0;
     bool small_tiles = false;  // mean tile of the finite solids under 512 cells (set_solids)
@@ -1843,20 +1914,29 @@ void launch_evolve(sdfibm_ctx* c, double dt, Vec3 g) {
     const int ns = (int)c->ns;
     if (ns == 0) return;
     const int n_sub = ns == 1 ? 1 : 20;  // solidcloud.cpp:524-526
-    if (contact_on(c) && ns > 1) {
-        // sub-steps alternate between the two solid buffers (20: the result ends in d_solids)
-        k_fill_i32<<<grid_for(2LL * c->ncells_c, 256), 256, 0, c->stream>>>(c->d_chead, 2LL * c->ncells_c, -1);
+    if (contact_on(c) && ns > 1 && ns <= kContactBlockSolids && c->ncells_c <= kContactBlockCells) {
+        // few solids: every sub-step in one workgroup (the result ends in d_solids: 20 sub-steps)
+        k_contact_evolve_block<<<1, 1024, 0, c->stream>>>(c->d_solids, c->d_solids_alt, c->d_dyn, c->d_ft, c->d_rb,
+                                                         ns, c->cgrid, c->d_chead, c->d_cnext, c->ncells_c,
+                                                         c->d_planes, (int)c->h_planes.size(), c->law, dt / n_sub, g,
+                                                         c->rho_f, n_sub);
+    } else if (contact_on(c) && ns > 1) {
+        // sub-steps alternate between the two solid buffers (20: the result ends in d_solids);
+        // cell lists: heads H[it % 3], nexts N[it % 2]
+        const long long nc = c->ncells_c;
+        k_fill_i32<<<grid_for(3 * nc, 256), 256, 0, c->stream>>>(c->d_chead, 3 * nc, -1);
         sdfibm_solid* buf[2] = {c->d_solids, c->d_solids_alt};
         const double dts = dt / n_sub;
         const int nplanes = (int)c->h_planes.size();
+        k_contact_insert<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_rb, ns, c->cgrid, c->d_chead,
+                                                                 c->d_cnext);
         for (int it = 0; it < n_sub; ++it) {
-            int* head = c->d_chead + (long long)(it & 1) * c->ncells_c;
-            int* other = c->d_chead + (long long)((it + 1) & 1) * c->ncells_c;
-            k_contact_insert<<<grid_for(ns, 64), 64, 0, c->stream>>>(buf[it & 1], c->d_rb, ns, c->cgrid, head,
-                                                                     c->d_cnext);
+            int* H[3] = {c->d_chead + (it % 3) * nc, c->d_chead + ((it + 1) % 3) * nc, c->d_chead + ((it + 2) % 3) * nc};
+            int* Nr = c->d_cnext + (long long)(it & 1) * c->contact_cap;
+            int* Nw = c->d_cnext + (long long)((it + 1) & 1) * c->contact_cap;
             k_contact_step<<<grid_for(ns, 64), 64, 0, c->stream>>>(
-                buf[it & 1], buf[(it + 1) & 1], c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, head, c->d_cnext, other,
-                c->ncells_c, c->d_planes, nplanes, c->law, dts, g, c->rho_f, it == n_sub - 1);
+                buf[it & 1], buf[(it + 1) & 1], c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, H[0], Nr, H[1], Nw, H[2],
+                (int)nc, c->d_planes, nplanes, c->law, dts, g, c->rho_f, it == n_sub - 1);
         }
     } else {
         k_evolve<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_dyn, c->d_ft, ns, n_sub, dt, g, c->rho_f);
@@ -2279,14 +2359,14 @@ int contact_prepare(sdfibm_ctx* c) {
         c->contact_cap = 0;
         if ((rc = dalloc(c, (void**)&c->d_rb, n * 8))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_solids_alt, n * sizeof(sdfibm_solid)))) return rc;
-        if ((rc = dalloc(c, (void**)&c->d_cnext, n * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_cnext, 2 * n * 4))) return rc;
         c->contact_cap = n;
     }
     if (total > c->chead_cap) {
         dfree(c, c->d_chead);
         c->d_chead = nullptr;
         c->chead_cap = 0;
-        if ((rc = dalloc(c, (void**)&c->d_chead, 2 * total * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_chead, 3 * total * 4))) return rc;
         c->chead_cap = total;
     }
     if (n > 0) HIPCHK(c, hipMemcpy(c->d_rb, rb.data(), n * 8, hipMemcpyHostToDevice));
