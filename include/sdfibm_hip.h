@@ -35,7 +35,9 @@ typedef enum {
     SDFIBM_ERR_HIP = -2,       /* a HIP runtime call failed (message has the HIP error) */
     SDFIBM_ERR_NOMEM = -3,     /* device allocation failed */
     SDFIBM_ERR_SHAPE = -4,     /* unknown shape id or invalid shape parameters */
-    SDFIBM_ERR_TILE = -5,      /* a solid's cell tile exceeds the kernel's limits */
+    SDFIBM_ERR_TILE = -5,      /* a solid's cell tile outgrew the tile pass after set_solids (a solid
+                                  whose tile is too big at set_solids, > 63 cells along x or > 4096
+                                  vertex rows, goes the planes' path instead; <= 8 such plus planes) */
     SDFIBM_ERR_STATE = -6,     /* call out of order (e.g. interact before set_solids) */
     SDFIBM_ERR_MESH = -7,      /* unsupported or inconsistent mesh description */
     SDFIBM_ERR_CAPACITY = -8,  /* an internal buffer was too small for this step */

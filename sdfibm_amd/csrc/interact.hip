@@ -135,12 +135,15 @@ __host__ __device__ inline int tile_queue(int cls) {
 template <int MK>
 __global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, SolidK* __restrict__ K,
                         TileInfo* __restrict__ T, long long* __restrict__ tile_cells, double* __restrict__ ft,
-                        int* __restrict__ qmeta, int* __restrict__ qlist, int* __restrict__ err) {
+                        int* __restrict__ qmeta, int* __restrict__ qlist, const unsigned char* __restrict__ unb,
+                        int* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     const SolidK k = make_solid_k(in[s]);
     K[s] = k;
     TileInfo t = make_tile<MK>(m, k);
+    // solids set_solids put on the unbounded path stay there (their tile box is the cull)
+    if (unb[s] && t.cls != TC_PLANE) t.cls = t.cls == TC_EMPTY ? TC_EMPTY : TC_HUGE;
     // structured meshes: the seed search (a box of cell centres) is spread over a wave by k_seed
     t.seed = t.cls == TC_EMPTY ? -1 : (MK == MK_RECT ? nearest_cell<MK>(m, k.c) : -2);
     if (t.cls == TC_TOOBIG) atomicOr(err, EF_TILE);
@@ -737,6 +740,17 @@ __device__ inline int plane_box_state(const SolidK& K, const double* bb) {
     return 2;
 }
 
+// Cull state of a brick (cell ranges [i0, i1] x [j0, j1] x [k0, k1], vertex box
+// bb) for a solid of the unbounded path: a plane by its half-space, a finite
+// (oversized) solid by its tile box -- 0: no vertex inside, 2: may have some.
+__device__ inline int unbounded_box_state(const SolidK& K, const TileInfo& T, const double* bb, int i0, int i1, int j0,
+                                          int j1, int k0, int k1) {
+    if (K.shape == SDFIBM_SHAPE_PLANE) return plane_box_state(K, bb);
+    const bool hit = i0 < T.lo[0] + T.n[0] && i1 >= T.lo[0] && j0 < T.lo[1] + T.n[1] && j1 >= T.lo[1] &&
+                     k0 < T.lo[2] + T.n[2] && k1 >= T.lo[2];
+    return hit ? 2 : 0;
+}
+
 template <int MK, int BX, int BY, int BZ>
 __device__ inline void brick_vertex_box(const MeshDev& m, const double* brick_box, long long b, int bx, int by, int bz,
                                         double* bb) {
@@ -760,14 +774,15 @@ __device__ inline bool bit_get(const unsigned* bits, long long c) { return (bits
 template <int MK, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(BX* BY* BZ)
     k_plane_active(MeshDev m, int nbx, int nby, const double* __restrict__ brick_box, const SolidK* __restrict__ Ks,
-                   int sp, unsigned* __restrict__ act, int* __restrict__ list, int* __restrict__ nlist, long long cap,
-                   int* __restrict__ minidx) {
+                   const TileInfo* __restrict__ Ts, int sp, unsigned* __restrict__ act, int* __restrict__ list,
+                   int* __restrict__ nlist, long long cap, int* __restrict__ minidx) {
     const SolidK K = Ks[sp];
     const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     const long long b = bx + (long long)nbx * (by + (long long)nby * bz);
     double bb[6];
     brick_vertex_box<MK, BX, BY, BZ>(m, brick_box, b, bx, by, bz, bb);
-    const int st = plane_box_state(K, bb);
+    const int st = unbounded_box_state(K, Ts[sp], bb, bx * BX, bx * BX + BX - 1, by * BY, by * BY + BY - 1, bz * BZ,
+                                       bz * BZ + BZ - 1);
     if (st == 0) return;
     const int tid = threadIdx.x;
     const int i = bx * BX + tid % BX, j = by * BY + (tid / BX) % BY, k = bz * BZ + tid / (BX * BY);
@@ -1051,7 +1066,8 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
             const SolidK K = Ks[sp];
             // plane: only cells of the seed's component (member bitmap); cull the brick first
             // (uniform over the workgroup: the brick's vertex box)
-            const int st = plane_box_state(K, bb);
+            const int st = unbounded_box_state(K, Ts[sp], bb, bx * BX, bx * BX + BX - 1, by * BY, by * BY + BY - 1,
+                                               bz * BZ, bz * BZ + BZ - 1);
             double* part = plane_partial + ((long long)slot * nbricks + b) * 6;
             if (st == 0) {
                 if (tid == 0)
@@ -1453,6 +1469,7 @@ struct sdfibm_ctx {
     double* d_plane_pose = nullptr;  // center + q of each plane at capture
     std::vector<int> h_planes;
     int* d_planes = nullptr;
+    unsigned char* d_unb = nullptr;  // per solid: 1 = unbounded path (planes and oversized solids)
     double* d_plane_partial = nullptr;
     // plane components: active bitmap, active list, union-find labels, member bitmaps
     unsigned* d_pact = nullptr;
@@ -1569,7 +1586,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     k_fill_i32<<<1, 64, 0, st>>>(c->qmeta, 8, 0);
     if (ns > 0) {
         k_setup<MK><<<grid_for(ns, 64), 64, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_cells,
-                                                       c->d_ft, c->qmeta, c->qlist, c->err);
+                                                       c->d_ft, c->qmeta, c->qlist, c->d_unb, c->err);
         if (MK != MK_RECT) k_seed<<<grid_for(ns, 4), 256, 0, st>>>(c->m, ns, c->d_K, c->d_T);
         scan<long long>(c->d_tile_cells, ns, c->d_tile_off, c->scan_tmp, st);
         k_tile_offsets<<<grid_for(ns, 128), 128, 0, st>>>(c->d_T, c->d_tile_off, ns, c->code_cap, c->err);
@@ -1642,12 +1659,12 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
         k_fill_i32<<<1, 64, 0, st>>>(c->d_pscalars + 1, 1, 0x7fffffff);
         if (c->nz == 1)
             k_plane_active<MK, kBX2, kBY2, kBZ2><<<dim3(c->nbx, c->nby, c->nbz), kBX2 * kBY2 * kBZ2, 0, st>>>(
-                c->m, c->nbx, c->nby, c->brick_box, c->d_K, sp, c->d_pact, c->d_plist, c->d_pscalars, c->N,
-                c->d_pscalars + 1);
+                c->m, c->nbx, c->nby, c->brick_box, c->d_K, c->d_T, sp, c->d_pact, c->d_plist, c->d_pscalars,
+                c->N, c->d_pscalars + 1);
         else
             k_plane_active<MK, kBX3, kBY3, kBZ3><<<dim3(c->nbx, c->nby, c->nbz), kBX3 * kBY3 * kBZ3, 0, st>>>(
-                c->m, c->nbx, c->nby, c->brick_box, c->d_K, sp, c->d_pact, c->d_plist, c->d_pscalars, c->N,
-                c->d_pscalars + 1);
+                c->m, c->nbx, c->nby, c->brick_box, c->d_K, c->d_T, sp, c->d_pact, c->d_plist, c->d_pscalars,
+                c->N, c->d_pscalars + 1);
         int nact = 0;
         if (hipMemcpyAsync(&nact, c->d_pscalars, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
@@ -2115,8 +2132,10 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
             continue;
         }
         TileInfo t = c->kind == SDFIBM_MESH_RECTILINEAR ? make_tile<MK_RECT>(c->hm, K) : make_tile<MK_STRUCT>(c->hm, K);
-        if (t.cls == TC_TOOBIG)
-            return fail(c, SDFIBM_ERR_TILE, "solid " + std::to_string(s) + ": " + err_text(EF_TILE));
+        if (t.cls == TC_TOOBIG) {  // oversized: the unbounded path, like a plane
+            planes.push_back((int)s);
+            continue;
+        }
         if (t.cls == TC_EMPTY) continue;
         tile_sum += (long long)t.n[0] * t.n[1] * t.n[2];
         ++n_tiles;
@@ -2126,7 +2145,9 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         cells += ex * ey * ez;
     }
     if ((int)planes.size() > kMaxPlanes)
-        return fail(c, SDFIBM_ERR_ARG, "at most " + std::to_string(kMaxPlanes) + " plane solids are supported");
+        return fail(c, SDFIBM_ERR_ARG,
+                    "at most " + std::to_string(kMaxPlanes) +
+                        " plane solids and solids whose cell tile exceeds 63 cells along x or 4096 vertex rows");
     int rc;
     if (n > c->solid_cap) {
         for (void* p : {(void*)c->d_solids, (void*)c->d_K, (void*)c->d_T, (void*)c->d_tile_cells,
@@ -2143,6 +2164,8 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         if ((rc = dalloc(c, (void**)&c->d_ncand, cap * sizeof(int2)))) return rc;
         dfree(c, c->qlist);
         if ((rc = dalloc(c, (void**)&c->qlist, cap * 4 * 4))) return rc;
+        dfree(c, c->d_unb);
+        if ((rc = dalloc(c, (void**)&c->d_unb, cap))) return rc;
         dfree(c, c->scan_tmp);
         if ((rc = dalloc(c, (void**)&c->scan_tmp, ((cap + kScanTile - 1) / kScanTile + 2) * 8))) return rc;
         c->solid_cap = cap;
@@ -2187,6 +2210,11 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     }
     c->h_planes = planes;
     c->small_tiles = n_tiles > 0 && tile_sum < 512 * n_tiles;
+    if (n > 0) {
+        std::vector<unsigned char> unb(n, 0);
+        for (int p : planes) unb[p] = 1;
+        HIPCHK(c, hipMemcpy(c->d_unb, unb.data(), n, hipMemcpyHostToDevice));
+    }
     c->h_solids.assign(solids, solids + n);
     if (n > 0) HIPCHK(c, hipMemsetAsync(c->d_ft, 0, n * 48, c->stream));  // evolve before any interact: no fluid force
     if (c->dyn_n != n) c->dyn_n = -1;
@@ -2428,7 +2456,8 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
         for (int a = 0; a < 3; ++a) still = still && ps.vel[a] == 0.0 && ps.omega[a] == 0.0;
         if (!still)
             return fail(c, SDFIBM_ERR_STATE,
-                        "run_steps needs fixed plane solids (motion mask b000000, zero velocity and spin): plane " +
+                        "run_steps needs fixed plane solids and fixed oversized solids (motion mask b000000, "
+                        "zero velocity and spin): solid " +
                             std::to_string(c->h_planes[p]));
         for (int a = 0; a < 3; ++a) pose[7 * p + a] = ps.center[a];
         for (int a = 0; a < 4; ++a) pose[7 * p + 3 + a] = ps.q[a];
@@ -2630,7 +2659,7 @@ int sdfibm_candidates(sdfibm_ctx* c, int64_t solid, int64_t* counts, int64_t* ce
     std::vector<unsigned char> raw;
     long long nt = 0;
     int lo[3] = {0, 0, 0}, n[3] = {0, 0, 0};
-    if (T.cls == TC_PLANE) {
+    if (T.cls == TC_PLANE || T.cls == TC_HUGE) {
         nt = c->N;
         raw.resize(nt * rs);
         void* tmp = nullptr;

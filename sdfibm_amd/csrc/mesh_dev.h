@@ -209,7 +209,10 @@ __device__ inline R cell_fraction(const MeshDev& m, const SolidK& K, int i, int 
 // ---------------------------------------------------------------------------
 // per-solid tiles
 // ---------------------------------------------------------------------------
-enum TileClass { TC_EMPTY = 0, TC_SMALL = 1, TC_LARGE = 2, TC_PLANE = 3, TC_TOOBIG = 4, TC_MEDIUM = 5 };
+// TC_HUGE: a finite solid whose tile is too big for the tile pass (> kMaxTileX
+// cells along x or > kRowsLarge vertex rows); it goes the planes' way (cells
+// over the mesh, union-find component of the seed) with its tile box as the cull.
+enum TileClass { TC_EMPTY = 0, TC_SMALL = 1, TC_LARGE = 2, TC_PLANE = 3, TC_TOOBIG = 4, TC_MEDIUM = 5, TC_HUGE = 6 };
 
 __host__ __device__ inline bool tile_is_finite(int cls) { return cls == TC_SMALL || cls == TC_MEDIUM || cls == TC_LARGE; }
 

@@ -75,6 +75,40 @@ def test_small_spheres_one_wave_per_solid(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("structured", [False, True])
+def test_oversized_solids(prec, structured):
+    """Solids whose tile exceeds 63 cells along x or 4096 vertex rows take the unbounded path
+    (mesh-wide cells, union-find component of the seed) among ordinary solids and a plane."""
+    mesh = M.uniform_box((96, 40, 36), (0, 0, 0), (1.2, 0.5, 0.45))
+    sol = S.make_solids(5)
+    sh, p = S.make_shape("Box", radiusa=0.45, radiusb=0.08, radiusc=0.07)  # 72 cells along x
+    sol["shape"][0], sol["prm"][0] = sh, p
+    sol["center"][0] = (0.6, 0.25, 0.22)
+    sol["q"][0] = S.quaternion_xyz((0.1, 0.2, 0.3))
+    sh, p = S.make_shape("Sphere", radius=0.2)  # an ordinary large tile
+    sol["shape"][1], sol["prm"][1] = sh, p
+    sol["center"][1] = (0.3, 0.27, 0.21)
+    sh, p = S.make_shape("Ellipsoid", radiusa=0.55, radiusb=0.1, radiusc=0.12)  # 90 cells along x
+    sol["shape"][2], sol["prm"][2] = sh, p
+    sol["center"][2] = (0.62, 0.2, 0.25)
+    sol["q"][2] = S.quaternion_xyz((0.0, 0.0, 0.15))
+    sh, p = S.make_shape("Sphere", radius=0.05)
+    sol["shape"][3], sol["prm"][3] = sh, p
+    sol["center"][3] = (0.9, 0.35, 0.3)
+    sol["shape"][4] = S.PLANE
+    sol["center"][4] = (0.0, 0.06, 0.0)
+    sol["q"][4] = S.quaternion_xyz((0.0, 0.0, 0.05))
+    rng = np.random.default_rng(3)
+    sol["vel"] = rng.normal(0, 0.1, (5, 3))
+    sol["omega"] = rng.normal(0, 1.0, (5, 3))
+    U = rng.normal(0, 1.0, (mesh.n_cells, 3))
+    om = oracle.OracleMesh(mesh.to_polymesh())
+    ref = om.interact(sol, U, rho_f=1.2, dt=1e-3, capture=True)
+    for c, ft in _run(mesh, sol, U, False, 1.2, 1e-3, prec, structured):
+        _compare(c, ref, ft, sol.size, prec, 1e-4 * np.abs(ref["ft"]).max())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_jittered_mesh(prec):
     sc = scenes.small_mixed_3d(20, seed=3, jitter=0.2)
     om = oracle.OracleMesh(sc.mesh.to_polymesh())
@@ -171,9 +205,11 @@ def test_error_paths():
         huge["center"] = (0.5, 0.5, 0.5)
     big = M.uniform_box((80, 8, 8), (0, 0, 0), (1, 0.1, 0.1))
     with SolidCloud(big) as c:
+        c.set_solids(huge)  # a tile wider than 63 cells goes the unbounded (plane) path
+        nine = np.concatenate([huge] * 9)  # more than 8 plane or oversized solids
         with pytest.raises(_lib.SdfibmError) as e:
-            c.set_solids(huge)
-        assert e.value.status == -5  # tile wider than 63 cells
+            c.set_solids(nine)
+        assert e.value.status == -1
 
 
 def test_repeat_is_bitwise_deterministic():
