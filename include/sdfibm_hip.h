@@ -37,7 +37,7 @@ typedef enum {
     SDFIBM_ERR_SHAPE = -4,     /* unknown shape id or invalid shape parameters */
     SDFIBM_ERR_TILE = -5,      /* a solid's cell tile outgrew the tile pass after set_solids (a solid
                                   whose tile is too big at set_solids, > 63 cells along x or > 4096
-                                  vertex rows, goes the planes' path instead; <= 8 such plus planes) */
+                                  vertex rows, goes the planes' path instead; <= 32 such plus planes) */
     SDFIBM_ERR_STATE = -6,     /* call out of order (e.g. interact before set_solids) */
     SDFIBM_ERR_MESH = -7,      /* unsupported or inconsistent mesh description */
     SDFIBM_ERR_CAPACITY = -8,  /* an internal buffer was too small for this step */

@@ -50,7 +50,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxPerBrick = 128;  // solids per brick handled by k_fields
-constexpr int kMaxPlanes = 8;
+constexpr int kMaxPlanes = 32;  // plane and oversized solids (unbounded path)
 constexpr int kContactBlockSolids = 1024;   // evolve with contacts in one workgroup up to this many solids
 constexpr int kContactBlockCells = 1 << 16;  // ... and contact grid cells
 

@@ -206,9 +206,9 @@ def test_error_paths():
     big = M.uniform_box((80, 8, 8), (0, 0, 0), (1, 0.1, 0.1))
     with SolidCloud(big) as c:
         c.set_solids(huge)  # a tile wider than 63 cells goes the unbounded (plane) path
-        nine = np.concatenate([huge] * 9)  # more than 8 plane or oversized solids
+        many = np.concatenate([huge] * 33)  # more than 32 plane or oversized solids
         with pytest.raises(_lib.SdfibmError) as e:
-            c.set_solids(nine)
+            c.set_solids(many)
         assert e.value.status == -1
 
 
