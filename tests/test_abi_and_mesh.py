@@ -69,6 +69,28 @@ def test_jittered_mesh_geometry_matches_oracle_and_fills_the_box():
     assert np.allclose(V, g["V"], rtol=1e-13, atol=0)
 
 
+def test_jittered_mesh_geometry_invariants():
+    """The face/cell geometry of non-planar faces (primitiveMesh's triangle fans): every cell's
+    outward face area vectors sum to zero (closed surface) and the cell volumes fill the box.
+    (The volume-weighted centres do not reproduce the box's first moment exactly: primitiveMesh
+    treats each face as planar with its area vector and centre in the pyramid centroids; on this
+    mesh the first moment is off by 3e-4 relative, which is the method, not an error.)"""
+    n = (7, 6, 5)
+    m = M.jittered_box(n, (0, 0, 0), (1, 2, 3), 0.3, seed=5)
+    g = m.geometry()
+    nx, ny, nz = n
+    fx = g["fx_s"].reshape(nz, ny, nx + 1, 3)
+    fy = g["fy_s"].reshape(nz, ny + 1, nx, 3)
+    fz = g["fz_s"].reshape(nz + 1, ny, nx, 3)
+    closure = (fx[:, :, 1:] - fx[:, :, :-1]) + (fy[:, 1:] - fy[:, :-1]) + (fz[1:] - fz[:-1])
+    assert np.abs(closure).max() < 1e-15
+    V, cc = g["V"], g["cc"]
+    assert abs(V.sum() - 6.0) < 1e-12
+    np.testing.assert_allclose((V[:, None] * cc).sum(axis=0), 6.0 * np.array([0.5, 1.0, 1.5]), rtol=1e-3)
+    # the jitter is real: faces are not planar
+    assert np.abs(g["fx_s"][:, 1:]).max() > 1e-3
+
+
 def test_foam_scalar_field_roundtrip(tmp_path):
     v = np.random.default_rng(0).random(1000)
     for binary in (True, False):
