@@ -2117,9 +2117,28 @@ int sdfibm_set_stream(sdfibm_ctx* c, void* s) {
 
 int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if (!c) return SDFIBM_ERR_ARG;
-    graph_reset(c);
     if (n < 0 || (n > 0 && !solids)) return fail(c, SDFIBM_ERR_ARG, "bad solid array");
     if (n > (1LL << 30)) return fail(c, SDFIBM_ERR_ARG, "too many solids");
+    // Per-step updates of the same solids (poses and velocities only; the reference's solver
+    // hands over its moved solids every step): no tile estimates, buffers, classes or captured
+    // step to rebuild -- the interact grows its code buffer itself if moving tiles need it.
+    bool same = c->have_solids && n == c->ns && n > 0;
+    for (int64_t s = 0; same && s < n; ++s)
+        same = solids[s].shape == c->h_solids[s].shape &&
+               std::memcmp(solids[s].prm, c->h_solids[s].prm, sizeof(solids[s].prm)) == 0;
+    if (same) {
+        for (int64_t s = 0; s < n; ++s) {
+            std::string why;
+            const int v = validate_solid(solids[s], s, why);
+            if (v) return fail(c, v, why);
+        }
+        HIPCHK(c, hipMemsetAsync(c->d_ft, 0, n * 48, c->stream));  // evolve before any interact: no fluid force
+        HIPCHK(c, hipMemcpyAsync(c->d_solids, solids, n * sizeof(sdfibm_solid), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->have_step = false;
+        return SDFIBM_OK;
+    }
+    graph_reset(c);
     std::vector<int> planes;
     long long cells = 0, tile_sum = 0, n_tiles = 0;
     for (int64_t s = 0; s < n; ++s) {
