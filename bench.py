@@ -110,6 +110,26 @@ def cpu_baseline(scene, n_sample, budget_s):
     return pairs / elapsed, done, pairs, elapsed
 
 
+def copy_bandwidth(nbytes=2 << 30, reps=10):
+    """Bytes moved per second (read + write) by a plain device-to-device copy on this GPU."""
+    import torch
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    del a, b
+    return 2.0 * nbytes / sec
+
+
 def cpu_baseline_fast(scene, per_solid_pairs, threads, budget_s):
     """The oracle with fast bookkeeping (stamped visited array, shared vertex cache), the solids
     split over `threads` host threads (the C++ runs outside the GIL), each with its own field
@@ -527,6 +547,9 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes": kernels[dom]["bytes"]},
     }
+    copy_bw = copy_bandwidth()
+    line["roofline"]["copy_peak_measured"] = copy_bw / 1e9
+    line["roofline"]["frac_of_copy_peak"] = achieved / copy_bw
     if args.precision == "fp32" and not args.no_surrogate:
         line["surrogate"] = surrogate_comparison(cloud, sc, ft, f["As"], ms_per_step, phases[5], args.steps)
     if not args.no_cpu:
