@@ -1345,6 +1345,27 @@ __global__ void __launch_bounds__(1024) k_contact_evolve_block(
     }
 }
 
+// Plane (and oversized) solids whose pose differs from the one their cell sets
+// were last labelled for: bit p of *dirty; the cached pose is updated.  A pose
+// that is bitwise the same keeps the member bitmaps and codes (NaN in the cache
+// forces a labelling).
+__global__ void k_plane_pose(const sdfibm_solid* __restrict__ S, const int* __restrict__ planes, int np,
+                             double* __restrict__ cache, int* __restrict__ dirty) {
+    const int p = threadIdx.x;
+    if (p == 0) dirty[0] = 0;
+    __syncthreads();
+    if (p >= np) return;
+    const sdfibm_solid& s = S[planes[p]];
+    bool same = true;
+    for (int a = 0; a < 3; ++a) same = same && s.center[a] == cache[8 * p + a];
+    for (int a = 0; a < 4; ++a) same = same && s.q[a] == cache[8 * p + 3 + a];
+    if (!same) {
+        for (int a = 0; a < 3; ++a) cache[8 * p + a] = s.center[a];
+        for (int a = 0; a < 4; ++a) cache[8 * p + 3 + a] = s.q[a];
+        atomicOr(dirty, 1 << p);
+    }
+}
+
 // Graph replays: keep every step's error flags, and flag a plane solid whose
 // pose moved away from the one its membership bitmaps were computed for (the
 // quaternion normalisation of Solid::move may change a fixed plane's pose by
@@ -1475,9 +1496,10 @@ struct sdfibm_ctx {
     unsigned* d_pact = nullptr;
     int* d_plist = nullptr;
     int* d_plabel = nullptr;
-    int* d_pscalars = nullptr;  // [0] list size, [1] lowest active index, [2] changed flag
+    int* d_pscalars = nullptr;  // [0] list size, [1] lowest active index, [2] changed flag, [3] moved planes
     unsigned* d_pmember = nullptr;
     void* d_pcode = nullptr;  // per plane: codes of its member cells (N each)
+    double* d_plane_cache = nullptr;  // per plane: the pose its cell sets were labelled for (8 doubles)
     int* d_pseed = nullptr;
     long long member_words = 0;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -1650,7 +1672,16 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
         if (!graph) (void)hipEventRecord(c->ev[4], st);
     }
     const int np = (int)c->h_planes.size();
+    unsigned dirty = 0;
+    if (!graph && np > 0) {
+        // only the planes that moved since their last labelling are labelled again
+        k_plane_pose<<<1, 64, 0, st>>>(c->d_solids, c->d_planes, np, c->d_plane_cache, c->d_pscalars + 3);
+        if (hipMemcpyAsync(&dirty, c->d_pscalars + 3, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return -1;
+    }
     for (int p = 0; p < (graph ? 0 : np); ++p) {
+        if (!((dirty >> p) & 1u)) continue;
         const int sp = c->h_planes[p];
         unsigned* member = c->d_pmember + (long long)p * c->member_words;
         k_fill_i32<<<grid_for(c->member_words, 256), 256, 0, st>>>((int*)c->d_pact, c->member_words, 0);
@@ -2226,6 +2257,9 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         if ((rc = dalloc(c, (void**)&c->d_pmember, planes.size() * c->member_words * 4))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_pseed, planes.size() * 4))) return rc;
         if ((rc = dalloc(c, &c->d_pcode, (long long)planes.size() * c->N * rsize(c)))) return rc;
+        if (!c->d_plane_cache && (rc = dalloc(c, (void**)&c->d_plane_cache, 8 * kMaxPlanes * 8))) return rc;
+        const std::vector<double> nan(8 * kMaxPlanes, std::nan(""));  // every plane labelled at the next interact
+        HIPCHK(c, hipMemcpy(c->d_plane_cache, nan.data(), nan.size() * 8, hipMemcpyHostToDevice));
     }
     c->h_planes = planes;
     c->small_tiles = n_tiles > 0 && tile_sum < 512 * n_tiles;

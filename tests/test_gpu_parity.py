@@ -275,17 +275,23 @@ def test_pose_update_matches_a_fresh_context():
     """set_solids with the same shapes and new poses (the per-step update of a host-side solver)
     gives what a fresh context gives for those poses, bit for bit."""
     sc = scenes.small_mixed_3d(24, seed=9)
-    moved = sc.solids.copy()
+    wall = S.make_solids(1)
+    wall["shape"] = S.PLANE
+    wall["center"] = (0.3, 0.2, 0.1)
+    wall["q"] = S.quaternion_xyz((0.1, 0.0, 0.2))
+    start = np.concatenate([sc.solids, wall])
+    moved = start.copy()
     rng = np.random.default_rng(1)
-    moved["center"] += rng.normal(0, 0.03, moved["center"].shape)
+    moved["center"] += rng.normal(0, 0.03, moved["center"].shape)  # the plane moves too
     q = moved["q"] + rng.normal(0, 0.1, moved["q"].shape)
     moved["q"] = q / np.linalg.norm(q, axis=1, keepdims=True)
     moved["vel"] += 0.1
     with SolidCloud(sc.mesh, rho_f=sc.rho_f, precision="fp32") as a, \
             SolidCloud(sc.mesh, rho_f=sc.rho_f, precision="fp32") as b:
-        a.set_solids(sc.solids)
+        a.set_solids(start)
         a.set_velocity(sc.U)
         a.interact(0.0, sc.dt)
+        a.interact(0.0, sc.dt)  # unmoved plane: its cell set is reused
         a.set_solids(moved)
         fa = a.interact(0.0, sc.dt)
         b.set_solids(moved)
