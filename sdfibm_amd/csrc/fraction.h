@@ -102,7 +102,8 @@ __device__ inline R hf(const Hex8<R>& h) {
 // The apex rule (geometrictools.cpp:25-45): A = first listed vertex, B = first
 // later vertex with phi_A*phi_B <= 0, else the last one.
 template <class R, int V0, int V1, int V2, int V3>
-__device__ inline R face_fraction(const Hex8<R>& h, R sf_mag) {
+__device__ inline R face_fraction(const Hex8<R>& h, R sf_mag, R l01, R l12, R l23, R l30) {
+    // l01 .. l30: line fractions of the listed edges (V0 V1) (V1 V2) (V2 V3) (V3 V0)
     const R p0 = hf<V0>(h), p1 = hf<V1>(h), p2 = hf<V2>(h), p3 = hf<V3>(h);
     const int ss = (p0 > 0 ? 1 : -1) + (p1 > 0 ? 1 : -1) + (p2 > 0 ? 1 : -1) + (p3 > 0 ? 1 : -1);
     if (ss == 4) return R(0);
@@ -120,10 +121,8 @@ __device__ inline R face_fraction(const Hex8<R>& h, R sf_mag) {
         pB = p1;
     }
     const r3<R> ap = apex_point(q0, p0, B, pB);
-    auto tri = [&](r3<R> O, r3<R> A, R pO, R pA) {
-        return fabs(R(0.5) * rmag(rcross(sub(A, O), sub(ap, O)))) * line_fraction<R>(pO, pA);
-    };
-    const R area = tri(q0, q1, p0, p1) + tri(q1, q2, p1, p2) + tri(q2, q3, p2, p3) + tri(q3, q0, p3, p0);
+    auto tri = [&](r3<R> O, r3<R> A, R lf) { return fabs(R(0.5) * rmag(rcross(sub(A, O), sub(ap, O)))) * lf; };
+    const R area = tri(q0, q1, l01) + tri(q1, q2, l12) + tri(q2, q3, l23) + tri(q3, q0, l30);
     return area / sf_mag;
 }
 
@@ -166,13 +165,24 @@ __device__ inline R hex_fraction(const Hex8<R>& h, const HexFaces<R>& F, R V, bo
     const R pB = blend(h.f1, h.f2, h.f3, h.f4, h.f5, h.f6, h.f7);
     r3<R> ap = apex_point(h.p0, f0, B, pB);
     if (two_d) ap.z = apex_z_rel;
+    // the 12 edges' line fractions, each formed once (calcLineFraction is symmetric in its
+    // two arguments, bit for bit)
+    const R x01 = line_fraction<R>(h.f0, h.f1), x23 = line_fraction<R>(h.f2, h.f3),
+            x45 = line_fraction<R>(h.f4, h.f5), x67 = line_fraction<R>(h.f6, h.f7);
+    const R y02 = line_fraction<R>(h.f0, h.f2), y13 = line_fraction<R>(h.f1, h.f3),
+            y46 = line_fraction<R>(h.f4, h.f6), y57 = line_fraction<R>(h.f5, h.f7);
+    const R z04 = line_fraction<R>(h.f0, h.f4), z15 = line_fraction<R>(h.f1, h.f5),
+            z26 = line_fraction<R>(h.f2, h.f6), z37 = line_fraction<R>(h.f3, h.f7);
     R eps[6];
-    eps[0] = F.bnd[0] ? face_fraction<R, 0, 4, 6, 2>(h, rmag(F.sf[0])) : face_fraction<R, 0, 2, 6, 4>(h, rmag(F.sf[0]));
-    eps[1] = face_fraction<R, 1, 3, 7, 5>(h, rmag(F.sf[1]));
-    eps[2] = F.bnd[1] ? face_fraction<R, 0, 1, 5, 4>(h, rmag(F.sf[2])) : face_fraction<R, 0, 4, 5, 1>(h, rmag(F.sf[2]));
-    eps[3] = face_fraction<R, 2, 6, 7, 3>(h, rmag(F.sf[3]));
-    eps[4] = F.bnd[2] ? face_fraction<R, 0, 2, 3, 1>(h, rmag(F.sf[4])) : face_fraction<R, 0, 1, 3, 2>(h, rmag(F.sf[4]));
-    eps[5] = face_fraction<R, 4, 5, 7, 6>(h, rmag(F.sf[5]));
+    eps[0] = F.bnd[0] ? face_fraction<R, 0, 4, 6, 2>(h, rmag(F.sf[0]), z04, y46, z26, y02)
+                      : face_fraction<R, 0, 2, 6, 4>(h, rmag(F.sf[0]), y02, z26, y46, z04);
+    eps[1] = face_fraction<R, 1, 3, 7, 5>(h, rmag(F.sf[1]), y13, z37, y57, z15);
+    eps[2] = F.bnd[1] ? face_fraction<R, 0, 1, 5, 4>(h, rmag(F.sf[2]), x01, z15, x45, z04)
+                      : face_fraction<R, 0, 4, 5, 1>(h, rmag(F.sf[2]), z04, x45, z15, x01);
+    eps[3] = face_fraction<R, 2, 6, 7, 3>(h, rmag(F.sf[3]), z26, x67, z37, x23);
+    eps[4] = F.bnd[2] ? face_fraction<R, 0, 2, 3, 1>(h, rmag(F.sf[4]), y02, x23, y13, x01)
+                      : face_fraction<R, 0, 1, 3, 2>(h, rmag(F.sf[4]), x01, y13, x23, y02);
+    eps[5] = face_fraction<R, 4, 5, 7, 6>(h, rmag(F.sf[5]), x45, y57, x67, y46);
     R vol = R(0);
 #pragma unroll
     for (int f = 0; f < 6; ++f)
