@@ -1002,67 +1002,71 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
         }
     };
 
-    int ia = 0, ip = 0;
     if (!HASP) nplanes = 0;
-    // codes of the next kBatch listed solids, loaded together (independent loads)
+    // The listed solids and the planes are taken in ascending solid index.  The listed solids
+    // between two planes form a segment; a segment is walked in batches of kBatch solids whose
+    // codes are loaded together (independent loads), then consumed in an unrolled pass.
     constexpr int kBatch = 8 / CPT;
-    R cb[kBatch][CPT];
-    int cb0 = -kBatch;
-    while (ia < cnt || ip < nplanes) {
-        const int sa = __builtin_amdgcn_readfirstlane(ia < cnt ? lst[ia] : 0x7fffffff);
-        const int sp = (HASP && ip < nplanes) ? planes[ip] : 0x7fffffff;
-        if (sa < sp) {
-            if (ia >= cb0 + kBatch) {
-                cb0 = ia;
+    int ia = 0;
+    for (int ip = 0;; ++ip) {
+        const bool plane_next = HASP && ip < nplanes;
+        const int sp = plane_next ? planes[ip] : 0x7fffffff;
+        int iend = cnt;
+        if (plane_next) {
+            iend = ia;
+            while (iend < cnt && __builtin_amdgcn_readfirstlane(lst[iend]) < sp) ++iend;
+        }
+        for (int cb0 = ia; cb0 < iend; cb0 += kBatch) {
+            R cb[kBatch][CPT];
+            int sid[kBatch];
 #pragma unroll
-                for (int u = 0; u < kBatch; ++u) {
+            for (int u = 0; u < kBatch; ++u) {
 #pragma unroll
-                    for (int q = 0; q < CPT; ++q) cb[u][q] = code_none<R>();
-                    const int a = ia + u;
-                    if (a < cnt) {
-                        const int sq = __builtin_amdgcn_readfirstlane(lst[a]);
-                        const TileInfo T = Ts[sq];  // uniform: scalar loads
-                        const bool hit = T.lo[0] <= wi1 && T.lo[0] + T.n[0] > wi0 && T.lo[1] <= wj1 &&
-                                         T.lo[1] + T.n[1] > wj0 &&
-                                         T.off + (long long)T.n[0] * T.n[1] * T.n[2] <= code_cap;
-                        const unsigned li = (unsigned)(i - T.lo[0]), lj = (unsigned)(j - T.lo[1]);
-                        const bool in_xy = li < (unsigned)T.n[0] && lj < (unsigned)T.n[1];
+                for (int q = 0; q < CPT; ++q) cb[u][q] = code_none<R>();
+                sid[u] = 0;
+                const int a = cb0 + u;
+                if (a < iend) {
+                    const int sq = __builtin_amdgcn_readfirstlane(lst[a]);
+                    sid[u] = sq;
+                    const TileInfo T = Ts[sq];  // uniform: scalar loads
+                    const bool hit = T.lo[0] <= wi1 && T.lo[0] + T.n[0] > wi0 && T.lo[1] <= wj1 &&
+                                     T.lo[1] + T.n[1] > wj0 &&
+                                     T.off + (long long)T.n[0] * T.n[1] * T.n[2] <= code_cap;
+                    const unsigned li = (unsigned)(i - T.lo[0]), lj = (unsigned)(j - T.lo[1]);
+                    const bool in_xy = li < (unsigned)T.n[0] && lj < (unsigned)T.n[1];
 #pragma unroll
-                        for (int q = 0; q < CPT; ++q) {
-                            const int lk = wk + q * LZ - T.lo[2];
-                            if (hit && lk >= 0 && lk < T.n[2] && valid[q] && in_xy)
-                                cb[u][q] = codes[T.off + (long long)(li + T.n[0] * (lj + T.n[1] * lk))];
-                        }
+                    for (int q = 0; q < CPT; ++q) {
+                        const int lk = wk + q * LZ - T.lo[2];
+                        if (hit && lk >= 0 && lk < T.n[2] && valid[q] && in_xy)
+                            cb[u][q] = codes[T.off + (long long)(li + T.n[0] * (lj + T.n[1] * lk))];
                     }
                 }
             }
-            R code[CPT];
 #pragma unroll
-            for (int q = 0; q < CPT; ++q) {
-                code[q] = cb[0][q];
+            for (int u = 0; u < kBatch; ++u) {
+                if (cb0 + u >= iend) break;
+                bool any = false;
 #pragma unroll
-                for (int u = 1; u < kBatch; ++u)
-                    if (ia - cb0 == u) code[q] = cb[u][q];
+                for (int q = 0; q < CPT; ++q) any = any || cb[u][q] == cb[u][q];
+                if (!__any(any)) continue;  // no candidate in the wave (NaN codes)
+                const int sa = sid[u];
+                const SolidK& Ka = Ks[sa];
+                const double cvw[9] = {Ka.c[0], Ka.c[1], Ka.c[2], Ka.v[0], Ka.v[1], Ka.v[2], Ka.w[0], Ka.w[1], Ka.w[2]};
+#pragma unroll
+                for (int q = 0; q < CPT; ++q) {
+                    const R cq = cb[u][q];
+                    if (cq != cq) continue;
+                    if (isinf(cq))
+                        contribute(q, cvw, 1.0, sa + 4, nullptr);
+                    else
+                        contribute(q, cvw, (double)fabs(cq), signbit(cq) ? 3 : 2, nullptr);
+                }
             }
-            ++ia;
-            bool any = false;
-#pragma unroll
-            for (int q = 0; q < CPT; ++q) any = any || code[q] == code[q];
-            if (!__any(any)) continue;  // no candidate in the wave (NaN codes)
-            const SolidK& Ka = Ks[sa];
-            const double cvw[9] = {Ka.c[0], Ka.c[1], Ka.c[2], Ka.v[0], Ka.v[1], Ka.v[2], Ka.w[0], Ka.w[1], Ka.w[2]};
-#pragma unroll
-            for (int q = 0; q < CPT; ++q) {
-                const R cq = code[q];
-                if (cq != cq) continue;
-                if (isinf(cq))
-                    contribute(q, cvw, 1.0, sa + 4, nullptr);
-                else
-                    contribute(q, cvw, (double)fabs(cq), signbit(cq) ? 3 : 2, nullptr);
-            }
-        } else if constexpr (HASP) {
+        }
+        ia = iend;
+        if (!plane_next) break;
+        if constexpr (HASP) {
             const int slot = ip;
-            ++ip;
             const SolidK K = Ks[sp];
             // plane: only cells of the seed's component (member bitmap); cull the brick first
             // (uniform over the workgroup: the brick's vertex box)
