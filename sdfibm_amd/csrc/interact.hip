@@ -50,6 +50,10 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxPerBrick = 128;  // solids per brick handled by k_fields
+#ifndef SDFIBM_FILL_REP
+#define SDFIBM_FILL_REP 1
+#endif
+constexpr int kFillRep = SDFIBM_FILL_REP;  // row updates per flood-fill pass
 constexpr int kMaxPlanes = 32;  // plane and oversized solids (unbounded path)
 constexpr int kContactBlockSolids = 1024;   // evolve with contacts in one workgroup up to this many solids
 constexpr int kContactBlockCells = 1 << 16;  // ... and contact grid cells
@@ -115,6 +119,16 @@ __device__ inline unsigned long long run_fill(unsigned long long a, unsigned lon
     const unsigned long long ra = __builtin_bitreverse64(a), rs = __builtin_bitreverse64(s);
     const unsigned long long dn = __builtin_bitreverse64((((ra + rs) ^ ra) & ra) | rs);
     return up | dn;
+}
+
+// LDS words read and written while other threads of the workgroup may write
+// them (the flood fill's relaxation): relaxed atomics keep every access a
+// single 64-bit LDS operation that the compiler neither drops nor reorders.
+__device__ inline unsigned long long lds_load_relaxed(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void lds_store_relaxed(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---------------------------------------------------------------------------
@@ -402,7 +416,12 @@ __global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK
     constexpr int NSYZ = MK == MK_RECT ? (MAXR / 2 + 1 < kStageYZ ? MAXR / 2 + 1 : kStageYZ) : 1;
     __shared__ double sx[NSX], sy[NSYZ], sz[NSYZ];
     __shared__ int scan_lds[32];
-    __shared__ int sh_first, sh_changed;
+    __shared__ int sh_first, sh_changed, sh_flag[3];
+    // connectivity summary of the active rows: nonzero rows, j-links, rows of
+    // more than one run; per slice k: nonzero, linked to slice k - 1
+    constexpr int kSW = (MAXR / 2 + 63) / 64;
+    __shared__ int sh_cnt[3];
+    __shared__ unsigned long long sh_knz[kSW], sh_kl[kSW];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nq = qmeta[Q];
     for (int item = blockIdx.x; item < nq; item += gridDim.x) {
@@ -426,6 +445,8 @@ __global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK
         for (int t = tid; t <= nz; t += NT) sz[t] = m.zs[T.lo[2] + t];
     }
     if (tid == 0) sh_first = 0x7fffffff;
+    if (tid < 3) sh_cnt[tid] = 0;
+    for (int t = tid; t < kSW; t += NT) sh_knz[t] = sh_kl[t] = 0ull;
     __syncthreads();
 
     // 1. inside test at every tile vertex (FP64, reference operation order).  A
@@ -478,6 +499,41 @@ __global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK
     }
     __syncthreads();
 
+    // 2b. is the active set connected?  It is when every nonzero row is one run
+    //     of cells, the nonzero rows of each slice k form one chain of rows that
+    //     overlap their j-neighbour (rows - links = 1 per slice), the nonempty
+    //     slices are consecutive, and each overlaps the one before it in some
+    //     row.  Then the flood fill from any active seed reaches every active
+    //     cell; otherwise the fill below runs.  (Convex solids pass this test.)
+    {
+        for (int r0 = wv * 64; r0 < CR; r0 += NT) {
+            const int r = r0 + lane;
+            bool nzr = false, jl = false, multi = false, kl = false;
+            int k = 0;
+            if (r < CR) {
+                int j;
+                divmod_small(r, ny, rny, k, j);
+                const unsigned long long a = act[r];
+                nzr = a != 0ull;
+                if (nzr) {
+                    const unsigned long long t = a >> __builtin_ctzll(a);
+                    multi = (t & (t + 1ull)) != 0ull;
+                    jl = j > 0 && (a & act[r - 1]) != 0ull;
+                    kl = k > 0 && (a & act[r - ny]) != 0ull;
+                }
+            }
+            const unsigned long long bn = __ballot(nzr), bj = __ballot(jl), bm = __ballot(multi);
+            if (lane == 0 && bn) {
+                atomicAdd(&sh_cnt[0], __popcll(bn));
+                atomicAdd(&sh_cnt[1], __popcll(bj));
+                atomicAdd(&sh_cnt[2], __popcll(bm));
+            }
+            if (nzr) atomicOr(&sh_knz[k >> 6], 1ull << (k & 63));
+            if (kl) atomicOr(&sh_kl[k >> 6], 1ull << (k & 63));
+        }
+    }
+    __syncthreads();
+
     // 3. seed: nearest cell if it has an inside vertex, else the first cell (index
     //    order) that has one.  Tile raster order is ascending cell index.
     if (tid == 0) {
@@ -497,34 +553,67 @@ __global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK
             seed_r = sh_first / 64;
             seed_b = sh_first % 64;
         }
+        bool conn = sh_cnt[2] == 0;
+        if (conn) {
+            int nslice = 0, kmin = 0x7fffffff, kmax = -1;
+            unsigned long long carry = 0ull;
+            for (int w = 0; w <= (nz - 1) >> 6; ++w) {
+                const unsigned long long z = sh_knz[w];
+                nslice += __popcll(z);
+                if (z) {
+                    kmin = min(kmin, w * 64 + __builtin_ctzll(z));
+                    kmax = w * 64 + 63 - __builtin_clzll(z);
+                }
+                if (z & ((z << 1) | carry) & ~sh_kl[w]) conn = false;  // slices k - 1, k both nonempty, not linked
+                carry = z >> 63;
+            }
+            conn = conn && nslice > 0 && kmax - kmin + 1 == nslice && sh_cnt[0] - sh_cnt[1] == nslice;
+        }
         if (seed_r >= 0) reach[seed_r] = run_fill(act[seed_r], 1ull << seed_b);
-        sh_changed = seed_r >= 0 ? 1 : 0;
+        // 0: no seed, 1: fill from the seed, 2: every active cell is reached
+        sh_changed = seed_r < 0 ? 0 : (conn ? 2 : 1);
+        sh_flag[0] = 0;
     }
     __syncthreads();
 
-    // 4. flood fill over face neighbours within cells that have an inside vertex
-    while (sh_changed) {
+    // 4. flood fill over face neighbours within cells that have an inside vertex,
+    //    by chaotic relaxation: between two barriers every thread applies the row
+    //    update kFillRep times, reading its neighbours' rows as the other threads
+    //    leave them (a row only gains bits, so any mix of old and new values is a
+    //    subset of the component).  A pass in which no thread changes a row has
+    //    read only the rows as they stood at its start, so the rows are then the
+    //    seed's component.  Three flags rotate: a pass's flag is cleared two
+    //    passes ahead of its use.
+    if (sh_changed == 2) {
+        for (int r = tid; r < CR; r += NT) reach[r] = act[r];
         __syncthreads();
-        if (tid == 0) sh_changed = 0;
-        __syncthreads();
-        int ch = 0;
-        for (int r = tid; r < CR; r += NT) {
-            int j, k;
-            divmod_small(r, ny, rny, k, j);
-            unsigned long long nb = reach[r];
-            if (j > 0) nb |= reach[r - 1];
-            if (j < ny - 1) nb |= reach[r + 1];
-            if (k > 0) nb |= reach[r - ny];
-            if (k < nz - 1) nb |= reach[r + ny];
-            const unsigned long long a = act[r];
-            const unsigned long long sd = nb & a;
-            if (sd & ~reach[r]) {
-                reach[r] = run_fill(a, sd);
-                ch = 1;
+    } else if (sh_changed == 1) {
+        for (int it = 0;; ++it) {
+            if (tid == 0) sh_flag[(it + 1) % 3] = 0;
+            int ch = 0;
+#pragma unroll 1
+            for (int rep = 0; rep < kFillRep; ++rep) {
+                for (int r = tid; r < CR; r += NT) {
+                    int j, k;
+                    divmod_small(r, ny, rny, k, j);
+                    unsigned long long nb = lds_load_relaxed(&reach[r]);
+                    const unsigned long long own = nb;
+                    if (j > 0) nb |= lds_load_relaxed(&reach[r - 1]);
+                    if (j < ny - 1) nb |= lds_load_relaxed(&reach[r + 1]);
+                    if (k > 0) nb |= lds_load_relaxed(&reach[r - ny]);
+                    if (k < nz - 1) nb |= lds_load_relaxed(&reach[r + ny]);
+                    const unsigned long long a = act[r];
+                    const unsigned long long sd = nb & a;
+                    if (sd & ~own) {
+                        lds_store_relaxed(&reach[r], run_fill(a, sd));
+                        ch = 1;
+                    }
+                }
             }
+            if (ch) sh_flag[it % 3] = 1;
+            __syncthreads();
+            if (!sh_flag[it % 3]) break;
         }
-        if (ch) sh_changed = 1;
-        __syncthreads();
     }
 
     // 5. candidate list: each thread takes a contiguous run of rows; ALL_INSIDE

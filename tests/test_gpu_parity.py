@@ -153,6 +153,40 @@ def test_sedimentation_with_walls(prec):
         _compare(c, ref, ft, sol.size, prec, 1e-4 * np.abs(ref["ft"]).max())
 
 
+@pytest.mark.parametrize("nz", [1, 3])
+def test_disconnected_pieces_are_dropped(nz):
+    """A disc with a thin tail at 60 poses: where the tail's cells do not connect to the disc's
+    through face neighbours, only the seed's component is a candidate (the GPU's flood fill runs
+    instead of its connected-set shortcut); elsewhere the shortcut takes the whole active set."""
+    n = 24
+    h = 1.0 / n
+    m = M.uniform_box((n, n, nz), (0, 0, -0.5 * nz * h), (1, 1, 0.5 * nz * h))
+    om = oracle.OracleMesh(m.to_polymesh())
+    U = np.random.default_rng(3).normal(size=(m.n_cells, 3))
+    P = m.points
+    n_split = 0
+    with SolidCloud(m, precision="fp64") as c:
+        for k in range(60):
+            s = S.make_solids(1)
+            s["shape"] = S.CIRCLE_TAIL
+            s["prm"][0, :3] = (2.2 * h, 4.0, 0.2 * h)
+            s["center"] = (0.31 + 0.013 * k / n, 0.5 + 0.37 * h, 0.0)
+            s["q"] = S.quaternion_xyz((0, 0, 0.3 + 0.05 * k))
+            ref = om.interact(s, U, capture=True)
+            c.set_solids(s)
+            c.set_velocity(U)
+            ft = c.interact(0.0, 1.0)
+            _compare(c, ref, ft, 1, "fp64", 1e-4 * np.abs(ref["ft"]).max())
+            ins = np.array([oracle.is_inside(s[0], p) for p in P]).reshape(nz + 1, n + 1, n + 1)
+            act = np.zeros((nz, n, n), dtype=bool)
+            for dz in (0, 1):
+                for dy in (0, 1):
+                    for dx in (0, 1):
+                        act |= ins[dz:dz + nz, dy:dy + n, dx:dx + n]
+            n_split += int(act.sum() > ref["counts"][0, :3].sum())
+    assert n_split > 0, "no pose left an active piece outside the seed's component"
+
+
 def test_seed_fallback_and_empty_solid():
     n = 16
     m = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
