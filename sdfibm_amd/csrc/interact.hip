@@ -327,6 +327,61 @@ __global__ void k_tile_offsets(TileInfo* __restrict__ T, const long long* __rest
     T[s].off = off[s];
 }
 
+// Up to kScanOne solids: the tile sizes' exclusive scan and the tile offsets in
+// one workgroup of 1024 threads (thread t scans values [16t, 16t + 16) in
+// registers, the waves by shuffles, the 16 wave totals through LDS).
+// off[ns] = total; the other entries of off are not written.
+constexpr int kScanOne = 16384;
+
+__global__ void __launch_bounds__(1024) k_tile_offsets_one(const long long* __restrict__ cells, int ns,
+                                                           TileInfo* __restrict__ T, long long* __restrict__ off,
+                                                           long long cap, int* __restrict__ err) {
+    constexpr int PER = kScanOne / 1024;
+    __shared__ long long wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int b = t * PER;
+    long long v[PER];
+    long long sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        v[q] = b + q < ns ? cells[b + q] : 0;
+        sum += v[q];
+    }
+    long long x = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const long long y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    long long base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const long long ws = wsum[w];
+        if (w < wv) base += ws;
+        tot += ws;
+    }
+    long long run = base + x - sum;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        if (b + q < ns) T[b + q].off = run;
+        run += v[q];
+    }
+    if (t == 0) {
+        off[ns] = tot;
+        if (tot > cap) atomicOr(err, EF_CODECAP);
+    }
+}
+
+// Per-step resets: error flags, queue counters, brick counts.
+__global__ void k_reset(int* __restrict__ err, int* __restrict__ qmeta, int* __restrict__ bcount, long long nbricks) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *err = 0;
+    if (i < 8) qmeta[i] = 0;
+    if (i < nbricks) bcount[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // per-solid tile pass, in two kernels
 //
@@ -1697,16 +1752,18 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     const int ns = (int)c->ns;
     hipStream_t st = c->stream;
     if (!graph) (void)hipEventRecord(c->ev[0], st);
-    k_fill_i32<<<1, 64, 0, st>>>(c->err, 1, 0);
-    k_fill_i32<<<1, 64, 0, st>>>(c->qmeta, 8, 0);
+    k_reset<<<grid_for(std::max(c->nbricks, 8LL), 256), 256, 0, st>>>(c->err, c->qmeta, c->bcount, c->nbricks);
     if (ns > 0) {
         k_setup<MK><<<grid_for(ns, 64), 64, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_cells,
                                                        c->d_ft, c->qmeta, c->qlist, c->d_unb, c->err);
         if (MK != MK_RECT) k_seed<<<grid_for(ns, 4), 256, 0, st>>>(c->m, ns, c->d_K, c->d_T);
-        scan<long long>(c->d_tile_cells, ns, c->d_tile_off, c->scan_tmp, st);
-        k_tile_offsets<<<grid_for(ns, 128), 128, 0, st>>>(c->d_T, c->d_tile_off, ns, c->code_cap, c->err);
+        if (ns <= kScanOne) {
+            k_tile_offsets_one<<<1, 1024, 0, st>>>(c->d_tile_cells, ns, c->d_T, c->d_tile_off, c->code_cap, c->err);
+        } else {
+            scan<long long>(c->d_tile_cells, ns, c->d_tile_off, c->scan_tmp, st);
+            k_tile_offsets<<<grid_for(ns, 128), 128, 0, st>>>(c->d_T, c->d_tile_off, ns, c->code_cap, c->err);
+        }
     }
-    k_fill_i32<<<grid_for(c->nbricks, 256), 256, 0, st>>>(c->bcount, c->nbricks, 0);
     if (ns > 0) {
         const unsigned gb = grid_for(64LL * ns, 256);
         if (c->nz == 1)

@@ -187,6 +187,39 @@ def test_disconnected_pieces_are_dropped(nz):
     assert n_split > 0, "no pose left an active piece outside the seed's component"
 
 
+def test_many_solids_multi_pass_scan():
+    """20,000 overlapping small spheres: more solids than the one-workgroup tile-offset scan
+    takes, so the tile offsets come from the three-pass scan.  Fields, forces and a sample of
+    the candidate lists against the oracle (FP64 tolerances above).  (128^3 cells keep every
+    brick of the field pass under its 128-solid capacity.)"""
+    n = 128
+    h = 1.0 / n
+    m = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
+    rng = np.random.default_rng(21)
+    ns = 20000
+    sol = S.make_solids(ns)
+    sol["shape"] = S.SPHERE
+    sol["prm"][:, 0] = rng.uniform(0.8, 2.0, ns) * h
+    sol["center"] = rng.uniform(0.05, 0.95, (ns, 3))
+    sol["vel"] = rng.normal(0, 0.1, (ns, 3))
+    sol["omega"] = rng.normal(0, 1.0, (ns, 3))
+    U = rng.normal(size=(m.n_cells, 3))
+    om = oracle.OracleMesh(m.to_polymesh())
+    ref = om.interact(sol, U, rho_f=1.0, dt=1e-3, capture=True, capacity=200 * ns)
+    tol = TOL["fp64"]
+    for c, ft in _run(m, sol, U, False, 1.0, 1e-3, "fp64"):
+        got = c.fields()
+        assert np.array_equal(got["Ct"], ref["Ct"])
+        assert np.max(np.abs(got["As"] - ref["As"])) <= tol["a"]
+        assert np.max(np.abs(got["Fs"] - ref["Fs"])) <= tol["f"] * np.abs(ref["Fs"]).max()
+        ft_scale = np.maximum(np.abs(ref["ft"]), tol["floor"] * np.abs(ref["ft"]).max())
+        assert np.max(np.abs(ft - ref["ft"]) / ft_scale) <= tol["ft"]
+        parts = split_candidates(ref, ns)
+        for s in range(0, ns, 97):
+            counts, cells, _ = c.candidates(s)
+            assert np.array_equal(counts, parts[s][0]) and np.array_equal(cells, parts[s][1]), s
+
+
 def test_seed_fallback_and_empty_solid():
     n = 16
     m = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
