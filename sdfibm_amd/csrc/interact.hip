@@ -1200,10 +1200,10 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
                 for (int q = 0; q < CPT; ++q) {
                     const R cq = cb[u][q];
                     if (cq != cq) continue;
-                    if (isinf(cq))
-                        contribute(q, cvw, 1.0, sa + 4, nullptr);
-                    else
-                        contribute(q, cvw, (double)fabs(cq), signbit(cq) ? 3 : 2, nullptr);
+                    // one instruction stream for ALL_INSIDE (alpha 1) and cut cells
+                    const bool all_in = isinf(cq);
+                    contribute(q, cvw, all_in ? 1.0 : (double)fabs(cq), all_in ? sa + 4 : (signbit(cq) ? 3 : 2),
+                               nullptr);
                 }
             }
         }
@@ -1232,10 +1232,8 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
                 if (!bit_get(plane_member + (long long)slot * member_words, c)) continue;
                 const R pc = pcodes[(long long)slot * m.nx * m.ny * m.nz + c];  // k_plane_alpha
                 if (pc != pc) continue;
-                if (isinf(pc))
-                    contribute(q, cvw, 1.0, sp + 4, pa);
-                else
-                    contribute(q, cvw, (double)fabs(pc), signbit(pc) ? 3 : 2, pa);
+                const bool all_in = isinf(pc);
+                contribute(q, cvw, all_in ? 1.0 : (double)fabs(pc), all_in ? sp + 4 : (signbit(pc) ? 3 : 2), pa);
             }
             block_reduce<6>(pa, red);  // the brick's share of the plane's force and torque
             if (tid == 0)
