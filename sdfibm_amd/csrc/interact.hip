@@ -328,9 +328,10 @@ __global__ void k_tile_offsets(TileInfo* __restrict__ T, const long long* __rest
 }
 
 // Up to kScanOne solids: the tile sizes' exclusive scan and the tile offsets in
-// one workgroup of 1024 threads (thread t scans values [16t, 16t + 16) in
-// registers, the waves by shuffles, the 16 wave totals through LDS).
-// off[ns] = total; the other entries of off are not written.
+// one workgroup of 1024 threads.  Wave w takes values [1024 w, 1024 w + 1024):
+// first their total (the 16 wave totals go through LDS), then 16 rounds of 64
+// consecutive values (coalesced loads) scanned by shuffles with a carry between
+// rounds.  off[ns] = total; the other entries of off are not written.
 constexpr int kScanOne = 16384;
 
 __global__ void __launch_bounds__(1024) k_tile_offsets_one(const long long* __restrict__ cells, int ns,
@@ -339,34 +340,33 @@ __global__ void __launch_bounds__(1024) k_tile_offsets_one(const long long* __re
     constexpr int PER = kScanOne / 1024;
     __shared__ long long wsum[16];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int b = t * PER;
-    long long v[PER];
+    const int base = wv * 64 * PER + lane;
     long long sum = 0;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        v[q] = b + q < ns ? cells[b + q] : 0;
-        sum += v[q];
-    }
-    long long x = sum;
+    for (int q = 0; q < PER; ++q) sum += base + 64 * q < ns ? cells[base + 64 * q] : 0;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const long long y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) wsum[wv] = x;
+    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if (lane == 0) wsum[wv] = sum;
     __syncthreads();
-    long long base = 0, tot = 0;
+    long long carry = 0, tot = 0;
 #pragma unroll
     for (int w = 0; w < 16; ++w) {
         const long long ws = wsum[w];
-        if (w < wv) base += ws;
+        if (w < wv) carry += ws;
         tot += ws;
     }
-    long long run = base + x - sum;
-#pragma unroll
+#pragma unroll 1
     for (int q = 0; q < PER; ++q) {
-        if (b + q < ns) T[b + q].off = run;
-        run += v[q];
+        const int e = base + 64 * q;
+        const long long x0 = e < ns ? cells[e] : 0;
+        long long x = x0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const long long y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (e < ns) T[e].off = carry + x - x0;
+        carry += __shfl(x, 63, 64);
     }
     if (t == 0) {
         off[ns] = tot;

@@ -366,7 +366,8 @@ __host__ __device__ inline TileInfo make_tile(const MeshDev& m, const SolidK& K)
     }
     const int nn[3] = {m.nx, m.ny, m.nz};
     int lo[3], hi[3];
-    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {  // unrolled: the per-axis arrays stay in registers
         const double a = wc[d] - wh[d], b = wc[d] + wh[d];
         int vlo, vhi;  // vertex index range that may hold inside vertices
         if (!(wh[d] < 1e300)) {
@@ -418,6 +419,7 @@ __host__ __device__ inline int nearest_cell(const MeshDev& m, const double* p) {
     const int nn[3] = {m.nx, m.ny, m.nz};
     if constexpr (MK == MK_RECT) {
         // per axis the nearest centre is one of two neighbours; search that 2x2x2 block
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double* xs = d == 0 ? m.xs : (d == 1 ? m.ys : m.zs);
             // first cell whose centre is >= p (centres increase), by guess and step
@@ -429,6 +431,7 @@ __host__ __device__ inline int nearest_cell(const MeshDev& m, const double* p) {
         }
         r = 1;
     } else {
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
             double f = floor((p[d] - m.o[d]) / m.h[d]);
             if (f < 0) f = 0;
