@@ -147,7 +147,7 @@ __host__ __device__ inline int tile_queue(int cls) {
 }
 
 template <int MK>
-__global__ void k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, SolidK* __restrict__ K,
+__global__ void __launch_bounds__(64) k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, SolidK* __restrict__ K,
                         TileInfo* __restrict__ T, long long* __restrict__ tile_cells, double* __restrict__ ft,
                         int* __restrict__ qmeta, int* __restrict__ qlist, const unsigned char* __restrict__ unb,
                         int* __restrict__ err) {
@@ -1015,7 +1015,7 @@ __global__ void k_plane_member(const int* __restrict__ list, const int* __restri
 // when the centre is outside), computed once per plane labelling so that the
 // field pass only reads them.  Cells that are not members are left as they are.
 template <class R, int MK>
-__global__ void k_plane_alpha(MeshDev m, const int* __restrict__ list, const int* __restrict__ nlist,
+__global__ void __launch_bounds__(256) k_plane_alpha(MeshDev m, const int* __restrict__ list, const int* __restrict__ nlist,
                               const SolidK* __restrict__ Ks, int solid, const unsigned* __restrict__ member,
                               R* __restrict__ pcode) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1271,7 +1271,7 @@ __global__ void k_plane_reduce(const double* __restrict__ partial, long long nbr
 
 // Candidate codes of a plane over the whole mesh (parity / diagnostics only).
 template <class R, int MK>
-__global__ void k_plane_codes(MeshDev m, const SolidK* __restrict__ Ks, int sp, const unsigned* __restrict__ member,
+__global__ void __launch_bounds__(256) k_plane_codes(MeshDev m, const SolidK* __restrict__ Ks, int sp, const unsigned* __restrict__ member,
                               R* __restrict__ out) {
     const SolidK K = Ks[sp];
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1281,7 +1281,8 @@ __global__ void k_plane_codes(MeshDev m, const SolidK* __restrict__ Ks, int sp, 
         out[c] = code_none<R>();
         return;
     }
-    const int i = (int)(c % m.nx), j = (int)((c / m.nx) % m.ny), k = (int)(c / ((long long)m.nx * m.ny));
+    int i, j, k;
+    cell_ijk(m, c, i, j, k);
     d3 P[8];
     cell_vertices<MK>(m, i, j, k, P);
     int nin = 0;
@@ -1311,7 +1312,7 @@ __global__ void k_update_velocity(long long n, R* __restrict__ Ux, R* __restrict
 // fixInternal (src/solidcloud.cpp:288-301): where Ct >= 4 the cell lies wholly in
 // solid Ct-4 and U takes that solid's rigid velocity at the cell centre.
 template <class R, int MK>
-__global__ void k_fix_internal(MeshDev m, const int* __restrict__ Ct, const sdfibm_solid* __restrict__ S, int ns,
+__global__ void __launch_bounds__(256) k_fix_internal(MeshDev m, const int* __restrict__ Ct, const sdfibm_solid* __restrict__ S, int ns,
                                R* __restrict__ Ux, R* __restrict__ Uy, R* __restrict__ Uz) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long N = (long long)m.nx * m.ny * m.nz;
@@ -1319,7 +1320,8 @@ __global__ void k_fix_internal(MeshDev m, const int* __restrict__ Ct, const sdfi
     const int t = Ct[c];
     if (t < 4 || t - 4 >= ns) return;
     const sdfibm_solid& s = S[t - 4];
-    const int i = (int)(c % m.nx), j = (int)((c / m.nx) % m.ny), k = (int)(c / ((long long)m.nx * m.ny));
+    int i, j, k;
+    cell_ijk(m, c, i, j, k);
     const d3 cc = cell_centre<MK>(m, i, j, k);
     const d3 r{cc.x - s.center[0], cc.y - s.center[1], cc.z - s.center[2]};
     const d3 u = d3{s.vel[0], s.vel[1], s.vel[2]} + cross(d3{s.omega[0], s.omega[1], s.omega[2]}, r);
@@ -1332,7 +1334,7 @@ __global__ void k_fix_internal(MeshDev m, const int* __restrict__ Ct, const sdfi
 struct Vec3 {
     double v[3];
 };
-__global__ void k_evolve(sdfibm_solid* __restrict__ S, SolidDyn* __restrict__ D, const double* __restrict__ ft,
+__global__ void __launch_bounds__(64) k_evolve(sdfibm_solid* __restrict__ S, SolidDyn* __restrict__ D, const double* __restrict__ ft,
                          int ns, int n_sub, double dt, Vec3 g, double rho_f) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
@@ -1363,7 +1365,7 @@ __device__ inline int contact_cell(const ContactGrid& g, const double* c, int* i
 }
 
 // Cell lists of the finite solids (bounding radius > 0): head per cell, next per solid.
-__global__ void k_contact_insert(const sdfibm_solid* __restrict__ S, const double* __restrict__ rb, int ns,
+__global__ void __launch_bounds__(64) k_contact_insert(const sdfibm_solid* __restrict__ S, const double* __restrict__ rb, int ns,
                                  ContactGrid g, int* __restrict__ head, int* __restrict__ next) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns || !(rb[s] > 0)) return;
@@ -1433,7 +1435,7 @@ __device__ inline void contact_substep(const sdfibm_solid* __restrict__ Sin, int
 // the order the cell lists were built in.  Three head arrays and two next
 // arrays rotate over the sub-steps; this launch also resets the head array the
 // sub-step after the next one fills (the previous sub-step read it).
-__global__ void k_contact_step(const sdfibm_solid* __restrict__ Sin, sdfibm_solid* __restrict__ Sout,
+__global__ void __launch_bounds__(64) k_contact_step(const sdfibm_solid* __restrict__ Sin, sdfibm_solid* __restrict__ Sout,
                                SolidDyn* __restrict__ D, const double* __restrict__ ft, const double* __restrict__ rb,
                                int ns, ContactGrid g, const int* __restrict__ head, const int* __restrict__ next,
                                int* __restrict__ head_ins, int* __restrict__ next_ins, int* __restrict__ head_reset,
@@ -1459,7 +1461,8 @@ __global__ void k_contact_step(const sdfibm_solid* __restrict__ Sin, sdfibm_soli
 // grid): per sub-step build the cell lists, move every solid, clear the lists,
 // with workgroup barriers between the phases instead of kernel boundaries.
 // The same per-solid arithmetic and partner order as k_contact_step.
-__global__ void __launch_bounds__(1024) k_contact_evolve_block(
+template <int NT>
+__global__ void __launch_bounds__(NT) k_contact_evolve_block(
     sdfibm_solid* __restrict__ S0, sdfibm_solid* __restrict__ S1, SolidDyn* __restrict__ D,
     const double* __restrict__ ft, const double* __restrict__ rb, int ns, ContactGrid g, int* __restrict__ head,
     int* __restrict__ next, int ncells, const int* __restrict__ planes, int nplanes, ContactLaw law, double dts,
@@ -2112,10 +2115,15 @@ void launch_evolve(sdfibm_ctx* c, double dt, Vec3 g) {
     const int n_sub = ns == 1 ? 1 : 20;  // solidcloud.cpp:524-526
     if (contact_on(c) && ns > 1 && ns <= kContactBlockSolids && c->ncells_c <= kContactBlockCells) {
         // few solids: every sub-step in one workgroup (the result ends in d_solids: 20 sub-steps)
-        k_contact_evolve_block<<<1, 1024, 0, c->stream>>>(c->d_solids, c->d_solids_alt, c->d_dyn, c->d_ft, c->d_rb,
-                                                         ns, c->cgrid, c->d_chead, c->d_cnext, c->ncells_c,
-                                                         c->d_planes, (int)c->h_planes.size(), c->law, dt / n_sub, g,
-                                                         c->rho_f, n_sub);
+        // (up to 256 solids: a 256-thread workgroup, whose register budget holds the sub-step)
+        if (ns <= 256)
+            k_contact_evolve_block<256><<<1, 256, 0, c->stream>>>(
+                c->d_solids, c->d_solids_alt, c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, c->d_chead, c->d_cnext,
+                c->ncells_c, c->d_planes, (int)c->h_planes.size(), c->law, dt / n_sub, g, c->rho_f, n_sub);
+        else
+            k_contact_evolve_block<1024><<<1, 1024, 0, c->stream>>>(
+                c->d_solids, c->d_solids_alt, c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, c->d_chead, c->d_cnext,
+                c->ncells_c, c->d_planes, (int)c->h_planes.size(), c->law, dt / n_sub, g, c->rho_f, n_sub);
     } else if (contact_on(c) && ns > 1) {
         // sub-steps alternate between the two solid buffers (20: the result ends in d_solids);
         // cell lists: heads H[it % 3], nexts N[it % 2]

@@ -33,6 +33,31 @@ __host__ __device__ inline long long cell_id(const MeshDev& m, int i, int j, int
     return (long long)i + (long long)m.nx * ((long long)j + (long long)m.ny * k);
 }
 
+// a = q d + r (0 <= r < d) for 0 <= a < 2^52 through the FP64 product a * (1/d),
+// which is within one of the quotient: one correction step (a 64-bit integer
+// division would cost far more instructions).
+__device__ inline void divmod_i64(long long a, int d, double inv_d, long long& q, int& r) {
+    long long t = (long long)((double)a * inv_d);
+    long long rr = a - t * d;
+    if (rr < 0) {
+        --t;
+        rr += d;
+    } else if (rr >= d) {
+        ++t;
+        rr -= d;
+    }
+    q = t;
+    r = (int)rr;
+}
+
+// (i, j, k) of cell c
+__device__ inline void cell_ijk(const MeshDev& m, long long c, int& i, int& j, int& k) {
+    long long q, kk;
+    divmod_i64(c, m.nx, 1.0 / m.nx, q, i);
+    divmod_i64(q, m.ny, 1.0 / m.ny, kk, j);
+    k = (int)kk;
+}
+
 template <int MK>
 __device__ inline d3 vertex_pos(const MeshDev& m, int i, int j, int k) {
     if constexpr (MK == MK_RECT) {
