@@ -1330,6 +1330,37 @@ __global__ void __launch_bounds__(256) k_fix_internal(MeshDev m, const int* __re
     Uz[c] = R(u.z);
 }
 
+// U -= Fs*dt then fixInternal, in one pass over the cells, for the coupled loop
+// whose order is interact, U -= Fs*dt, evolve, fixInternal: evolve reads no
+// velocity field, so the two cell passes can both follow it.  A cell wholly in
+// a solid takes that solid's rigid velocity (its U - Fs*dt would be
+// overwritten); every other cell gets k_update_velocity's arithmetic.
+template <class R, int MK>
+__global__ void __launch_bounds__(256) k_update_fix(MeshDev m, const int* __restrict__ Ct,
+                                                    const sdfibm_solid* __restrict__ S, int ns, R* __restrict__ Ux,
+                                                    R* __restrict__ Uy, R* __restrict__ Uz, const R* __restrict__ Fx,
+                                                    const R* __restrict__ Fy, const R* __restrict__ Fz, double dt) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long N = (long long)m.nx * m.ny * m.nz;
+    if (c >= N) return;
+    const int t = Ct[c];
+    if (t < 4 || t - 4 >= ns) {
+        Ux[c] = R((double)Ux[c] - (double)Fx[c] * dt);
+        Uy[c] = R((double)Uy[c] - (double)Fy[c] * dt);
+        Uz[c] = R((double)Uz[c] - (double)Fz[c] * dt);
+        return;
+    }
+    const sdfibm_solid& s = S[t - 4];
+    int i, j, k;
+    cell_ijk(m, c, i, j, k);
+    const d3 cc = cell_centre<MK>(m, i, j, k);
+    const d3 r{cc.x - s.center[0], cc.y - s.center[1], cc.z - s.center[2]};
+    const d3 u = d3{s.vel[0], s.vel[1], s.vel[2]} + cross(d3{s.omega[0], s.omega[1], s.omega[2]}, r);
+    Ux[c] = R(u.x);
+    Uy[c] = R(u.y);
+    Uz[c] = R(u.z);
+}
+
 // SolidCloud::evolve (solidcloud.cpp:521-562), one thread per solid.
 struct Vec3 {
     double v[3];
@@ -2109,6 +2140,28 @@ void launch_fix_internal(sdfibm_ctx* c) {
 #undef SDFIBM_FIX
 }
 
+// U -= Fs*dt and fixInternal fused, after evolve (the coupled loop only)
+void launch_update_fix(sdfibm_ctx* c, double dt) {
+    const unsigned g = grid_for(c->N, 256);
+    const int ns = (int)c->ns;
+#define SDFIBM_UFIX(R_, MK_)                                                                                     \
+    k_update_fix<R_, MK_><<<g, 256, 0, c->stream>>>(c->m, c->Ct, c->d_solids, ns, (R_*)c->U[0], (R_*)c->U[1],   \
+                                                   (R_*)c->U[2], (const R_*)c->Fs[0], (const R_*)c->Fs[1],     \
+                                                   (const R_*)c->Fs[2], dt)
+    if (c->prec == SDFIBM_FP64) {
+        if (c->kind == SDFIBM_MESH_RECTILINEAR)
+            SDFIBM_UFIX(double, MK_RECT);
+        else
+            SDFIBM_UFIX(double, MK_STRUCT);
+    } else {
+        if (c->kind == SDFIBM_MESH_RECTILINEAR)
+            SDFIBM_UFIX(float, MK_RECT);
+        else
+            SDFIBM_UFIX(float, MK_STRUCT);
+    }
+#undef SDFIBM_UFIX
+}
+
 void launch_evolve(sdfibm_ctx* c, double dt, Vec3 g) {
     const int ns = (int)c->ns;
     if (ns == 0) return;
@@ -2699,9 +2752,8 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int r = dispatch_interact(c, dt, true);
         if (r == SDFIBM_OK) {
-            launch_update_velocity(c, dt);
             launch_evolve(c, dt, g);
-            launch_fix_internal(c);
+            launch_update_fix(c, dt);  // U -= Fs dt, then fixInternal with the evolved solids
             k_flag_or<<<1, 64, 0, c->stream>>>(c->err, c->d_sticky);
             if (np > 0)
                 k_plane_static<<<1, 64, 0, c->stream>>>(c->d_solids, c->d_planes, np, c->d_plane_pose, c->d_sticky);
