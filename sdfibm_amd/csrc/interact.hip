@@ -328,22 +328,26 @@ __global__ void k_tile_offsets(TileInfo* __restrict__ T, const long long* __rest
 }
 
 // Up to kScanOne solids: the tile sizes' exclusive scan and the tile offsets in
-// one workgroup of 1024 threads.  Wave w takes values [1024 w, 1024 w + 1024):
-// first their total (the 16 wave totals go through LDS), then 16 rounds of 64
-// consecutive values (coalesced loads) scanned by shuffles with a carry between
+// one workgroup of 1024 threads.  With R = ceil(ns / 4096) rounds, wave w takes
+// values [256 R w, 256 R (w + 1)): first their total (the 16 wave totals go
+// through LDS), then R rounds of 256 consecutive values, 4 per lane (coalesced
+// loads): sums within the lane, a shuffle scan of the lane sums, a carry between
 // rounds.  off[ns] = total; the other entries of off are not written.
 constexpr int kScanOne = 16384;
 
 __global__ void __launch_bounds__(1024) k_tile_offsets_one(const long long* __restrict__ cells, int ns,
                                                            TileInfo* __restrict__ T, long long* __restrict__ off,
                                                            long long cap, int* __restrict__ err) {
-    constexpr int PER = kScanOne / 1024;
     __shared__ long long wsum[16];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int base = wv * 64 * PER + lane;
+    const int rounds = (ns + 4095) / 4096;  // <= 4
+    const int wbase = wv * 256 * rounds;
     long long sum = 0;
+    for (int q = 0; q < rounds; ++q) {
+        const int e = wbase + 256 * q + 4 * lane;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) sum += base + 64 * q < ns ? cells[base + 64 * q] : 0;
+        for (int u = 0; u < 4; ++u) sum += e + u < ns ? cells[e + u] : 0;
+    }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
     if (lane == 0) wsum[wv] = sum;
@@ -355,17 +359,24 @@ __global__ void __launch_bounds__(1024) k_tile_offsets_one(const long long* __re
         if (w < wv) carry += ws;
         tot += ws;
     }
-#pragma unroll 1
-    for (int q = 0; q < PER; ++q) {
-        const int e = base + 64 * q;
-        const long long x0 = e < ns ? cells[e] : 0;
-        long long x = x0;
+    for (int q = 0; q < rounds && wbase + 256 * q < ns; ++q) {  // wave-uniform bound
+        const int e = wbase + 256 * q + 4 * lane;
+        long long v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = e + u < ns ? cells[e + u] : 0;
+        const long long own = v[0] + v[1] + v[2] + v[3];
+        long long x = own;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const long long y = __shfl_up(x, d, 64);
             if (lane >= d) x += y;
         }
-        if (e < ns) T[e].off = carry + x - x0;
+        long long run = carry + x - own;  // exclusive prefix of the lane's first value
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (e + u < ns) T[e + u].off = run;
+            run += v[u];
+        }
         carry += __shfl(x, 63, 64);
     }
     if (t == 0) {
