@@ -550,6 +550,15 @@ def main():
     copy_bw = copy_bandwidth()
     line["roofline"]["copy_peak_measured"] = copy_bw / 1e9
     line["roofline"]["frac_of_copy_peak"] = achieved / copy_bw
+    # the two kernels with algorithmic bytes run within a few per cent of each other: both lines
+    per_kernel = {}
+    for k in ("k_fields", "k_cells"):
+        a = kernels[k]["bytes"] / (kernels[k]["ms"] * 1e-3)
+        per_kernel[k] = {"ms": kernels[k]["ms"], "achieved": a / 1e9, "frac": a / HBM_PEAK,
+                         "algorithmic_bytes": kernels[k]["bytes"],
+                         "traffic": measured_traffic(k, f"cloud3d_{args.n}_{args.solids}_{args.precision}")}
+    per_kernel["k_cells"]["limit"] = "VALU issue (about two thirds of the SIMD issue rate), not HBM"
+    line["roofline"]["per_kernel"] = per_kernel
     if args.precision == "fp32" and not args.no_surrogate:
         line["surrogate"] = surrogate_comparison(cloud, sc, ft, f["As"], ms_per_step, phases[5], args.steps)
     if not args.no_cpu:

@@ -1606,7 +1606,7 @@ __global__ void k_fields_to_f64(const R* __restrict__ As, const R* __restrict__ 
 // ===========================================================================
 // host side
 // ===========================================================================
-constexpr int kBX3 = 32, kBY3 = 4, kBZ3 = 2, kCPT3 = 1;  // brick of a 3-D mesh, cells per thread
+constexpr int kBX3 = 32, kBY3 = 4, kBZ3 = 4, kCPT3 = 2;  // brick of a 3-D mesh, cells per thread
 constexpr int kBX2 = 32, kBY2 = 8, kBZ2 = 1;  // brick of a one-cell-thick (2-D) mesh
 
 struct sdfibm_ctx {
