@@ -118,6 +118,47 @@ def test_jittered_mesh(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("structured", [False, True])
+def test_ragged_stretched_mesh(prec, structured):
+    """37 x 23 x 13 cells (no axis a multiple of the field pass's brick), solids crossing the
+    mesh boundary, and a plane: every partial brick and every cell-per-thread slot past the mesh
+    edge.  Rectilinear: stretched spacing along x and z; structured: interior vertices jittered
+    by up to 0.2 cells (the structured kind takes meshes within 0.45 cells of a uniform grid)."""
+    rng = np.random.default_rng(77)
+    if structured:
+        m = M.jittered_box((37, 23, 13), (0.0, 0.0, 0.0), (1.0, 23 / 37, 13 / 37), 0.2, 78)
+    else:
+        xs = np.cumsum(np.r_[0.0, rng.uniform(0.8, 1.25, 37)])
+        xs /= xs[-1]
+        ys = np.linspace(0.0, 23 / 37, 24)
+        zs = np.cumsum(np.r_[0.0, rng.uniform(0.7, 1.3, 13)])
+        zs *= (13 / 37) / zs[-1]
+        m = M.rectilinear(xs, ys, zs)
+    h = 1.0 / 37
+    kinds = [("Sphere", dict(radius=3.1 * h)), ("Ellipsoid", dict(radiusa=4.2 * h, radiusb=2.3 * h, radiusc=1.7 * h)),
+             ("Box", dict(radiusa=2.5 * h, radiusb=1.6 * h, radiusc=3.3 * h)), ("Sphere", dict(radius=2.2 * h)),
+             ("Ellipsoid", dict(radiusa=3.0 * h, radiusb=3.0 * h, radiusc=2.0 * h)), ("Plane", {})]
+    sol = S.make_solids(len(kinds))
+    centres = [(0.02, 0.3, 0.1), (0.5, 0.31, 0.17), (0.97, 0.05, 0.33), (0.7, 0.6, 0.02), (0.3, 0.12, 0.3),
+               (0.5, 0.1 * h + 0.04, 0.2)]
+    for t, (name, prm) in enumerate(kinds):
+        sid, pv = S.make_shape(name, **prm)
+        sol[t]["shape"] = sid
+        sol[t]["prm"] = pv
+        sol[t]["center"] = centres[t]
+        sol[t]["q"] = S.random_unit_quaternions(rng, 1)[0] if name != "Plane" else S.quaternion_xyz((0.2, 0.0, 0.1))
+        if name != "Plane":
+            sol[t]["vel"] = rng.normal(0, 0.1, 3)
+            sol[t]["omega"] = rng.normal(0, 1.0, 3)
+    U = rng.normal(size=(m.n_cells, 3))
+    om = oracle.OracleMesh(m.to_polymesh())
+    ref = om.interact(sol, U, rho_f=1.1, dt=1e-3, capture=True)
+    assert (ref["counts"][:, :3].sum(1) > 0).all()
+    for c, ft in _run(m, sol, U, False, 1.1, 1e-3, prec, structured):
+        _compare(c, ref, ft, sol.size, prec, 1e-4 * np.abs(ref["ft"]).max())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_toolvof_fixture_on_gpu(prec):
     """The reference's alpha.water through the structured-mesh path (mesh points from the file)."""
     m, g = toolvof_mesh()
