@@ -386,6 +386,10 @@ __global__ void __launch_bounds__(1024) k_tile_offsets_one(const long long* __re
 }
 
 // Per-step resets: error flags, queue counters, brick counts.
+__global__ void k_flags_out(const int* __restrict__ err, int* __restrict__ host_word) {
+    if (threadIdx.x == 0) *(volatile int*)host_word = *err;
+}
+
 __global__ void k_reset(int* __restrict__ err, int* __restrict__ qmeta, int* __restrict__ bcount, long long nbricks) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *err = 0;
@@ -1615,6 +1619,8 @@ struct sdfibm_ctx {
     bool own_stream = false;
     hipStream_t aux = nullptr;                   // second stream of the tile classification
     hipEvent_t fork = nullptr, join = nullptr;  // its dependencies on the main stream
+    int* h_flags = nullptr;  // pinned, mapped host word: the step's error flags, written by k_flags_out
+    int* d_hflags = nullptr;  // its device address
     int kind = 0, prec = 0, two_d = 0;
     int nx = 0, ny = 0, nz = 0;
     long long N = 0;
@@ -2299,6 +2305,11 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
         c->msg = "hipEventCreate failed";
         return bail(SDFIBM_ERR_HIP);
     }
+    if (hipHostMalloc((void**)&c->h_flags, 64, hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->d_hflags, c->h_flags, 0) != hipSuccess) {
+        c->msg = "pinned host buffer for the error flags failed";
+        return bail(SDFIBM_ERR_HIP);
+    }
     const size_t rs = rsize(c);
     for (int a = 0; a < 3; ++a) {
         if ((rc = dalloc(c, &c->U[a], c->N * rs))) return bail(rc);
@@ -2344,6 +2355,7 @@ int sdfibm_destroy(sdfibm_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
+    if (c->h_flags) (void)hipHostFree(c->h_flags);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2530,9 +2542,11 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
     int rc = dispatch_interact(c, dt);
     if (rc) return rc;
-    int flags = 0;
-    HIPCHK(c, hipMemcpyAsync(&flags, c->err, 4, hipMemcpyDeviceToHost, c->stream));
+    // the flags come back through a pinned mapped word (no copy-engine round trip)
+    k_flags_out<<<1, 64, 0, c->stream>>>(c->err, c->d_hflags);
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    int flags = *(volatile int*)c->h_flags;
     if ((flags & EF_CODECAP) && !(flags & EF_TILE) && c->ns > 0) {
         // the solids moved or turned since set_solids and their tiles outgrew the code
         // buffer: grow it to the size this step needs (with headroom) and run the step again
