@@ -1619,7 +1619,7 @@ struct sdfibm_ctx {
     bool own_stream = false;
     hipStream_t aux = nullptr;                   // second stream of the tile classification
     hipEvent_t fork = nullptr, join = nullptr;  // its dependencies on the main stream
-    int* h_flags = nullptr;  // pinned, mapped host word: the step's error flags, written by k_flags_out
+    int* h_flags = nullptr;  // pinned, mapped host words written by k_flags_out: [0] error flags, [1] moved planes
     int* d_hflags = nullptr;  // its device address
     int kind = 0, prec = 0, two_d = 0;
     int nx = 0, ny = 0, nz = 0;
@@ -1875,9 +1875,9 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     if (!graph && np > 0) {
         // only the planes that moved since their last labelling are labelled again
         k_plane_pose<<<1, 64, 0, st>>>(c->d_solids, c->d_planes, np, c->d_plane_cache, c->d_pscalars + 3);
-        if (hipMemcpyAsync(&dirty, c->d_pscalars + 3, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-            return -1;
+        k_flags_out<<<1, 64, 0, st>>>(c->d_pscalars + 3, c->d_hflags + 1);  // pinned mapped word
+        if (hipStreamSynchronize(st) != hipSuccess) return -1;
+        dirty = (unsigned)*(volatile int*)(c->h_flags + 1);
     }
     for (int p = 0; p < (graph ? 0 : np); ++p) {
         if (!((dirty >> p) & 1u)) continue;
