@@ -261,6 +261,45 @@ def test_many_solids_multi_pass_scan():
             assert np.array_equal(counts, parts[s][0]) and np.array_equal(cells, parts[s][1]), s
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_code_buffer_grows_when_tiles_grow(prec):
+    """Elongated ellipsoids set along x, then turned by the per-step pose update (same shapes: no
+    re-estimate of the tile codes' size): their tiles outgrow the code buffer, which interact
+    grows before running the step again.  The result equals a fresh context's bit for bit."""
+    n = 24
+    h = 1.0 / n
+    m = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
+    rng = np.random.default_rng(31)
+    ns = 20
+    sol = S.make_solids(ns)
+    sol["shape"] = S.ELLIPSOID
+    sol["prm"][:, :3] = (6.0 * h, 1.5 * h, 1.5 * h)
+    sol["center"] = rng.uniform(0.3, 0.7, (ns, 3))
+    sol["q"] = (1.0, 0.0, 0.0, 0.0)
+    sol["vel"] = rng.normal(0, 0.1, (ns, 3))
+    U = rng.normal(size=(m.n_cells, 3))
+    turned = sol.copy()
+    turned["q"] = S.quaternion_xyz((0.7, 0.6, 0.8))
+    with SolidCloud(m, precision=prec) as a, SolidCloud(m, precision=prec) as b:
+        a.set_solids(sol)
+        a.set_velocity(U)
+        a.interact(0.0, 1e-3)
+        a.set_solids(turned)  # same shapes: the fast path, no new size estimate
+        fa = a.interact(0.0, 1e-3)
+        ga = a.fields()
+        b.set_solids(turned)
+        b.set_velocity(U)
+        fb = b.interact(0.0, 1e-3)
+        gb = b.fields()
+        np.testing.assert_array_equal(fa, fb)
+        for key in ("As", "Fs", "Ts", "Ct"):
+            np.testing.assert_array_equal(ga[key], gb[key])
+        for s_ in range(0, ns, 7):
+            ca, xa, aa = a.candidates(s_)
+            cb, xb, ab = b.candidates(s_)
+            assert np.array_equal(ca, cb) and np.array_equal(xa, xb) and np.array_equal(aa, ab)
+
+
 def test_seed_fallback_and_empty_solid():
     n = 16
     m = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
