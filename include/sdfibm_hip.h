@@ -35,9 +35,9 @@ typedef enum {
     SDFIBM_ERR_HIP = -2,       /* a HIP runtime call failed (message has the HIP error) */
     SDFIBM_ERR_NOMEM = -3,     /* device allocation failed */
     SDFIBM_ERR_SHAPE = -4,     /* unknown shape id or invalid shape parameters */
-    SDFIBM_ERR_TILE = -5,      /* a solid's cell tile outgrew the tile pass after set_solids (a solid
-                                  whose tile is too big at set_solids, > 63 cells along x or > 4096
-                                  vertex rows, goes the planes' path instead; <= 32 such plus planes) */
+    SDFIBM_ERR_TILE = -5,      /* a solid's cell tile grew beyond 2^22 vertices during run_steps' graph
+                                  replays (such a solid goes the planes' path instead -- at set_solids,
+                                  or at the interact that finds it grown; <= 32 such plus planes) */
     SDFIBM_ERR_STATE = -6,     /* call out of order (e.g. interact before set_solids) */
     SDFIBM_ERR_MESH = -7,      /* unsupported or inconsistent mesh description */
     SDFIBM_ERR_CAPACITY = -8,  /* an internal buffer was too small for this step */
@@ -55,8 +55,39 @@ typedef enum {
     SDFIBM_SHAPE_RECTANGLE = 5,      /* rectangle.h:10-60     prm: radiusa, radiusb, -, com(3) */
     SDFIBM_SHAPE_PLANE = 6,          /* plane.h:10-30         no parameters; inside: local y<0 */
     SDFIBM_SHAPE_CIRCLE_TAIL = 7,    /* circle_tail.h:10-63   prm: radius, ratio, thickness, com(3) */
-    SDFIBM_SHAPE_CIRCLE_TWOTAIL = 8  /* circle_twotail.h:10-68 prm: radius, ratio, thickness, com(3) */
+    SDFIBM_SHAPE_CIRCLE_TWOTAIL = 8, /* circle_twotail.h:10-68 prm: radius, ratio, thickness, com(3) */
+    SDFIBM_SHAPE_CSG = 9             /* prm[0]: index of its shape in the table of sdfibm_set_csg_shapes */
 } sdfibm_shape_id;
+
+/* CSG shapes: up to SDFIBM_CSG_MAX_PRIMS primitives in the solid's local frame,
+ * combined left to right with the reference's CSG operators (src/libshape/sdf/sdf.h:133-142:
+ * union U = min / or, difference D(a, b) = max(a, -b) / a and not b, intersection
+ * I = max / and), the signed distance filtered at the end like every shape
+ * (sdf.h:147-150).  Primitives: box and ellipsoid as sdf.h:43-57 and 72-83 (the
+ * ellipsoid's first-order distance), sphere, and a finite cylinder (distance: the
+ * rectangle formula of sdf.h:29-37 in radius/axial coordinates).  The reference has
+ * the operators and primitives but no 3-D CSG shape plugin; this one is the
+ * library's.  Like a shape plugin it declares its mass properties (IShape::m_volume,
+ * m_moi, m_radiusB, ishape.h:32-38), used by evolve and the contacts. */
+#define SDFIBM_CSG_MAX_PRIMS 8
+typedef enum { SDFIBM_CSG_BOX = 0, SDFIBM_CSG_ELLIPSOID = 1, SDFIBM_CSG_CYLINDER = 2, SDFIBM_CSG_SPHERE = 3 } sdfibm_csg_kind;
+typedef enum { SDFIBM_CSG_UNION = 0, SDFIBM_CSG_DIFFERENCE = 1, SDFIBM_CSG_INTERSECTION = 2 } sdfibm_csg_op;
+typedef struct {
+    int32_t kind;      /* sdfibm_csg_kind */
+    int32_t op;        /* sdfibm_csg_op combining it with the primitives before it (ignored for the first) */
+    int32_t axis;      /* cylinder axis in the solid's frame: 0, 1 or 2 */
+    int32_t reserved;
+    double center[3];  /* in the solid's frame */
+    double size[3];    /* box: half sides; ellipsoid: semi-axes; cylinder: radius, half length; sphere: radius */
+} sdfibm_csg_prim;     /* 64 bytes */
+typedef struct {
+    int32_t n_prims;   /* 1 .. SDFIBM_CSG_MAX_PRIMS */
+    int32_t reserved;
+    double volume;     /* > 0 */
+    double moi[3];     /* principal moments per unit density about the solid's origin, > 0 */
+    double radius_b;   /* bounding radius for the contact search, >= 0 */
+    sdfibm_csg_prim prim[SDFIBM_CSG_MAX_PRIMS];
+} sdfibm_csg_shape;    /* 560 bytes */
 
 /* One solid: shape + state.  Replaces Solid (src/solid.h:14-50) as far as the
  * coupling step reads it: centre, orientation, velocity, omega
@@ -115,8 +146,15 @@ int sdfibm_create(const sdfibm_mesh_desc* mesh, const sdfibm_config* cfg, sdfibm
 int sdfibm_destroy(sdfibm_ctx* ctx);
 
 /* Upload the solids (index order = the reference's m_solids order,
- * solidcloud.h:49).  Validates shape ids and parameters. */
+ * solidcloud.h:49).  Validates shape ids and parameters.  Solids of any size:
+ * tiles beyond the on-chip tile pass (more than 63 cells along x or 4096 vertex
+ * rows) take the big-tile pass; beyond 2^22 tile vertices a solid takes the
+ * planes' path (<= 32 such plus planes). */
 int sdfibm_set_solids(sdfibm_ctx* ctx, const sdfibm_solid* solids, int64_t n);
+
+/* The CSG shape table the context's CSG solids refer to (copied).  The solids
+ * must be set again afterwards (SDFIBM_ERR_STATE until then). */
+int sdfibm_set_csg_shapes(sdfibm_ctx* ctx, const sdfibm_csg_shape* shapes, int64_t n);
 
 /* Fluid velocity U: host AoS FP64 [N*3] (a volVectorField's internal field),
  * converted to the resident SoA layout in the context's precision. */
@@ -189,7 +227,9 @@ int sdfibm_set_contact(sdfibm_ctx* ctx, double stiffness, double damping);
  * plane solids must have motion mask b000000 and zero velocity and spin
  * (SDFIBM_ERR_STATE otherwise, and if a plane's pose changes during the
  * replays).  Capacity errors of any step are reported after the last one; the
- * state from the failing step on is then not valid.  ft: [ns*6] force/torque
+ * step whose interact failed and every later step leave the solids and U as the
+ * last valid step left them (its fields and force/torque are not valid), so the
+ * caller can call run_steps again, which grows the buffers first.  ft: [ns*6] force/torque
  * of the last step's interact, or NULL.  time: unused (no time-dependent
  * motions or forcers).  sdfibm_timing's total is the replays' device time. */
 int sdfibm_run_steps(sdfibm_ctx* ctx, double time, double dt, int64_t n_steps, const double* gravity, double* ft);
@@ -274,6 +314,12 @@ int sdfibm_candidates(sdfibm_ctx* ctx, int64_t solid, int64_t* counts, int64_t* 
  * (solid, cell) updates, sum_s |cellids_s| of solidcloud.cpp:403-422.  For
  * benchmarks. */
 int sdfibm_pair_counts(sdfibm_ctx* ctx, int64_t* out);
+
+/* Sizes of the context's step buffers (diagnostics and tests): out[0] = cells the
+ * tile-code buffer holds, out[1] = solids the per-solid buffers hold.  interact grows
+ * the code buffer when moving solids' tiles outgrow it.  Returns the number of values
+ * written (<= n, at most 2).  No counterpart in the reference. */
+int sdfibm_buffer_capacity(sdfibm_ctx* ctx, int64_t* out, int n);
 
 /* Timing of the phases of the last interact (milliseconds, HIP events on the
  * context's stream): [0] setup+binning, [1] per-solid tiles (classification,

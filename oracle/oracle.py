@@ -63,6 +63,7 @@ def lib():
         L.orc_fix_internal.argtypes = [P, P, P, i64, P]
         L.orc_mass_properties.argtypes = [P, P, P]
         L.orc_evolve.argtypes = [i64, P, P, P, C.c_double, C.c_double, P, C.c_double]
+        L.orc_set_csg.argtypes = [P, i64]
         L.orc_bounding_radius.restype = C.c_double
         L.orc_bounding_radius.argtypes = [P]
         L.orc_contact_forces.argtypes = [i64, P, C.c_double, C.c_double, P]
@@ -196,6 +197,16 @@ class OracleMesh:
         if rc != 0:
             raise RuntimeError(f"orc_interact_range failed: {rc}")
         return fields
+
+
+def set_csg(shapes):
+    """The CSG shape table (records of the product's CSG_DTYPE layout) the oracle's CSG solids use."""
+    global _csg_keep
+    _csg_keep = np.ascontiguousarray(shapes)
+    lib().orc_set_csg(_ptr(_csg_keep), int(_csg_keep.size))
+
+
+_csg_keep = None
 
 
 def is_inside(solid, p):

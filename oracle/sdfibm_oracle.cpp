@@ -90,6 +90,101 @@ static inline V3 rot30(V3 p) {
 }
 static inline V3 flipy(V3 p) { return {p.x, -p.y, p.z}; }
 
+// CSG shapes: the table set by orc_set_csg.  Primitive predicates are strict
+// comparisons without square roots; distances are the exact box (sdf.h:43-57),
+// the reference's first-order ellipsoid (sdf.h:72-83), the sphere and a finite
+// cylinder (sdf.h:29-37's rectangle in radius/axial coordinates); the operators
+// are sdf.h:133-142 (U: or / min; D(a, b): a and not b / max(a, -b); I: and / max).
+static std::vector<orc_csg_shape> g_csg;
+
+static inline void csg_frame(const orc_csg_prim& g, V3 p, double& x, double& y, double& z) {
+    x = p.x - g.center[0];
+    y = p.y - g.center[1];
+    z = p.z - g.center[2];
+}
+// radial pair and axial coordinate of a point for a cylinder along `axis`
+static inline void cyl_coords(int axis, double x, double y, double z, double& u, double& v, double& w) {
+    if (axis == 0) { u = y; v = z; w = x; }
+    else if (axis == 1) { u = x; v = z; w = y; }
+    else { u = x; v = y; w = z; }
+}
+
+static bool csg_prim_in(const orc_csg_prim& g, V3 p) {
+    double x, y, z;
+    csg_frame(g, p, x, y, z);
+    switch (g.kind) {
+    case ORC_CSG_BOX: return std::fabs(x) < g.size[0] && std::fabs(y) < g.size[1] && std::fabs(z) < g.size[2];
+    case ORC_CSG_ELLIPSOID: {
+        const double ai = 1.0 / (g.size[0] * g.size[0]), bi = 1.0 / (g.size[1] * g.size[1]),
+                     ci = 1.0 / (g.size[2] * g.size[2]);
+        return x * x * ai + y * y * bi + z * z * ci < 1.0;
+    }
+    case ORC_CSG_SPHERE: return x * x + y * y + z * z < g.size[0] * g.size[0];
+    case ORC_CSG_CYLINDER: {
+        double u, v, w;
+        cyl_coords(g.axis, x, y, z, u, v, w);
+        return u * u + v * v < g.size[0] * g.size[0] && std::fabs(w) < g.size[1];
+    }
+    default: return false;
+    }
+}
+
+static double csg_prim_dist(const orc_csg_prim& g, V3 p) {
+    double x, y, z;
+    csg_frame(g, p, x, y, z);
+    switch (g.kind) {
+    case ORC_CSG_BOX: {
+        const double dx = std::fabs(x) - g.size[0], dy = std::fabs(y) - g.size[1], dz = std::fabs(z) - g.size[2];
+        const double dxp = std::max(0.0, dx), dyp = std::max(0.0, dy), dzp = std::max(0.0, dz);
+        return std::sqrt(dxp * dxp + dyp * dyp + dzp * dzp) + std::min(0.0, std::max(dz, std::max(dx, dy)));
+    }
+    case ORC_CSG_ELLIPSOID: {
+        const double ai = 1.0 / (g.size[0] * g.size[0]), bi = 1.0 / (g.size[1] * g.size[1]),
+                     ci = 1.0 / (g.size[2] * g.size[2]);
+        const double xa = x * x * ai, yb = y * y * bi, zc = z * z * ci;
+        return 0.5 * (xa + yb + zc - 1.0) / (std::sqrt(xa * ai + yb * bi + zc * ci));
+    }
+    case ORC_CSG_SPHERE: return std::sqrt(x * x + y * y + z * z) - g.size[0];
+    case ORC_CSG_CYLINDER: {
+        double u, v, w;
+        cyl_coords(g.axis, x, y, z, u, v, w);
+        return rect_sd(V3{std::sqrt(u * u + v * v), w, 0.0}, g.size[0], g.size[1]);
+    }
+    default: return 1.0;
+    }
+}
+
+static bool csg_in(const orc_csg_shape& G, V3 p) {
+    bool in = csg_prim_in(G.prim[0], p);
+    for (int t = 1; t < G.n_prims; ++t) {
+        const bool b = csg_prim_in(G.prim[t], p);
+        switch (G.prim[t].op) {
+        case ORC_CSG_UNION: in = in || b; break;
+        case ORC_CSG_DIFFERENCE: in = in && !b; break;
+        default: in = in && b; break;
+        }
+    }
+    return in;
+}
+
+static double csg_dist(const orc_csg_shape& G, V3 p) {
+    double d = csg_prim_dist(G.prim[0], p);
+    for (int t = 1; t < G.n_prims; ++t) {
+        const double e = csg_prim_dist(G.prim[t], p);
+        switch (G.prim[t].op) {
+        case ORC_CSG_UNION: d = std::min(d, e); break;
+        case ORC_CSG_DIFFERENCE: d = std::max(d, -e); break;
+        default: d = std::max(d, e); break;
+        }
+    }
+    return d;
+}
+
+static const orc_csg_shape* csg_of(const orc_solid& s) {
+    const long long k = (long long)s.prm[0];
+    return (k >= 0 && k < (long long)g_csg.size()) ? &g_csg[k] : nullptr;
+}
+
 static inline V3 com_of(const orc_solid& s) {
     switch (s.shape) {
     case ORC_SPHERE:
@@ -135,6 +230,10 @@ static bool is_inside_local(const orc_solid& s, V3 p) {
         V3 q = com_of(s) + p;
         q.z = 0.0;
         return rect_bool(q, s.prm[0], s.prm[1]);
+    }
+    case ORC_CSG: {
+        const orc_csg_shape* G = csg_of(s);
+        return G ? csg_in(*G, p) : false;
     }
     case ORC_PLANE: return p.y < 0;  // plane.h:20-23
     case ORC_CIRCLE_TAIL: {          // circle_tail.h:42-50
@@ -201,6 +300,10 @@ static double signed_distance_local(const orc_solid& s, V3 p) {
         V3 q = com_of(s) + p;
         q.z = 0.0;
         return filter(rect_sd(q, s.prm[0], s.prm[1]));
+    }
+    case ORC_CSG: {
+        const orc_csg_shape* G = csg_of(s);
+        return G ? filter(csg_dist(*G, p)) : 1.0;
     }
     case ORC_PLANE: return p.y;  // plane.h:25-28 (no filter)
     case ORC_CIRCLE_TAIL: {      // circle_tail.h:52-61
@@ -945,6 +1048,10 @@ int64_t orc_mesh_cell_points(const orc_mesh* om, int64_t* offsets, int32_t* labe
     return (int64_t)m.cp.size();
 }
 
+void orc_set_csg(const orc_csg_shape* shapes, int64_t n) {
+    g_csg.assign(shapes, shapes + (n > 0 ? n : 0));
+}
+
 int orc_is_inside(const orc_solid* s, const double* p) { return phi01(*s, V3{p[0], p[1], p[2]}) ? 1 : 0; }
 
 double orc_signed_distance(const orc_solid* s, const double* p) { return phi(*s, V3{p[0], p[1], p[2]}); }
@@ -1102,6 +1209,16 @@ void orc_mass_properties(const orc_solid* sp, double* volume, double* moi3) {
         const double a = s.prm[0], b = s.prm[1];
         V = 4.0 * a * b;
         m0 = m1 = m2 = 1.0 / 3.0 * V * (a * a + b * b);
+        break;
+    }
+    case ORC_CSG: {  // the values the CSG shape record declares (IShape::m_volume, m_moi)
+        const orc_csg_shape* G = csg_of(s);
+        if (G) {
+            V = G->volume;
+            m0 = G->moi[0];
+            m1 = G->moi[1];
+            m2 = G->moi[2];
+        }
         break;
     }
     default: break;
@@ -1293,6 +1410,10 @@ double orc_bounding_radius(const orc_solid* s) {
     case 7:  // Circle_Tail, circle_tail.h:28,33: m_radiusB = 2 m_radiusa, m_radiusa = (ratio + 1) r / 2
     case 8:  // Circle_TwoTail, circle_twotail.h:26,31
         return 2 * ((s->prm[1] + 1) * 0.5 * s->prm[0]);
+    case ORC_CSG: {  // declared by the CSG shape record (IShape::m_radiusB)
+        const orc_csg_shape* G = csg_of(*s);
+        return G ? G->radius_b : 0.0;
+    }
     default:
         return 0.0;
     }

@@ -31,8 +31,30 @@ enum {
     ORC_PLANE = 6,
     ORC_CIRCLE_TAIL = 7,
     ORC_CIRCLE_TWOTAIL = 8,
-    ORC_NSHAPES = 9
+    ORC_CSG = 9,
+    ORC_NSHAPES = 10
 };
+
+/* CSG shapes (the product's sdfibm_csg_shape, same layout): up to 8 primitives
+ * (box, ellipsoid, finite cylinder, sphere) in the solid's local frame combined
+ * left to right with the reference's operators (sdf.h:133-142).  A CSG solid's
+ * prm[0] is the index of its shape in the table given to orc_set_csg. */
+enum { ORC_CSG_BOX = 0, ORC_CSG_ELLIPSOID = 1, ORC_CSG_CYLINDER = 2, ORC_CSG_SPHERE = 3 };
+enum { ORC_CSG_UNION = 0, ORC_CSG_DIFFERENCE = 1, ORC_CSG_INTERSECTION = 2 };
+typedef struct {
+    int32_t kind, op, axis, reserved;
+    double center[3];
+    double size[3];
+} orc_csg_prim;
+typedef struct {
+    int32_t n_prims, reserved;
+    double volume;
+    double moi[3];
+    double radius_b;
+    orc_csg_prim prim[8];
+} orc_csg_shape;
+/* The CSG table the oracle's CSG solids refer to (copied; process-wide). */
+void orc_set_csg(const orc_csg_shape* shapes, int64_t n);
 
 /* One rigid solid: shape parameters as written in a solidDict plus state.
  * prm meaning per shape (unused slots ignored):

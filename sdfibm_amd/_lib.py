@@ -19,7 +19,7 @@ EXPORTS = (
     "sdfibm_abi_version", "sdfibm_update_velocity", "sdfibm_fix_internal", "sdfibm_get_velocity",
     "sdfibm_set_dynamics", "sdfibm_evolve", "sdfibm_get_solids", "sdfibm_set_contact", "sdfibm_run_steps", "sdfibm_pair_counts",
     "sdfibm_surrogate_train", "sdfibm_surrogate_samples", "sdfibm_surrogate_eval", "sdfibm_surrogate_grad",
-    "sdfibm_set_surrogate", "sdfibm_surrogate_count",
+    "sdfibm_set_surrogate", "sdfibm_surrogate_count", "sdfibm_buffer_capacity", "sdfibm_set_csg_shapes",
 )
 
 SDFIBM_OK = 0
@@ -93,6 +93,8 @@ def load():
         "sdfibm_surrogate_grad": (i32, [P, P, P, i64, P, P]),
         "sdfibm_set_surrogate": (i32, [P, P, d, d, d]),
         "sdfibm_surrogate_count": (i32, [P, P]),
+        "sdfibm_buffer_capacity": (i32, [P, P, i32]),
+        "sdfibm_set_csg_shapes": (i32, [P, P, i64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from .mesh import StructuredHexMesh
-from .solids import SOLID_DTYPE
+from .solids import CSG_DTYPE, SOLID_DTYPE
 
 __all__ = ["SolidCloud", "DYNAMICS_DTYPE", "motion_mask"]
 
@@ -109,6 +109,12 @@ class SolidCloud:
         solids = np.ascontiguousarray(solids, dtype=SOLID_DTYPE)
         _lib.check(_lib.load().sdfibm_set_solids(self._h, _p(solids), solids.size), self._h)
         self.n_solids = solids.size
+
+    def set_csg_shapes(self, shapes: np.ndarray):
+        """The CSG shape table (records of solids.CSG_DTYPE, e.g. from solids.csg_shape); a CSG
+        solid's prm[0] is its index here.  Call set_solids again afterwards."""
+        shapes = np.ascontiguousarray(np.atleast_1d(shapes), dtype=CSG_DTYPE)
+        _lib.check(_lib.load().sdfibm_set_csg_shapes(self._h, _p(shapes), shapes.size), self._h)
 
     def set_velocity(self, U: np.ndarray):
         U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.n_cells, 3)
@@ -209,6 +215,12 @@ class SolidCloud:
         """(ALL_INSIDE pairs, cut pairs) of the last interact, summed over all solids."""
         out = np.zeros(2, dtype=np.int64)
         _lib.check(_lib.load().sdfibm_pair_counts(self._h, _p(out)), self._h)
+        return int(out[0]), int(out[1])
+
+    def buffer_capacity(self) -> tuple:
+        """(cells the tile-code buffer holds, solids the per-solid buffers hold)."""
+        out = np.zeros(2, dtype=np.int64)
+        _lib.load().sdfibm_buffer_capacity(self._h, _p(out), 2)
         return int(out[0]), int(out[1])
 
     def timing(self) -> np.ndarray:

@@ -15,6 +15,7 @@ struct SolidDyn {
     int first_fluid;    // Solid::first_fluid_step
     double ff_old[3];   // fluid force and torque of the previous step (storeOldForce)
     double ft_old[3];
+    double vol, moi[3]; // CSG solids: the mass properties their shape record declares
 };
 
 // Shape volume and principal moments of inertia per unit density: the m_volume
@@ -79,6 +80,10 @@ struct BodyConst {
 __host__ __device__ inline BodyConst body_const(const sdfibm_solid& s, const SolidDyn& d) {
     double vol, moi[3];
     mass_properties(s, vol, moi);
+    if (s.shape == SDFIBM_SHAPE_CSG) {
+        vol = d.vol;
+        for (int a = 0; a < 3; ++a) moi[a] = d.moi[a];
+    }
     const double volinv = vol > 0 ? 1.0 / vol : 0.0;
     BodyConst b;
     b.mass = vol * d.rho;

@@ -35,6 +35,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -58,7 +59,7 @@ constexpr int kMaxPlanes = 32;  // plane and oversized solids (unbounded path)
 constexpr int kContactBlockSolids = 1024;   // evolve with contacts in one workgroup up to this many solids
 constexpr int kContactBlockCells = 1 << 16;  // ... and contact grid cells
 
-enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_BRICK = 8, EF_PLANE_MOVED = 16 };
+enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_BRICK = 8, EF_PLANE_MOVED = 16, EF_LABEL = 32 };
 
 // ---------------------------------------------------------------------------
 // small device helpers
@@ -140,27 +141,31 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
 }
 
 // Work queues of the per-solid tile pass, one per tile class (0 small, 1 medium,
-// 2 large; k_setup moves small tiles of at most kRowsTiny vertex rows to 3, tiny):
-// qmeta[q] = number of solids queued, qlist[q * ns + t] = the t-th of them.
+// 2 large; k_setup moves small tiles of at most kRowsTiny vertex rows to 3, tiny;
+// 4 big, the big-tile pass): qmeta[q] = number of solids queued, qlist[q * ns + t]
+// = the t-th of them.
+constexpr int kQueues = 5;
 __host__ __device__ inline int tile_queue(int cls) {
-    return cls == TC_SMALL ? 0 : (cls == TC_MEDIUM ? 1 : (cls == TC_LARGE ? 2 : -1));
+    return cls == TC_SMALL ? 0 : (cls == TC_MEDIUM ? 1 : (cls == TC_LARGE ? 2 : (cls == TC_BIG ? 4 : -1)));
 }
 
 template <int MK>
 __global__ void __launch_bounds__(64) k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, SolidK* __restrict__ K,
                         TileInfo* __restrict__ T, long long* __restrict__ tile_cells, double* __restrict__ ft,
                         int* __restrict__ qmeta, int* __restrict__ qlist, const unsigned char* __restrict__ unb,
-                        int* __restrict__ err) {
+                        int* __restrict__ err, const CsgK* __restrict__ csg, int big_on) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    const SolidK k = make_solid_k(in[s]);
+    const SolidK k = make_solid_k(in[s], csg);
     K[s] = k;
     TileInfo t = make_tile<MK>(m, k);
     // solids set_solids put on the unbounded path stay there (their tile box is the cull)
     if (unb[s] && t.cls != TC_PLANE) t.cls = t.cls == TC_EMPTY ? TC_EMPTY : TC_HUGE;
     // structured meshes: the seed search (a box of cell centres) is spread over a wave by k_seed
     t.seed = t.cls == TC_EMPTY ? -1 : (MK == MK_RECT ? nearest_cell<MK>(m, k.c) : -2);
-    if (t.cls == TC_TOOBIG) atomicOr(err, EF_TILE);
+    // a tile beyond 2^22 vertices, or a big tile when set_solids found none possible (the big-tile
+    // pass is not launched): interact classifies the solids again
+    if (t.cls == TC_TOOBIG || (t.cls == TC_BIG && !big_on)) atomicOr(err, EF_TILE);
     tile_cells[s] = tile_is_finite(t.cls) ? (long long)t.n[0] * t.n[1] * t.n[2] : 0;
     T[s] = t;
     for (int q = 0; q < 6; ++q) ft[6 * s + q] = 0.0;  // solids that touch no cell exert nothing
@@ -169,7 +174,7 @@ __global__ void __launch_bounds__(64) k_setup(const sdfibm_solid* __restrict__ i
     int q = tile_queue(t.cls);
     if (q == 0 && (t.n[1] + 1) * (t.n[2] + 1) <= kRowsTiny) q = 3;  // tiny tiles: a wave each
     const int lane = threadIdx.x & 63;
-    for (int qq = 0; qq < 4; ++qq) {
+    for (int qq = 0; qq < kQueues; ++qq) {
         const unsigned long long b = __ballot(q == qq);
         if (!b) continue;
         const int leader = __builtin_ctzll(b);
@@ -734,6 +739,45 @@ __global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK
     }  // solids of the workgroup
 }
 
+// The forcing of one candidate cell (solidcloud.cpp:384-422): f = alpha (uf - us),
+// force += f V / dt, torque += (cc - c) x f V / dt, in FP64.
+template <class R, int MK>
+__device__ inline void add_force(const MeshDev& m, const SolidK& K, int gi, int gj, int gk, double alpha,
+                                 const R* __restrict__ Ux, const R* __restrict__ Uy, const R* __restrict__ Uz,
+                                 double dtINV, double* acc) {
+    const long long c = cell_id(m, gi, gj, gk);
+    const d3 cc = cell_centre<MK>(m, gi, gj, gk);
+    const double V = cell_volume<MK>(m, gi, gj, gk);
+    const d3 rv = {cc.x - K.c[0], cc.y - K.c[1], cc.z - K.c[2]};
+    const d3 us = d3{K.v[0], K.v[1], K.v[2]} + cross(d3{K.w[0], K.w[1], K.w[2]}, rv);
+    const d3 uf{(double)Ux[c], (double)Uy[c], (double)Uz[c]};
+    const d3 f = alpha * (uf - us);
+    const d3 tq = cross(rv, f);
+    acc[0] += (f.x * V) * dtINV;
+    acc[1] += (f.y * V) * dtINV;
+    acc[2] += (f.z * V) * dtINV;
+    acc[3] += (tq.x * V) * dtINV;
+    acc[4] += (tq.y * V) * dtINV;
+    acc[5] += (tq.z * V) * dtINV;
+}
+
+// One candidate cell: a cut cell's fraction and centre test (its code: the
+// fraction, negative when the centre is outside), then the forcing (alpha = 1
+// for ALL_INSIDE cells, solidcloud.cpp:408-410).
+template <class R, int MK>
+__device__ inline void cell_force(const MeshDev& m, const SolidK& K, int gi, int gj, int gk, bool cut,
+                                  const R* __restrict__ Ux, const R* __restrict__ Uy, const R* __restrict__ Uz,
+                                  R* code, double dtINV, double* acc) {
+    double alpha = 1.0;
+    if (cut) {
+        const R a = cell_fraction<R, MK>(m, K, gi, gj, gk);
+        const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
+        *code = cin ? a : -a;
+        alpha = (double)a;
+    }
+    add_force<R, MK>(m, K, gi, gj, gk, alpha, Ux, Uy, Uz, dtINV, acc);
+}
+
 // Forcing, fractions and the force/torque reduction over the candidate lists:
 // one workgroup per queued solid, largest tile class first.  SG: the learned
 // surrogate gives the fraction of a cut cell (FP32, rectilinear mesh, 3-D,
@@ -791,8 +835,8 @@ __global__ void __launch_bounds__(NT) k_cells(MeshDev m, int ns, const SolidK* _
         const bool cut = live && q >= nin;
         const int li = i + nx * (j + ny * k);
         const int gi = T.lo[0] + i, gj = T.lo[1] + j, gk = T.lo[2] + k;
-        double alpha = 1.0;
         if constexpr (SG) {
+            double alpha = 1.0;
             Phi8<float> h{0, 0, 0, 0, 0, 0, 0, 0};
             d3 lc{0, 0, 0};
             float g[3] = {0.f, 0.f, 0.f};
@@ -822,29 +866,10 @@ __global__ void __launch_bounds__(NT) k_cells(MeshDev m, int ns, const SolidK* _
                 tcodes[li] = R(cin ? a : -a);
                 alpha = (double)a;
             }
+            if (live) add_force<R, MK>(m, K, gi, gj, gk, alpha, Ux, Uy, Uz, dtINV, acc);
         } else {
-            if (cut) {
-                const R a = cell_fraction<R, MK>(m, K, gi, gj, gk);
-                const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
-                tcodes[li] = cin ? a : -a;
-                alpha = (double)a;
-            }
+            if (live) cell_force<R, MK>(m, K, gi, gj, gk, cut, Ux, Uy, Uz, tcodes + li, dtINV, acc);
         }
-        if (!live) continue;
-        const long long c = cell_id(m, gi, gj, gk);
-        const d3 cc = cell_centre<MK>(m, gi, gj, gk);
-        const double V = cell_volume<MK>(m, gi, gj, gk);
-        const d3 rv = {cc.x - K.c[0], cc.y - K.c[1], cc.z - K.c[2]};
-        const d3 us = d3{K.v[0], K.v[1], K.v[2]} + cross(d3{K.w[0], K.w[1], K.w[2]}, rv);
-        const d3 uf{(double)Ux[c], (double)Uy[c], (double)Uz[c]};
-        const d3 f = alpha * (uf - us);
-        const d3 tq = cross(rv, f);
-        acc[0] += (f.x * V) * dtINV;
-        acc[1] += (f.y * V) * dtINV;
-        acc[2] += (f.z * V) * dtINV;
-        acc[3] += (tq.x * V) * dtINV;
-        acc[4] += (tq.y * V) * dtINV;
-        acc[5] += (tq.z * V) * dtINV;
     }
     if constexpr (SG) {
         for (int off = 32; off >= 1; off >>= 1) n_net += __shfl_xor(n_net, off, 64);
@@ -1053,6 +1078,8 @@ __global__ void __launch_bounds__(256) k_plane_alpha(MeshDev m, const int* __res
     }
     pcode[c] = code;
 }
+
+#include "big_tiles.h"
 
 // Field pass: one workgroup per brick of BX*BY*BZ cells, CPT cells per thread
 // (the thread's cells are CPT layers apart along z: lane (ti, tj) of layer tk
@@ -1354,10 +1381,11 @@ template <class R, int MK>
 __global__ void __launch_bounds__(256) k_update_fix(MeshDev m, const int* __restrict__ Ct,
                                                     const sdfibm_solid* __restrict__ S, int ns, R* __restrict__ Ux,
                                                     R* __restrict__ Uy, R* __restrict__ Uz, const R* __restrict__ Fx,
-                                                    const R* __restrict__ Fy, const R* __restrict__ Fz, double dt) {
+                                                    const R* __restrict__ Fy, const R* __restrict__ Fz, double dt,
+                                                    const int* __restrict__ gate) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long N = (long long)m.nx * m.ny * m.nz;
-    if (c >= N) return;
+    if (c >= N || (gate && *gate)) return;
     const int t = Ct[c];
     if (t < 4 || t - 4 >= ns) {
         Ux[c] = R((double)Ux[c] - (double)Fx[c] * dt);
@@ -1381,9 +1409,9 @@ struct Vec3 {
     double v[3];
 };
 __global__ void __launch_bounds__(64) k_evolve(sdfibm_solid* __restrict__ S, SolidDyn* __restrict__ D, const double* __restrict__ ft,
-                         int ns, int n_sub, double dt, Vec3 g, double rho_f) {
+                         int ns, int n_sub, double dt, Vec3 g, double rho_f, const int* __restrict__ gate) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
+    if (s >= ns || (gate && *gate)) return;
     sdfibm_solid sol = S[s];
     SolidDyn d = D[s];
     evolve_solid(sol, d, ft + 6LL * s, n_sub, dt, g.v, rho_f);
@@ -1412,9 +1440,10 @@ __device__ inline int contact_cell(const ContactGrid& g, const double* c, int* i
 
 // Cell lists of the finite solids (bounding radius > 0): head per cell, next per solid.
 __global__ void __launch_bounds__(64) k_contact_insert(const sdfibm_solid* __restrict__ S, const double* __restrict__ rb, int ns,
-                                 ContactGrid g, int* __restrict__ head, int* __restrict__ next) {
+                                 ContactGrid g, int* __restrict__ head, int* __restrict__ next,
+                                 const int* __restrict__ gate) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns || !(rb[s] > 0)) return;
+    if (s >= ns || !(rb[s] > 0) || (gate && *gate)) return;
     int ijk[3];
     const int cell = contact_cell(g, S[s].center, ijk);
     next[s] = atomicExch(head + cell, s);
@@ -1486,7 +1515,8 @@ __global__ void __launch_bounds__(64) k_contact_step(const sdfibm_solid* __restr
                                int ns, ContactGrid g, const int* __restrict__ head, const int* __restrict__ next,
                                int* __restrict__ head_ins, int* __restrict__ next_ins, int* __restrict__ head_reset,
                                int ncells, const int* __restrict__ planes, int nplanes, ContactLaw law, double dts,
-                               Vec3 gv, double rho_f, int last) {
+                               Vec3 gv, double rho_f, int last, const int* __restrict__ gate) {
+    if (gate && *gate) return;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (int t = tid; t < ncells; t += gridDim.x * blockDim.x) head_reset[t] = -1;
     if (tid >= ns) return;
@@ -1512,7 +1542,8 @@ __global__ void __launch_bounds__(NT) k_contact_evolve_block(
     sdfibm_solid* __restrict__ S0, sdfibm_solid* __restrict__ S1, SolidDyn* __restrict__ D,
     const double* __restrict__ ft, const double* __restrict__ rb, int ns, ContactGrid g, int* __restrict__ head,
     int* __restrict__ next, int ncells, const int* __restrict__ planes, int nplanes, ContactLaw law, double dts,
-    Vec3 gv, double rho_f, int n_sub) {
+    Vec3 gv, double rho_f, int n_sub, const int* __restrict__ gate) {
+    if (gate && *gate) return;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int t = tid; t < ncells; t += nt) head[t] = -1;
     __syncthreads();
@@ -1676,6 +1707,7 @@ struct sdfibm_ctx {
     // d_cnext: two next arrays of contact_cap each
     int* d_cnext = nullptr;
     long long contact_cap = 0, chead_cap = 0;
+    bool contact_ready = false;  // contact buffers sized for the current solids (contact_prepare)
     bool small_tiles = false;  // mean tile of the finite solids under 512 cells (set_solids)
     ContactGrid cgrid{};
     int ncells_c = 0;
@@ -1699,6 +1731,18 @@ struct sdfibm_ctx {
     double* d_plane_cache = nullptr;  // per plane: the pose its cell sets were labelled for (8 doubles)
     int* d_pseed = nullptr;
     long long member_words = 0;
+    double* d_ustage = nullptr;  // set_velocity's AoS staging buffer, N * 3 doubles (first call)
+    // CSG shapes (set_csg_shapes): the caller's records and their derived constants
+    std::vector<sdfibm_csg_shape> h_csg_raw;
+    std::vector<CsgK> h_csg;
+    CsgK* d_csg = nullptr;
+    // big-tile pass (tiles beyond the on-chip tile pass): launched when some solid can have one
+    bool big_on = false;
+    BigMeta* d_big_meta = nullptr;  // kBigMaxSolids
+    long long* d_big_tot = nullptr;  // kBigTotN
+    unsigned long long* d_big_vbits = nullptr;
+    double* d_big_part = nullptr;
+    long long big_word_cap = 0, big_chunk_cap = 0;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double timing[6] = {0, 0, 0, 0, 0, 0};
     std::string msg;
@@ -1726,6 +1770,10 @@ int fail(sdfibm_ctx* c, int code, const std::string& m) {
 int dalloc(sdfibm_ctx* c, void** p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) bytes = 8;
+    // failure injection for the error-path tests: allocations above this many bytes fail
+    if (const char* lim = std::getenv("SDFIBM_TEST_ALLOC_LIMIT"))
+        if (*lim && bytes > (size_t)std::strtoull(lim, nullptr, 10))
+            return fail(c, SDFIBM_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + ") refused by SDFIBM_TEST_ALLOC_LIMIT");
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
@@ -1736,11 +1784,16 @@ int dalloc(sdfibm_ctx* c, void** p, size_t bytes) {
     return SDFIBM_OK;
 }
 
-void dfree(sdfibm_ctx* c, void* p) {
+// Frees a context allocation and nulls the caller's pointer, so a later failing
+// allocation never leaves a freed address behind in the context.
+template <class T>
+void dfree(sdfibm_ctx* c, T*& p) {
     if (!p) return;
-    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+    void* v = (void*)p;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), v);
     if (it != c->allocs.end()) c->allocs.erase(it);
-    (void)hipFree(p);
+    (void)hipFree(v);
+    p = nullptr;
 }
 
 size_t rsize(const sdfibm_ctx* c) { return c->prec == SDFIBM_FP64 ? sizeof(double) : sizeof(float); }
@@ -1753,8 +1806,8 @@ bool finite3(const double* v, int n) {
     return true;
 }
 
-int validate_solid(const sdfibm_solid& s, long long idx, std::string& why) {
-    if (s.shape < 0 || s.shape > SDFIBM_SHAPE_CIRCLE_TWOTAIL) {
+int validate_solid(const sdfibm_solid& s, long long idx, std::string& why, long long n_csg) {
+    if (s.shape < 0 || s.shape > SDFIBM_SHAPE_CSG) {
         why = "solid " + std::to_string(idx) + ": unknown shape id " + std::to_string(s.shape);
         return SDFIBM_ERR_SHAPE;
     }
@@ -1790,7 +1843,60 @@ int validate_solid(const sdfibm_solid& s, long long idx, std::string& why) {
         why = "solid " + std::to_string(idx) + ": tail ratio must be >= 0 and thickness > 0";
         return SDFIBM_ERR_SHAPE;
     }
+    if (s.shape == SDFIBM_SHAPE_CSG && !(s.prm[0] >= 0 && s.prm[0] < (double)n_csg && s.prm[0] == std::floor(s.prm[0]))) {
+        why = "solid " + std::to_string(idx) + ": CSG shape index " + std::to_string(s.prm[0]) +
+              " is not in the table of set_csg_shapes (" + std::to_string(n_csg) + " shapes)";
+        return SDFIBM_ERR_SHAPE;
+    }
     return SDFIBM_OK;
+}
+
+// A CSG shape record: primitive count, kinds, operations, axes, sizes, mass properties.
+int validate_csg(const sdfibm_csg_shape& g, long long idx, std::string& why) {
+    auto bad = [&](const std::string& w) {
+        why = "CSG shape " + std::to_string(idx) + ": " + w;
+        return SDFIBM_ERR_SHAPE;
+    };
+    if (g.n_prims < 1 || g.n_prims > SDFIBM_CSG_MAX_PRIMS) return bad("1 to 8 primitives");
+    if (!(g.volume > 0) || !std::isfinite(g.volume) || !(g.radius_b >= 0) || !std::isfinite(g.radius_b))
+        return bad("volume must be > 0 and the bounding radius >= 0");
+    for (int a = 0; a < 3; ++a)
+        if (!(g.moi[a] > 0) || !std::isfinite(g.moi[a])) return bad("moments of inertia must be > 0");
+    for (int t = 0; t < g.n_prims; ++t) {
+        const sdfibm_csg_prim& p = g.prim[t];
+        if (p.kind < SDFIBM_CSG_BOX || p.kind > SDFIBM_CSG_SPHERE) return bad("unknown primitive kind");
+        if (p.op < SDFIBM_CSG_UNION || p.op > SDFIBM_CSG_INTERSECTION) return bad("unknown operation");
+        if (p.kind == SDFIBM_CSG_CYLINDER && (p.axis < 0 || p.axis > 2)) return bad("cylinder axis must be 0, 1 or 2");
+        if (!finite3(p.center, 3) || !finite3(p.size, 3)) return bad("non-finite primitive");
+        const int npos = p.kind == SDFIBM_CSG_SPHERE ? 1 : (p.kind == SDFIBM_CSG_CYLINDER ? 2 : 3);
+        for (int a = 0; a < npos; ++a)
+            if (!(p.size[a] > 0)) return bad("primitive sizes must be > 0");
+    }
+    return SDFIBM_OK;
+}
+
+// The big-tile pass (big_tiles.h) on stream st: every big solid of the step, in
+// persistent grids that take their work from the device-side layout.
+template <class R, int MK>
+void launch_big(sdfibm_ctx* c, double dt, hipStream_t st) {
+    const int ns = (int)c->ns;
+    const unsigned G = (unsigned)(4 * c->n_cu);
+    R* codes = (R*)c->d_codes;
+    k_big_layout<<<1, 1024, 0, st>>>(ns, c->d_T, c->qmeta, c->qlist, c->d_big_meta, c->d_big_tot, c->big_word_cap,
+                                     c->err);
+    k_big_vertices<MK><<<G, 256, 0, st>>>(c->m, c->d_K, c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits);
+    k_big_cells<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, c->code_cap);
+    k_big_hook<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand, c->code_cap);
+    k_big_compress<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand, c->code_cap);
+    k_big_member<R><<<G, 256, 0, st>>>(c->m, c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, codes,
+                                       c->code_cap, c->err);
+    k_big_list<R><<<(unsigned)c->n_cu, 1024, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, codes, c->d_cand,
+                                                      c->d_ncand, c->code_cap);
+    k_big_chunks<<<1, 1024, 0, st>>>(c->d_big_meta, c->d_big_tot, c->big_chunk_cap, c->err);
+    k_big_force<R, MK, kBlock><<<G, kBlock, 0, st>>>(c->m, c->d_K, c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand,
+                                                     c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
+                                                     (const R*)c->U[2], codes, c->code_cap, dt, c->d_big_part);
+    k_big_ft<<<4, 256, 0, st>>>(c->d_big_meta, c->d_big_tot, c->d_big_part, c->rho_f, c->d_ft);
 }
 
 // graph: the launches are being captured into a HIP graph -- no timing events
@@ -1804,7 +1910,8 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     k_reset<<<grid_for(std::max(c->nbricks, 8LL), 256), 256, 0, st>>>(c->err, c->qmeta, c->bcount, c->nbricks);
     if (ns > 0) {
         k_setup<MK><<<grid_for(ns, 64), 64, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_cells,
-                                                       c->d_ft, c->qmeta, c->qlist, c->d_unb, c->err);
+                                                       c->d_ft, c->qmeta, c->qlist, c->d_unb, c->err, c->d_csg,
+                                                       c->big_on ? 1 : 0);
         if (MK != MK_RECT) k_seed<<<grid_for(ns, 4), 256, 0, st>>>(c->m, ns, c->d_K, c->d_T);
         if (ns <= kScanOne) {
             k_tile_offsets_one<<<1, 1024, 0, st>>>(c->d_tile_cells, ns, c->d_T, c->d_tile_off, c->code_cap, c->err);
@@ -1835,6 +1942,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
             c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
         k_classify<R, MK, kRowsMedium, 1, 1024><<<std::min(g, 4 * cus), 1024, 0, c->aux>>>(
             c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
+        if (c->big_on) launch_big<R, MK>(c, dt, c->aux);
         if (hipEventRecord(c->join, c->aux) != hipSuccess) return -1;
         k_classify<R, MK, kRowsSmall, 0, kBlock><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
                                                                c->d_cand, c->code_cap, c->d_ncand);
@@ -2114,6 +2222,7 @@ int dispatch_interact(sdfibm_ctx* c, double dt, bool graph = false) {
 
 bool contact_on(const sdfibm_ctx* c);
 int contact_prepare(sdfibm_ctx* c);
+int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n);
 
 // Drop the captured step (its launches hold the buffers and settings of capture time).
 void graph_reset(sdfibm_ctx* c) {
@@ -2158,13 +2267,14 @@ void launch_fix_internal(sdfibm_ctx* c) {
 }
 
 // U -= Fs*dt and fixInternal fused, after evolve (the coupled loop only)
-void launch_update_fix(sdfibm_ctx* c, double dt) {
+// gate: a device word that skips the pass when nonzero (the graph loop's sticky error flags), or null.
+void launch_update_fix(sdfibm_ctx* c, double dt, const int* gate) {
     const unsigned g = grid_for(c->N, 256);
     const int ns = (int)c->ns;
 #define SDFIBM_UFIX(R_, MK_)                                                                                     \
     k_update_fix<R_, MK_><<<g, 256, 0, c->stream>>>(c->m, c->Ct, c->d_solids, ns, (R_*)c->U[0], (R_*)c->U[1],   \
                                                    (R_*)c->U[2], (const R_*)c->Fs[0], (const R_*)c->Fs[1],     \
-                                                   (const R_*)c->Fs[2], dt)
+                                                   (const R_*)c->Fs[2], dt, gate)
     if (c->prec == SDFIBM_FP64) {
         if (c->kind == SDFIBM_MESH_RECTILINEAR)
             SDFIBM_UFIX(double, MK_RECT);
@@ -2179,7 +2289,8 @@ void launch_update_fix(sdfibm_ctx* c, double dt) {
 #undef SDFIBM_UFIX
 }
 
-void launch_evolve(sdfibm_ctx* c, double dt, Vec3 g) {
+// gate: as for launch_update_fix.
+void launch_evolve(sdfibm_ctx* c, double dt, Vec3 g, const int* gate) {
     const int ns = (int)c->ns;
     if (ns == 0) return;
     const int n_sub = ns == 1 ? 1 : 20;  // solidcloud.cpp:524-526
@@ -2189,11 +2300,11 @@ void launch_evolve(sdfibm_ctx* c, double dt, Vec3 g) {
         if (ns <= 256)
             k_contact_evolve_block<256><<<1, 256, 0, c->stream>>>(
                 c->d_solids, c->d_solids_alt, c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, c->d_chead, c->d_cnext,
-                c->ncells_c, c->d_planes, (int)c->h_planes.size(), c->law, dt / n_sub, g, c->rho_f, n_sub);
+                c->ncells_c, c->d_planes, (int)c->h_planes.size(), c->law, dt / n_sub, g, c->rho_f, n_sub, gate);
         else
             k_contact_evolve_block<1024><<<1, 1024, 0, c->stream>>>(
                 c->d_solids, c->d_solids_alt, c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, c->d_chead, c->d_cnext,
-                c->ncells_c, c->d_planes, (int)c->h_planes.size(), c->law, dt / n_sub, g, c->rho_f, n_sub);
+                c->ncells_c, c->d_planes, (int)c->h_planes.size(), c->law, dt / n_sub, g, c->rho_f, n_sub, gate);
     } else if (contact_on(c) && ns > 1) {
         // sub-steps alternate between the two solid buffers (20: the result ends in d_solids);
         // cell lists: heads H[it % 3], nexts N[it % 2]
@@ -2203,26 +2314,39 @@ void launch_evolve(sdfibm_ctx* c, double dt, Vec3 g) {
         const double dts = dt / n_sub;
         const int nplanes = (int)c->h_planes.size();
         k_contact_insert<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_rb, ns, c->cgrid, c->d_chead,
-                                                                 c->d_cnext);
+                                                                 c->d_cnext, gate);
         for (int it = 0; it < n_sub; ++it) {
             int* H[3] = {c->d_chead + (it % 3) * nc, c->d_chead + ((it + 1) % 3) * nc, c->d_chead + ((it + 2) % 3) * nc};
             int* Nr = c->d_cnext + (long long)(it & 1) * c->contact_cap;
             int* Nw = c->d_cnext + (long long)((it + 1) & 1) * c->contact_cap;
             k_contact_step<<<grid_for(ns, 64), 64, 0, c->stream>>>(
                 buf[it & 1], buf[(it + 1) & 1], c->d_dyn, c->d_ft, c->d_rb, ns, c->cgrid, H[0], Nr, H[1], Nw, H[2],
-                (int)nc, c->d_planes, nplanes, c->law, dts, g, c->rho_f, it == n_sub - 1);
+                (int)nc, c->d_planes, nplanes, c->law, dts, g, c->rho_f, it == n_sub - 1, gate);
         }
     } else {
-        k_evolve<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_dyn, c->d_ft, ns, n_sub, dt, g, c->rho_f);
+        k_evolve<<<grid_for(ns, 64), 64, 0, c->stream>>>(c->d_solids, c->d_dyn, c->d_ft, ns, n_sub, dt, g, c->rho_f,
+                                                         gate);
     }
+}
+
+// Staged AoS FP64 velocities of cells [first, first + n) into the resident SoA U.
+void launch_aos_to_soa(sdfibm_ctx* c, const double* staged, long long first, long long n) {
+    if (n <= 0) return;
+    if (c->prec == SDFIBM_FP64)
+        k_aos_to_soa<double><<<grid_for(n, 256), 256, 0, c->stream>>>(
+            staged, n, (double*)c->U[0] + first, (double*)c->U[1] + first, (double*)c->U[2] + first);
+    else
+        k_aos_to_soa<float><<<grid_for(n, 256), 256, 0, c->stream>>>(
+            staged, n, (float*)c->U[0] + first, (float*)c->U[1] + first, (float*)c->U[2] + first);
 }
 
 std::string err_text(int flags) {
     std::string s;
     if (flags & EF_TILE) s += "a solid's cell tile exceeds 63 cells along x or 4096 vertex rows; ";
-    if (flags & EF_CODECAP) s += "tile code buffer too small; ";
+    if (flags & EF_CODECAP) s += "tile code buffer too small (or more than 4096 big tiles); ";
     if (flags & EF_BRICK) s += "more than 128 solids overlap one brick; ";
     if (flags & EF_PLANE_MOVED) s += "a plane solid moved during a graph replay; ";
+    if (flags & EF_LABEL) s += "the big-tile component labelling did not converge; ";
     return s;
 }
 
@@ -2392,7 +2516,7 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if (same) {
         for (int64_t s = 0; s < n; ++s) {
             std::string why;
-            const int v = validate_solid(solids[s], s, why);
+            const int v = validate_solid(solids[s], s, why, (long long)c->h_csg.size());
             if (v) return fail(c, v, why);
         }
         HIPCHK(c, hipMemsetAsync(c->d_ft, 0, n * 48, c->stream));  // evolve before any interact: no fluid force
@@ -2401,14 +2525,70 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         c->have_step = false;
         return SDFIBM_OK;
     }
-    graph_reset(c);
-    std::vector<int> planes;
-    long long cells = 0, tile_sum = 0, n_tiles = 0;
     for (int64_t s = 0; s < n; ++s) {
         std::string why;
-        const int v = validate_solid(solids[s], s, why);
+        const int v = validate_solid(solids[s], s, why, (long long)c->h_csg.size());
         if (v) return fail(c, v, why);
-        const SolidK K = make_solid_k(solids[s]);
+    }
+    return classify_solids(c, solids, n);
+}
+
+namespace {
+
+// Finest cell spacing per axis: the rectilinear coordinates' smallest step; a
+// structured block's nominal spacing less twice the largest point displacement.
+void min_spacing(const sdfibm_ctx* c, double* h) {
+    if (c->kind == SDFIBM_MESH_RECTILINEAR) {
+        const std::vector<double>* v[3] = {&c->h_xs, &c->h_ys, &c->h_zs};
+        for (int d = 0; d < 3; ++d) {
+            h[d] = __builtin_huge_val();
+            for (size_t t = 1; t < v[d]->size(); ++t) h[d] = std::min(h[d], (*v[d])[t] - (*v[d])[t - 1]);
+        }
+    } else {
+        for (int d = 0; d < 3; ++d) h[d] = c->hm.h[d] * std::max(0.05, 1.0 - 2.0 * c->hm.jit);
+    }
+}
+
+// Buffers of the big-tile pass: metadata once, the vertex bits and chunk partials
+// grown to at least the given sizes (with headroom).
+int grow_big(sdfibm_ctx* c, long long words, long long chunks) {
+    int rc;
+    if (!c->d_big_meta && (rc = dalloc(c, (void**)&c->d_big_meta, kBigMaxSolids * sizeof(BigMeta)))) return rc;
+    if (!c->d_big_tot && (rc = dalloc(c, (void**)&c->d_big_tot, kBigTotN * 8))) return rc;
+    if (words > c->big_word_cap) {
+        graph_reset(c);
+        dfree(c, c->d_big_vbits);
+        c->big_word_cap = 0;
+        const long long want = words + words / 4 + 1024;
+        if ((rc = dalloc(c, (void**)&c->d_big_vbits, want * 8))) return rc;
+        c->big_word_cap = want;
+    }
+    if (chunks > c->big_chunk_cap) {
+        graph_reset(c);
+        dfree(c, c->d_big_part);
+        c->big_chunk_cap = 0;
+        const long long want = chunks + chunks / 4 + 256;
+        if ((rc = dalloc(c, (void**)&c->d_big_part, want * 6 * 8))) return rc;
+        c->big_chunk_cap = want;
+    }
+    return SDFIBM_OK;
+}
+
+// The full set-up of a solid array: tile classes, which solids take the planes' path, buffer
+// sizes, uploads.  Used by set_solids for new solids and by interact when a solid's tile
+// outgrew the tile pass since set_solids (it then takes the planes' path, as it would in a
+// fresh context).  The solids are valid (validate_solid).  Until it succeeds the context holds
+// no solids: a failed allocation leaves no freed buffer reachable from a later call.
+int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
+    graph_reset(c);
+    std::vector<int> planes;
+    long long cells = 0, tile_sum = 0, n_tiles = 0, big_words = 0, big_chunks = 0;
+    bool big_on = false;
+    double hmin[3];
+    min_spacing(c, hmin);
+    const long long nn[3] = {c->nx, c->ny, c->nz};
+    for (int64_t s = 0; s < n; ++s) {
+        const SolidK K = make_solid_k(solids[s], c->h_csg.data());
         if (solids[s].shape == SDFIBM_SHAPE_PLANE) {
             planes.push_back((int)s);
             continue;
@@ -2418,9 +2598,27 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
             planes.push_back((int)s);
             continue;
         }
+        // can the solid's tile, in any pose, outgrow the on-chip tile pass?  (its bounding
+        // sphere in cells of the finest spacing)
+        {
+            d3 lc, lh;
+            long long e[3] = {nn[0], nn[1], nn[2]};
+            if (local_bounds(K, lc, lh)) {
+                const double R = std::sqrt(dot(lc, lc)) + std::sqrt(dot(lh, lh));
+                for (int d = 0; d < 3; ++d)
+                    if (std::isfinite(R)) e[d] = std::min<long long>(nn[d], (long long)std::ceil(2.0 * R / hmin[d]) + 2);
+            }
+            if (e[0] > kMaxTileX || (e[1] + 1) * (e[2] + 1) > kRowsLarge) {
+                big_on = true;
+                big_words += (e[0] + 2) * (e[1] + 2) * (e[2] + 2) / 64 + 2;
+                big_chunks += e[0] * e[1] * e[2] / kBigChunk + 2;
+            }
+        }
         if (t.cls == TC_EMPTY) continue;
-        tile_sum += (long long)t.n[0] * t.n[1] * t.n[2];
-        ++n_tiles;
+        if (t.cls != TC_BIG) {
+            tile_sum += (long long)t.n[0] * t.n[1] * t.n[2];
+            ++n_tiles;
+        }
         // allow one extra cell layer per axis for motion between set_solids calls
         const long long ex = std::min(t.n[0] + 1, c->nx), ey = std::min(t.n[1] + 1, c->ny),
                         ez = std::min(t.n[2] + 1, c->nz);
@@ -2429,12 +2627,21 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if ((int)planes.size() > kMaxPlanes)
         return fail(c, SDFIBM_ERR_ARG,
                     "at most " + std::to_string(kMaxPlanes) +
-                        " plane solids and solids whose cell tile exceeds 63 cells along x or 4096 vertex rows");
+                        " plane solids and solids whose cell tile has more than 2^22 vertices");
+    c->have_solids = false;
+    c->have_step = false;
+    c->have_ct = false;
+    c->ns = 0;
+    c->contact_ready = false;
     int rc;
     if (n > c->solid_cap) {
-        for (void* p : {(void*)c->d_solids, (void*)c->d_K, (void*)c->d_T, (void*)c->d_tile_cells,
-                        (void*)c->d_tile_off, (void*)c->d_ft})
-            dfree(c, p);
+        dfree(c, c->d_solids);
+        dfree(c, c->d_K);
+        dfree(c, c->d_T);
+        dfree(c, c->d_tile_cells);
+        dfree(c, c->d_tile_off);
+        dfree(c, c->d_ft);
+        c->solid_cap = 0;
         const long long cap = std::max<long long>(n, 16);
         if ((rc = dalloc(c, (void**)&c->d_solids, cap * sizeof(sdfibm_solid)))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_K, cap * sizeof(SolidK)))) return rc;
@@ -2445,7 +2652,7 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         dfree(c, c->d_ncand);
         if ((rc = dalloc(c, (void**)&c->d_ncand, cap * sizeof(int2)))) return rc;
         dfree(c, c->qlist);
-        if ((rc = dalloc(c, (void**)&c->qlist, cap * 4 * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->qlist, cap * kQueues * 4))) return rc;
         dfree(c, c->d_unb);
         if ((rc = dalloc(c, (void**)&c->d_unb, cap))) return rc;
         dfree(c, c->scan_tmp);
@@ -2456,8 +2663,6 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if (want_codes > c->code_cap) {
         dfree(c, c->d_codes);
         dfree(c, c->d_cand);
-        c->d_codes = nullptr;
-        c->d_cand = nullptr;
         c->code_cap = 0;
         if ((rc = dalloc(c, &c->d_codes, want_codes * rsize(c)))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_cand, want_codes * 4))) return rc;
@@ -2465,26 +2670,21 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     }
     dfree(c, c->d_planes);
     dfree(c, c->d_plane_partial);
-    c->d_planes = nullptr;
-    c->d_plane_partial = nullptr;
     if (!planes.empty()) {
         if ((rc = dalloc(c, (void**)&c->d_planes, planes.size() * 4))) return rc;
         HIPCHK(c, hipMemcpy(c->d_planes, planes.data(), planes.size() * 4, hipMemcpyHostToDevice));
         if ((rc = dalloc(c, (void**)&c->d_plane_partial, planes.size() * c->nbricks * 6 * 8))) return rc;
     }
-    if (!planes.empty() && !c->d_pact) {
+    if (!planes.empty()) {  // mesh-sized labelling buffers, kept once allocated
         c->member_words = (c->N + 31) / 32;
-        if ((rc = dalloc(c, (void**)&c->d_pact, c->member_words * 4))) return rc;
-        if ((rc = dalloc(c, (void**)&c->d_plist, c->N * 4))) return rc;
-        if ((rc = dalloc(c, (void**)&c->d_plabel, c->N * 4))) return rc;
-        if ((rc = dalloc(c, (void**)&c->d_pscalars, 64))) return rc;
+        if (!c->d_pact && (rc = dalloc(c, (void**)&c->d_pact, c->member_words * 4))) return rc;
+        if (!c->d_plist && (rc = dalloc(c, (void**)&c->d_plist, c->N * 4))) return rc;
+        if (!c->d_plabel && (rc = dalloc(c, (void**)&c->d_plabel, c->N * 4))) return rc;
+        if (!c->d_pscalars && (rc = dalloc(c, (void**)&c->d_pscalars, 64))) return rc;
     }
     dfree(c, c->d_pmember);
     dfree(c, c->d_pseed);
     dfree(c, c->d_pcode);
-    c->d_pmember = nullptr;
-    c->d_pseed = nullptr;
-    c->d_pcode = nullptr;
     if (!planes.empty()) {
         if ((rc = dalloc(c, (void**)&c->d_pmember, planes.size() * c->member_words * 4))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_pseed, planes.size() * 4))) return rc;
@@ -2495,6 +2695,8 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     }
     c->h_planes = planes;
     c->small_tiles = n_tiles > 0 && tile_sum < 512 * n_tiles;
+    c->big_on = big_on;
+    if (big_on && (rc = grow_big(c, big_words, big_chunks))) return rc;
     if (n > 0) {
         std::vector<unsigned char> unb(n, 0);
         for (int p : planes) unb[p] = 1;
@@ -2513,25 +2715,44 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     return SDFIBM_OK;
 }
 
+}  // namespace
+
+int sdfibm_set_csg_shapes(sdfibm_ctx* c, const sdfibm_csg_shape* shapes, int64_t n) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (n < 0 || (n > 0 && !shapes)) return fail(c, SDFIBM_ERR_ARG, "bad CSG shape array");
+    if (n > (1LL << 24)) return fail(c, SDFIBM_ERR_ARG, "too many CSG shapes");
+    for (int64_t t = 0; t < n; ++t) {
+        std::string why;
+        if (const int v = validate_csg(shapes[t], t, why)) return fail(c, v, why);
+    }
+    // the solids refer to the table by index: they must be set again
+    graph_reset(c);
+    c->have_solids = false;
+    c->have_step = false;
+    c->have_ct = false;
+    c->h_csg_raw.assign(shapes, shapes + n);
+    c->h_csg.resize(n);
+    for (int64_t t = 0; t < n; ++t) c->h_csg[t] = make_csg_k(shapes[t]);
+    dfree(c, c->d_csg);
+    int rc;
+    if (n > 0) {
+        if ((rc = dalloc(c, (void**)&c->d_csg, n * sizeof(CsgK)))) return rc;
+        HIPCHK(c, hipMemcpy(c->d_csg, c->h_csg.data(), n * sizeof(CsgK), hipMemcpyHostToDevice));
+    }
+    return SDFIBM_OK;
+}
+
 int sdfibm_set_velocity(sdfibm_ctx* c, const double* U) {
     if (!c) return SDFIBM_ERR_ARG;
     if (!U) return fail(c, SDFIBM_ERR_ARG, "U is NULL");
-    void* tmp = nullptr;
     int rc;
-    if ((rc = dalloc(c, &tmp, c->N * 24))) return rc;
-    hipError_t e = hipMemcpyAsync(tmp, U, c->N * 24, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) {
-        if (c->prec == SDFIBM_FP64)
-            k_aos_to_soa<double><<<grid_for(c->N, 256), 256, 0, c->stream>>>(
-                (const double*)tmp, c->N, (double*)c->U[0], (double*)c->U[1], (double*)c->U[2]);
-        else
-            k_aos_to_soa<float><<<grid_for(c->N, 256), 256, 0, c->stream>>>(
-                (const double*)tmp, c->N, (float*)c->U[0], (float*)c->U[1], (float*)c->U[2]);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    dfree(c, tmp);
-    if (e != hipSuccess) return fail(c, SDFIBM_ERR_HIP, std::string("set_velocity: ") + hipGetErrorString(e));
+    // the AoS staging buffer is allocated by the first call and kept (a solver hands U over
+    // every step)
+    if (!c->d_ustage && (rc = dalloc(c, (void**)&c->d_ustage, c->N * 24))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_ustage, U, c->N * 24, hipMemcpyHostToDevice, c->stream));
+    launch_aos_to_soa(c, c->d_ustage, 0, c->N);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return SDFIBM_OK;
 }
 
@@ -2547,23 +2768,45 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int flags = *(volatile int*)c->h_flags;
-    if ((flags & EF_CODECAP) && !(flags & EF_TILE) && c->ns > 0) {
-        // the solids moved or turned since set_solids and their tiles outgrew the code
-        // buffer: grow it to the size this step needs (with headroom) and run the step again
+    if ((flags & EF_TILE) && c->ns > 0) {
+        // a solid moved or turned since set_solids until its tile no longer fits the tile pass:
+        // classify the current solids again (that solid takes the planes' path, as in a fresh
+        // context given these solids) and run the step again
+        std::vector<sdfibm_solid> cur(c->ns);
+        HIPCHK(c, hipMemcpy(cur.data(), c->d_solids, c->ns * sizeof(sdfibm_solid), hipMemcpyDeviceToHost));
+        if ((rc = classify_solids(c, cur.data(), (int64_t)cur.size()))) return rc;
+        if ((rc = dispatch_interact(c, dt))) return rc;
+        k_flags_out<<<1, 64, 0, c->stream>>>(c->err, c->d_hflags);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        flags = *(volatile int*)c->h_flags;
+    }
+    for (int attempt = 0; attempt < 3 && (flags & EF_CODECAP) && !(flags & EF_TILE) && c->ns > 0; ++attempt) {
+        // the solids moved or turned since set_solids and their tiles outgrew the code buffer
+        // (or the big-tile pass's buffers): grow them to the sizes this step needs (with
+        // headroom) and run the step again
         long long need = 0;
         HIPCHK(c, hipMemcpy(&need, c->d_tile_off + c->ns, 8, hipMemcpyDeviceToHost));
-        const long long want = need + need / 4 + 1024;
-        graph_reset(c);
-        dfree(c, c->d_codes);
-        dfree(c, c->d_cand);
-        c->d_codes = nullptr;
-        c->d_cand = nullptr;
-        c->code_cap = 0;
-        if ((rc = dalloc(c, &c->d_codes, want * rsize(c)))) return rc;
-        if ((rc = dalloc(c, (void**)&c->d_cand, want * 4))) return rc;
-        c->code_cap = want;
+        if (need > c->code_cap) {
+            const long long want = need + need / 4 + 1024;
+            graph_reset(c);
+            dfree(c, c->d_codes);
+            dfree(c, c->d_cand);
+            c->code_cap = 0;
+            if ((rc = dalloc(c, &c->d_codes, want * rsize(c)))) return rc;
+            if ((rc = dalloc(c, (void**)&c->d_cand, want * 4))) return rc;
+            c->code_cap = want;
+        }
+        if (c->big_on) {
+            long long bt[kBigTotN];
+            HIPCHK(c, hipMemcpy(bt, c->d_big_tot, sizeof(bt), hipMemcpyDeviceToHost));
+            if ((rc = grow_big(c, bt[5], bt[6]))) return rc;
+        }
         if ((rc = dispatch_interact(c, dt))) return rc;
-        HIPCHK(c, hipMemcpyAsync(&flags, c->err, 4, hipMemcpyDeviceToHost, c->stream));
+        k_flags_out<<<1, 64, 0, c->stream>>>(c->err, c->d_hflags);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        flags = *(volatile int*)c->h_flags;
     }
     if (ft && c->ns > 0) HIPCHK(c, hipMemcpyAsync(ft, c->d_ft, c->ns * 48, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2622,11 +2865,18 @@ int sdfibm_set_dynamics(sdfibm_ctx* c, const sdfibm_dynamics* dyn, int64_t n) {
         h[s].mask = dyn[s].motion_mask;
         h[s].first_fluid = 1;
         for (int a = 0; a < 3; ++a) h[s].ff_old[a] = h[s].ft_old[a] = 0.0;
+        h[s].vol = 0.0;
+        h[s].moi[0] = h[s].moi[1] = h[s].moi[2] = 1.0;
+        if (c->h_solids[s].shape == SDFIBM_SHAPE_CSG) {  // validated index (set_solids)
+            const sdfibm_csg_shape& g = c->h_csg_raw[(size_t)c->h_solids[s].prm[0]];
+            h[s].vol = g.volume;
+            for (int a = 0; a < 3; ++a) h[s].moi[a] = g.moi[a];
+        }
     }
     int rc;
+    c->dyn_n = -1;  // no dynamics until the records below are on the device
     if (c->d_dyn == nullptr || c->dyn_cap < n) {
         dfree(c, c->d_dyn);
-        c->d_dyn = nullptr;
         c->dyn_cap = 0;
         if ((rc = dalloc(c, (void**)&c->d_dyn, std::max<long long>(n, 1) * sizeof(SolidDyn)))) return rc;
         c->dyn_cap = n;
@@ -2642,11 +2892,13 @@ bool contact_on(const sdfibm_ctx* c) { return c->law.k > 0 || c->law.c > 0; }
 
 // Bounding radii, the contact grid and its buffers for the current solids.
 int contact_prepare(sdfibm_ctx* c) {
+    c->contact_ready = false;  // until the buffers below exist: evolve and run_steps refuse
     const long long n = c->ns;
     std::vector<double> rb(std::max<long long>(n, 1), 0.0);
     double rmax = 0.0;
     for (long long s = 0; s < n; ++s) {
-        rb[s] = bounding_radius(c->h_solids[s]);
+        rb[s] = c->h_solids[s].shape == SDFIBM_SHAPE_CSG ? c->h_csg_raw[(size_t)c->h_solids[s].prm[0]].radius_b
+                                                         : bounding_radius(c->h_solids[s]);
         rmax = std::max(rmax, rb[s]);
     }
     double cell = 2.0 * rmax, ext[3];
@@ -2667,10 +2919,9 @@ int contact_prepare(sdfibm_ctx* c) {
     g.inv_cell = 1.0 / cell;
     int rc;
     if (n > c->contact_cap) {
-        for (void* p : {(void*)c->d_rb, (void*)c->d_solids_alt, (void*)c->d_cnext}) dfree(c, p);
-        c->d_rb = nullptr;
-        c->d_solids_alt = nullptr;
-        c->d_cnext = nullptr;
+        dfree(c, c->d_rb);
+        dfree(c, c->d_solids_alt);
+        dfree(c, c->d_cnext);
         c->contact_cap = 0;
         if ((rc = dalloc(c, (void**)&c->d_rb, n * 8))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_solids_alt, n * sizeof(sdfibm_solid)))) return rc;
@@ -2679,7 +2930,6 @@ int contact_prepare(sdfibm_ctx* c) {
     }
     if (total > c->chead_cap) {
         dfree(c, c->d_chead);
-        c->d_chead = nullptr;
         c->chead_cap = 0;
         if ((rc = dalloc(c, (void**)&c->d_chead, 3 * total * 4))) return rc;
         c->chead_cap = total;
@@ -2687,6 +2937,18 @@ int contact_prepare(sdfibm_ctx* c) {
     if (n > 0) HIPCHK(c, hipMemcpy(c->d_rb, rb.data(), n * 8, hipMemcpyHostToDevice));
     c->cgrid = g;
     c->ncells_c = (int)total;
+    c->contact_ready = true;
+    return SDFIBM_OK;
+}
+
+// evolve and run_steps: every buffer their launches read exists for the current solids.
+int check_dynamics_ready(sdfibm_ctx* c, const char* who) {
+    if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, std::string(who) + " before set_solids");
+    if (c->dyn_n != c->ns || c->dyn_n < 0)
+        return fail(c, SDFIBM_ERR_STATE, std::string(who) + " needs set_dynamics for these solids");
+    if (contact_on(c) && !c->contact_ready)
+        return fail(c, SDFIBM_ERR_STATE,
+                    std::string(who) + ": the contact buffers are missing (an allocation failed); call set_contact again");
     return SDFIBM_OK;
 }
 
@@ -2705,13 +2967,13 @@ int sdfibm_set_contact(sdfibm_ctx* c, double stiffness, double damping) {
 int sdfibm_evolve(sdfibm_ctx* c, double time, double dt, const double* gravity) {
     (void)time;
     if (!c) return SDFIBM_ERR_ARG;
-    if (c->dyn_n != c->ns || c->dyn_n < 0) return fail(c, SDFIBM_ERR_STATE, "evolve needs set_dynamics for these solids");
+    if (int rc = check_dynamics_ready(c, "evolve")) return rc;
     if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
     Vec3 g{{0.0, 0.0, 0.0}};
     if (gravity)
         for (int a = 0; a < 3; ++a) g.v[a] = gravity[a];
     if (c->ns == 0) return SDFIBM_OK;
-    launch_evolve(c, dt, g);
+    launch_evolve(c, dt, g, nullptr);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_step = false;  // the tiles of the last interact no longer match the solids
@@ -2720,9 +2982,7 @@ int sdfibm_evolve(sdfibm_ctx* c, double time, double dt, const double* gravity) 
 
 int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, const double* gravity, double* ft) {
     if (!c) return SDFIBM_ERR_ARG;
-    if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "run_steps before set_solids");
-    if (c->dyn_n != c->ns || c->dyn_n < 0)
-        return fail(c, SDFIBM_ERR_STATE, "run_steps needs set_dynamics for these solids");
+    if (int rc = check_dynamics_ready(c, "run_steps")) return rc;
     if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
     if (n_steps < 0) return fail(c, SDFIBM_ERR_ARG, "n_steps must be >= 0");
     Vec3 g{{0.0, 0.0, 0.0}};
@@ -2731,6 +2991,10 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
     if (!std::isfinite(g.v[0]) || !std::isfinite(g.v[1]) || !std::isfinite(g.v[2]))
         return fail(c, SDFIBM_ERR_ARG, "gravity must be finite");
     const int ns = (int)c->ns;
+    int rc;
+    // an eager pass over the current state: plane memberships and the code buffer size (it
+    // also moves a solid whose tile outgrew the tile pass to the planes' path)
+    if (n_steps > 0 && (rc = sdfibm_interact(c, time, dt, nullptr))) return rc;
     const int np = (int)c->h_planes.size();
     // the plane memberships are computed once, before the replays: the planes must stay put
     std::vector<double> pose(7 * std::max(np, 1));
@@ -2750,9 +3014,6 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
         for (int a = 0; a < 4; ++a) pose[7 * p + 3 + a] = ps.q[a];
     }
     if (n_steps == 0) return SDFIBM_OK;
-    // an eager pass over the current state: plane memberships and the code buffer size
-    int rc = sdfibm_interact(c, time, dt, nullptr);
-    if (rc) return rc;
     long long need = 0;
     if (ns > 0) HIPCHK(c, hipMemcpy(&need, c->d_tile_off + ns, 8, hipMemcpyDeviceToHost));
     const long long want = need + need / 2 + 4096;  // headroom: tiles grow as the solids move and turn
@@ -2760,12 +3021,15 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
         graph_reset(c);
         dfree(c, c->d_codes);
         dfree(c, c->d_cand);
-        c->d_codes = nullptr;
-        c->d_cand = nullptr;
         c->code_cap = 0;
         if ((rc = dalloc(c, &c->d_codes, want * rsize(c)))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_cand, want * 4))) return rc;
         c->code_cap = want;
+    }
+    if (c->big_on) {  // the same headroom for the big-tile pass
+        long long bt[kBigTotN];
+        HIPCHK(c, hipMemcpy(bt, c->d_big_tot, sizeof(bt), hipMemcpyDeviceToHost));
+        if ((rc = grow_big(c, bt[5] + bt[5] / 2 + 64, bt[6] + bt[6] / 2 + 16))) return rc;
     }
     if (!c->d_sticky && (rc = dalloc(c, (void**)&c->d_sticky, 64))) return rc;
     if (!c->d_plane_pose && (rc = dalloc(c, (void**)&c->d_plane_pose, 7 * kMaxPlanes * 8))) return rc;
@@ -2777,9 +3041,11 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int r = dispatch_interact(c, dt, true);
         if (r == SDFIBM_OK) {
-            launch_evolve(c, dt, g);
-            launch_update_fix(c, dt);  // U -= Fs dt, then fixInternal with the evolved solids
+            // a step whose interact failed, and every later one, leaves the solids and U alone:
+            // the state stays the last valid step's, and the caller can grow buffers and go on
             k_flag_or<<<1, 64, 0, c->stream>>>(c->err, c->d_sticky);
+            launch_evolve(c, dt, g, c->d_sticky);
+            launch_update_fix(c, dt, c->d_sticky);  // U -= Fs dt, then fixInternal with the evolved solids
             if (np > 0)
                 k_plane_static<<<1, 64, 0, c->stream>>>(c->d_solids, c->d_planes, np, c->d_plane_pose, c->d_sticky);
         }
@@ -2819,7 +3085,7 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
                     (flags & EF_TILE)          ? SDFIBM_ERR_TILE
                     : (flags & EF_PLANE_MOVED) ? SDFIBM_ERR_STATE
                                                : SDFIBM_ERR_CAPACITY,
-                    err_text(flags) + "the state after the first failing step is not valid");
+                    err_text(flags) + "the solids and U are those of the last valid step");
     return SDFIBM_OK;
 }
 
@@ -2882,6 +3148,14 @@ int sdfibm_get_velocity(sdfibm_ctx* c, double* U) {
             U[3 * q + a] = rs == 8 ? ((const double*)buf.data())[q] : (double)((const float*)buf.data())[q];
     }
     return SDFIBM_OK;
+}
+
+int sdfibm_buffer_capacity(sdfibm_ctx* c, int64_t* out, int n) {
+    if (!c || !out || n < 0) return SDFIBM_ERR_ARG;
+    const long long v[2] = {c->code_cap, c->solid_cap};
+    const int k = n < 2 ? n : 2;
+    for (int t = 0; t < k; ++t) out[t] = v[t];
+    return k;
 }
 
 int sdfibm_last_timing(sdfibm_ctx* c, double* ms, int n) {

@@ -234,12 +234,19 @@ __device__ inline R cell_fraction(const MeshDev& m, const SolidK& K, int i, int 
 // ---------------------------------------------------------------------------
 // per-solid tiles
 // ---------------------------------------------------------------------------
-// TC_HUGE: a finite solid whose tile is too big for the tile pass (> kMaxTileX
-// cells along x or > kRowsLarge vertex rows); it goes the planes' way (cells
-// over the mesh, union-find component of the seed) with its tile box as the cull.
-enum TileClass { TC_EMPTY = 0, TC_SMALL = 1, TC_LARGE = 2, TC_PLANE = 3, TC_TOOBIG = 4, TC_MEDIUM = 5, TC_HUGE = 6 };
+// TC_BIG: a tile too big for the on-chip tile pass (> kMaxTileX cells along x or
+// > kRowsLarge vertex rows) but with fewer than kBigVerts vertices: the big-tile
+// pass (global-memory labelling, chunked cut cells) with the same outputs.
+// TC_TOOBIG: a tile beyond that; set_solids sends such a solid the planes' way
+// (TC_HUGE: cells over the mesh, union-find component of the seed, its tile box
+// as the cull).
+enum TileClass {
+    TC_EMPTY = 0, TC_SMALL = 1, TC_LARGE = 2, TC_PLANE = 3, TC_TOOBIG = 4, TC_MEDIUM = 5, TC_HUGE = 6, TC_BIG = 7
+};
 
-__host__ __device__ inline bool tile_is_finite(int cls) { return cls == TC_SMALL || cls == TC_MEDIUM || cls == TC_LARGE; }
+__host__ __device__ inline bool tile_is_finite(int cls) {
+    return cls == TC_SMALL || cls == TC_MEDIUM || cls == TC_LARGE || cls == TC_BIG;
+}
 
 struct TileInfo {
     int lo[3];
@@ -254,6 +261,7 @@ constexpr int kRowsTiny = 64;      // vertex rows of a tiny tile (classified by 
 constexpr int kRowsSmall = 400;    // vertex rows (ny+1)(nz+1) of a small tile
 constexpr int kRowsMedium = 1024;  // ... of a medium tile
 constexpr int kRowsLarge = 4096;   // ... of a large tile
+constexpr long long kBigVerts = 1LL << 22;  // vertices of a big tile (the big pass's 22-bit index arithmetic)
 
 // Searches in an increasing array a[0..n): start from a linear-interpolation
 // guess and step to the exact answer, which takes one or two probes on a
@@ -426,7 +434,7 @@ __host__ __device__ inline TileInfo make_tile(const MeshDev& m, const SolidK& K)
     }
     const long long vrows = (long long)(T.n[1] + 1) * (T.n[2] + 1);
     if (T.n[0] > kMaxTileX || vrows > kRowsLarge)
-        T.cls = TC_TOOBIG;
+        T.cls = (vrows * (T.n[0] + 1) < kBigVerts && T.n[0] < 4095 && T.n[1] < 4095) ? TC_BIG : TC_TOOBIG;
     else
         T.cls = vrows > kRowsMedium ? TC_LARGE : (vrows > kRowsSmall ? TC_MEDIUM : TC_SMALL);
     return T;

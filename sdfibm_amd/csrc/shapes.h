@@ -32,11 +32,61 @@ __host__ __device__ inline d3 cross(d3 a, d3 b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// Derived constants of a CSG shape (sdfibm_csg_shape), computed once per
+// set_csg_shapes: per primitive the kind, the operation combining it with the
+// primitives before it, the cylinder axis, the centre in the solid's frame, the
+// half extents and the squares the strict predicates compare against.
+struct CsgPrimK {
+    int kind, op, axis, pad;
+    double c[3];
+    double h[3];  // half extents: box half sides, ellipsoid semi-axes, sphere (r, r, r), cylinder (r, half length, r)
+    double k[3];  // ellipsoid 1/(a*a), 1/(b*b), 1/(c*c); sphere and cylinder r*r in k[0]
+};
+struct CsgK {
+    int n, pad;
+    CsgPrimK p[SDFIBM_CSG_MAX_PRIMS];
+};
+
+__host__ inline CsgK make_csg_k(const sdfibm_csg_shape& g) {
+    CsgK K{};
+    K.n = g.n_prims;
+    for (int t = 0; t < g.n_prims && t < SDFIBM_CSG_MAX_PRIMS; ++t) {
+        const sdfibm_csg_prim& a = g.prim[t];
+        CsgPrimK& b = K.p[t];
+        b.kind = a.kind;
+        b.op = a.op;
+        b.axis = a.axis;
+        for (int d = 0; d < 3; ++d) b.c[d] = a.center[d];
+        switch (a.kind) {
+        case SDFIBM_CSG_BOX:
+            for (int d = 0; d < 3; ++d) b.h[d] = a.size[d];
+            break;
+        case SDFIBM_CSG_ELLIPSOID:  // the ellipsoid constructor's inverse squares (ellipsoid.h:18-20)
+            for (int d = 0; d < 3; ++d) {
+                b.h[d] = a.size[d];
+                b.k[d] = 1.0 / (a.size[d] * a.size[d]);
+            }
+            break;
+        case SDFIBM_CSG_SPHERE:
+            b.h[0] = b.h[1] = b.h[2] = a.size[0];
+            b.k[0] = a.size[0] * a.size[0];
+            break;
+        case SDFIBM_CSG_CYLINDER:
+            for (int d = 0; d < 3; ++d) b.h[d] = d == a.axis ? a.size[1] : a.size[0];
+            b.k[0] = a.size[0] * a.size[0];
+            break;
+        default: break;
+        }
+    }
+    return K;
+}
+
 // Per-solid constants derived once per step from the solid record, with the
 // same expressions the reference's shape constructors use (e.g. 1.0/(a*a)).
 struct SolidK {
     int shape;
     int pad;
+    const CsgK* csg;  // CSG solids: their shape's constants (device or host table, as make_solid_k was given)
     double c[3];    // centre
     double q[4];    // (w, x, y, z)
     double com[3];  // shape com offset (zero where the shape has none)
@@ -46,9 +96,12 @@ struct SolidK {
     double rt[9];   // R(q)^T row-major: local = rt (p - c), for the signed distance
 };
 
-__host__ __device__ inline SolidK make_solid_k(const sdfibm_solid& s) {
+// csg_table: the CSG shape constants (device or host copy, whichever side
+// evaluates K); a CSG solid's prm[0] is its index there.
+__host__ __device__ inline SolidK make_solid_k(const sdfibm_solid& s, const CsgK* csg_table = nullptr) {
     SolidK K{};
     K.shape = s.shape;
+    K.csg = (s.shape == SDFIBM_SHAPE_CSG && csg_table) ? csg_table + (long long)s.prm[0] : nullptr;
     for (int d = 0; d < 3; ++d) {
         K.c[d] = s.center[d];
         K.v[d] = s.vel[d];
@@ -136,6 +189,32 @@ __host__ __device__ inline bool rect_bool(double x, double y, double ra, double 
     return fabs(x) < ra && fabs(y) < rb;
 }
 
+// CSG: the strict predicates of the primitives combined with the reference's
+// boolean operators (sdf.h:133-142: U = or, D(a, b) = a and not b, I = and),
+// left to right.
+__host__ __device__ inline bool csg_prim_inside(const CsgPrimK& g, d3 p) {
+    const double x = p.x - g.c[0], y = p.y - g.c[1], z = p.z - g.c[2];
+    switch (g.kind) {
+    case SDFIBM_CSG_BOX: return fabs(x) < g.h[0] && fabs(y) < g.h[1] && fabs(z) < g.h[2];
+    case SDFIBM_CSG_ELLIPSOID: return x * x * g.k[0] + y * y * g.k[1] + z * z * g.k[2] < 1.0;
+    case SDFIBM_CSG_SPHERE: return x * x + y * y + z * z < g.k[0];
+    case SDFIBM_CSG_CYLINDER: {
+        const double u = g.axis == 0 ? y : x, v = g.axis == 2 ? y : z, w = g.axis == 0 ? x : (g.axis == 1 ? y : z);
+        return u * u + v * v < g.k[0] && fabs(w) < g.h[g.axis];
+    }
+    default: return false;
+    }
+}
+
+__host__ __device__ inline bool csg_inside(const CsgK& G, d3 p) {
+    bool in = csg_prim_inside(G.p[0], p);
+    for (int t = 1; t < G.n; ++t) {
+        const bool b = csg_prim_inside(G.p[t], p);
+        in = G.p[t].op == SDFIBM_CSG_UNION ? (in || b) : (G.p[t].op == SDFIBM_CSG_DIFFERENCE ? (in && !b) : (in && b));
+    }
+    return in;
+}
+
 // Strict inside test in local coordinates (each shape's isInside).
 __host__ __device__ inline bool inside_local(const SolidK& K, d3 p) {
     switch (K.shape) {
@@ -178,6 +257,7 @@ __host__ __device__ inline bool inside_local(const SolidK& K, d3 p) {
         const bool d2 = rect_bool(r2x - K.k[2], r2y - 0.0, K.k[2], K.k[3]);
         return dc || d1 || d2;
     }
+    case SDFIBM_SHAPE_CSG: return csg_inside(*K.csg, p);
     default: return false;
     }
 }
@@ -197,6 +277,44 @@ __host__ __device__ inline R rect_sd(R x, R y, R ra, R rb) {
     const R dxp = fmax(R(0), dx);
     const R dyp = fmax(R(0), dy);
     return sqrt(dxp * dxp + dyp * dyp) + fmin(R(0), fmax(dx, dy));
+}
+
+// CSG signed distance in R before the filter: the primitives' distances (box
+// and ellipsoid: sdf.h:43-57, 72-83; sphere: |x| - r; finite cylinder: the
+// rectangle formula of sdf.h:29-37 in (radius, axial) coordinates) combined
+// with the reference's operators (sdf.h:133-142: U = min, D(a, b) = max(a, -b),
+// I = max), left to right.
+template <class R>
+__host__ __device__ inline R csg_prim_sd(const CsgPrimK& g, R px, R py, R pz) {
+    const R x = px - R(g.c[0]), y = py - R(g.c[1]), z = pz - R(g.c[2]);
+    switch (g.kind) {
+    case SDFIBM_CSG_BOX: {
+        const R dx = fabs(x) - R(g.h[0]), dy = fabs(y) - R(g.h[1]), dz = fabs(z) - R(g.h[2]);
+        const R dxp = fmax(R(0), dx), dyp = fmax(R(0), dy), dzp = fmax(R(0), dz);
+        return sqrt(dxp * dxp + dyp * dyp + dzp * dzp) + fmin(R(0), fmax(dz, fmax(dx, dy)));
+    }
+    case SDFIBM_CSG_ELLIPSOID: {
+        const R ai = R(g.k[0]), bi = R(g.k[1]), ci = R(g.k[2]);
+        const R xa = x * x * ai, yb = y * y * bi, zc = z * z * ci;
+        return R(0.5) * (xa + yb + zc - R(1)) / sqrt(xa * ai + yb * bi + zc * ci);
+    }
+    case SDFIBM_CSG_SPHERE: return sqrt(x * x + y * y + z * z) - R(g.h[0]);
+    case SDFIBM_CSG_CYLINDER: {
+        const R u = g.axis == 0 ? y : x, v = g.axis == 2 ? y : z, w = g.axis == 0 ? x : (g.axis == 1 ? y : z);
+        return rect_sd<R>(sqrt(u * u + v * v), w, R(g.h[0 == g.axis ? 1 : 0]), R(g.h[g.axis]));
+    }
+    default: return R(1);
+    }
+}
+
+template <class R>
+__host__ __device__ inline R csg_sd(const CsgK& G, R px, R py, R pz) {
+    R d = csg_prim_sd<R>(G.p[0], px, py, pz);
+    for (int t = 1; t < G.n; ++t) {
+        const R e = csg_prim_sd<R>(G.p[t], px, py, pz);
+        d = G.p[t].op == SDFIBM_CSG_UNION ? fmin(d, e) : (G.p[t].op == SDFIBM_CSG_DIFFERENCE ? fmax(d, -e) : fmax(d, e));
+    }
+    return d;
 }
 
 // signedDistance of each shape, local coordinates given in R.
@@ -250,6 +368,7 @@ __host__ __device__ inline R sdf_local(const SolidK& K, R px, R py, R pz) {
         const R d2 = rect_sd<R>(r2x - R(K.k[2]), r2y, R(K.k[2]), R(K.k[3]));
         return filter_sd<R>(fmin(fmin(dc, d1), d2));
     }
+    case SDFIBM_SHAPE_CSG: return filter_sd<R>(csg_sd<R>(*K.csg, px, py, pz));
     default: return R(1);
     }
 }
@@ -317,6 +436,40 @@ __device__ inline float sdf_local_mixed(const SolidK& K, d3 p) {
         const float d2 = rect(r2x - K.k[2], r2y, K.k[2], K.k[3]);
         return filter_sd<float>(fminf(fminf(dc, d1), d2));
     }
+    case SDFIBM_SHAPE_CSG: {
+        const CsgK& G = *K.csg;
+        auto prim = [&](const CsgPrimK& g) -> float {
+            const double x = p.x - g.c[0], y = p.y - g.c[1], z = p.z - g.c[2];
+            switch (g.kind) {
+            case SDFIBM_CSG_BOX: {
+                const float dx = (float)(fabs(x) - g.h[0]), dy = (float)(fabs(y) - g.h[1]), dz = (float)(fabs(z) - g.h[2]);
+                const float dxp = fmaxf(0.f, dx), dyp = fmaxf(0.f, dy), dzp = fmaxf(0.f, dz);
+                return __builtin_amdgcn_sqrtf(dxp * dxp + dyp * dyp + dzp * dzp) + fminf(0.f, fmaxf(dz, fmaxf(dx, dy)));
+            }
+            case SDFIBM_CSG_ELLIPSOID: {
+                const double xa = x * x * g.k[0], yb = y * y * g.k[1], zc = z * z * g.k[2];
+                const float qm1 = (float)(xa + yb + zc - 1.0);
+                const float gg = (float)(xa * g.k[0] + yb * g.k[1] + zc * g.k[2]);
+                return 0.5f * qm1 * __builtin_amdgcn_rsqf(gg);
+            }
+            case SDFIBM_CSG_SPHERE: return sph(x * x + y * y + z * z, g.h[0], g.k[0]);
+            case SDFIBM_CSG_CYLINDER: {
+                const double u = g.axis == 0 ? y : x, v = g.axis == 2 ? y : z, w = g.axis == 0 ? x : (g.axis == 1 ? y : z);
+                const float dr = sph(u * u + v * v, g.h[g.axis == 0 ? 1 : 0], g.k[0]);
+                const float dw = (float)(fabs(w) - g.h[g.axis]);
+                const float drp = fmaxf(0.f, dr), dwp = fmaxf(0.f, dw);
+                return __builtin_amdgcn_sqrtf(drp * drp + dwp * dwp) + fminf(0.f, fmaxf(dr, dw));
+            }
+            default: return 1.f;
+            }
+        };
+        float d = prim(G.p[0]);
+        for (int t = 1; t < G.n; ++t) {
+            const float e = prim(G.p[t]);
+            d = G.p[t].op == SDFIBM_CSG_UNION ? fminf(d, e) : (G.p[t].op == SDFIBM_CSG_DIFFERENCE ? fmaxf(d, -e) : fmaxf(d, e));
+        }
+        return filter_sd<float>(d);
+    }
     default: return 1.f;
     }
 }
@@ -356,6 +509,32 @@ __host__ __device__ inline bool local_bounds(const SolidK& K, d3& lc, d3& lh) {
         const double R = fmax(r, sqrt(L * L + rb * rb));
         lc = mcom;
         lh = {R, R, INF};
+        return true;
+    }
+    case SDFIBM_SHAPE_CSG: {
+        // the box of the first primitive grown by the united ones and cut by the intersected ones
+        // (a difference removes inside points only)
+        const CsgK& G = *K.csg;
+        double lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = G.p[0].c[d] - G.p[0].h[d];
+            hi[d] = G.p[0].c[d] + G.p[0].h[d];
+        }
+        for (int t = 1; t < G.n; ++t)
+            for (int d = 0; d < 3; ++d) {
+                const double a = G.p[t].c[d] - G.p[t].h[d], b = G.p[t].c[d] + G.p[t].h[d];
+                if (G.p[t].op == SDFIBM_CSG_UNION) {
+                    lo[d] = fmin(lo[d], a);
+                    hi[d] = fmax(hi[d], b);
+                } else if (G.p[t].op == SDFIBM_CSG_INTERSECTION) {
+                    lo[d] = fmax(lo[d], a);
+                    hi[d] = fmin(hi[d], b);
+                }
+            }
+        for (int d = 0; d < 3; ++d)
+            if (!(hi[d] >= lo[d])) hi[d] = lo[d];  // empty intersection: a point, no vertex inside
+        lc = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+        lh = {0.5 * (hi[0] - lo[0]), 0.5 * (hi[1] - lo[1]), 0.5 * (hi[2] - lo[2])};
         return true;
     }
     default: return false;

@@ -36,6 +36,7 @@ class Scene:
     rho_f: float
     dt: float
     info: dict
+    csg: np.ndarray | None = None  # CSG shape table of the scene's CSG solids
 
 
 def _rng(seed):
@@ -62,9 +63,11 @@ def fourier_velocity(points: np.ndarray, seed: int, n_modes: int = 16, kmax: int
     return U / rms
 
 
-def place_nonoverlapping(rng, radius_b, lo, hi, gap: float = 0.0, max_tries: int = 4000) -> np.ndarray:
+def place_nonoverlapping(rng, radius_b, lo, hi, gap: float = 0.0, max_tries: int = 4000,
+                         avoid=None) -> np.ndarray:
     """Random sequential placement of bounding spheres inside the box lo..hi, largest first,
-    rejecting overlaps (plus `gap`) through a uniform-grid hash of the placed centres."""
+    rejecting overlaps (plus `gap`) through a uniform-grid hash of the placed centres, and
+    overlaps with the spheres `avoid` = (centres [m, 3], radii [m]) when given."""
     n_solids = radius_b.size
     lo = np.asarray(lo, dtype=np.float64)
     hi = np.asarray(hi, dtype=np.float64)
@@ -81,7 +84,13 @@ def place_nonoverlapping(rng, radius_b, lo, hi, gap: float = 0.0, max_tries: int
             cand = rng.uniform(lo + r, hi - r, size=(64, 3))
             gi = np.minimum(((cand - lo) / (hi - lo) * ng).astype(np.int64), ng - 1)
             good = np.ones(64, dtype=bool)
+            if avoid is not None:
+                ac, ar = avoid
+                d2 = np.sum((cand[:, None, :] - ac[None, :, :]) ** 2, axis=2)
+                good &= np.all(d2 >= (ar[None, :] + r + gap) ** 2, axis=1)
             for c in range(64):
+                if not good[c]:
+                    continue
                 key0 = gi[c]
                 for dz in (-1, 0, 1):
                     for dy in (-1, 0, 1):
@@ -151,6 +160,73 @@ def cloud_3d(n: int = 256, n_solids: int = 10000, frac_sphere: float = 0.7, d_me
     info = dict(n=n, n_solids=n_solids, seed=seed, solid_volume_fraction=vol,
                 n_spheres=int(is_sphere.sum()), d_cells=(float(d.min()), float(np.median(d)), float(d.max())))
     return Scene(f"cloud3d_{n}_{n_solids}", mesh, sol, U, False, 1.0, 1e-3, info)
+
+
+def csg_family(rng, kind: int, D: float) -> np.ndarray:
+    """One CSG shape of overall size D (world units): unions and differences of boxes,
+    cylinders, ellipsoids and spheres.  kind 0: box with a cylindrical through-hole; 1: ellipsoid
+    pierced by a rod (union); 2: cross of three bars (union); 3: cylinder with a slot cut out;
+    4: ellipsoid with an inner spherical cavity (difference); 5: cube rounded by a sphere
+    (intersection)."""
+    u = rng.uniform(0.8, 1.0, 3)
+    if kind == 0:
+        prims = [dict(kind="box", size=(0.5 * D, 0.38 * D * u[0], 0.3 * D * u[1])),
+                 dict(kind="cylinder", op="difference", size=(0.15 * D, 0.6 * D), axis=2, center=(0.1 * D, 0, 0))]
+    elif kind == 1:
+        prims = [dict(kind="ellipsoid", size=(0.32 * D, 0.3 * D * u[0], 0.28 * D * u[1])),
+                 dict(kind="cylinder", op="union", size=(0.09 * D, 0.5 * D), axis=0)]
+    elif kind == 2:
+        w = 0.12 * D * u[0]
+        prims = [dict(kind="box", size=(0.5 * D, w, w)), dict(kind="box", op="union", size=(w, 0.45 * D, w)),
+                 dict(kind="box", op="union", size=(w, w, 0.4 * D))]
+    elif kind == 3:
+        prims = [dict(kind="cylinder", size=(0.35 * D, 0.5 * D * u[0]), axis=1),
+                 dict(kind="box", op="difference", size=(0.16 * D, 0.6 * D, 0.08 * D), center=(0.3 * D, 0, 0))]
+    elif kind == 4:
+        prims = [dict(kind="ellipsoid", size=(0.5 * D, 0.4 * D * u[0], 0.36 * D * u[1])),
+                 dict(kind="sphere", op="difference", size=(0.18 * D,), center=(0.12 * D, 0, 0))]
+    else:
+        prims = [dict(kind="box", size=(0.4 * D * u[0], 0.4 * D * u[1], 0.4 * D * u[2])),
+                 dict(kind="sphere", op="intersection", size=(0.5 * D,))]
+    return S.csg_shape(prims, samples=64)
+
+
+def polydisperse_csg(n: int = 384, n_small: int = 100000, n_big: int = 50, seed: int = DEFAULT_SEED,
+                     d_small=(2.0, 4.0), d_big=(40.0, 80.0), with_velocity: bool = True) -> Scene:
+    """Small spheres (diameter uniform in d_small cells) and large CSG solids (overall size uniform
+    in d_big cells, the six csg_family kinds in turn) in the unit cube of n^3 cells, placed
+    without overlap of their bounding spheres (the large ones first, with a gap of 2 cells);
+    random orientations, v ~ N(0, 0.1), omega ~ N(0, 1).  The solids are the CSG ones first
+    (index order), then the spheres.  Returns the scene; its `csg` entry is the CSG table."""
+    rng = _rng(seed)
+    h = 1.0 / n
+    mesh = M.uniform_box((n, n, n), (0, 0, 0), (1, 1, 1))
+    Db = rng.uniform(d_big[0], d_big[1], n_big) * h
+    table = np.zeros(n_big, dtype=S.CSG_DTYPE)
+    for b in range(n_big):
+        table[b] = csg_family(rng, b % 6, Db[b])
+    rb_big = table["radius_b"].astype(np.float64)
+    pos_big = place_nonoverlapping(rng, rb_big, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 2.0 * h)
+    d = rng.uniform(d_small[0], d_small[1], n_small)
+    r_small = 0.5 * d * h
+    pos_small = place_nonoverlapping(rng, r_small, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0,
+                                     avoid=(pos_big, rb_big + 2.0 * h))
+    ns = n_big + n_small
+    sol = S.make_solids(ns)
+    sol["shape"][:n_big] = S.CSG
+    sol["prm"][:n_big, 0] = np.arange(n_big)
+    sol["center"][:n_big] = pos_big
+    sol["shape"][n_big:] = S.SPHERE
+    sol["prm"][n_big:, 0] = r_small
+    sol["center"][n_big:] = pos_small
+    sol["q"] = S.random_unit_quaternions(rng, ns)
+    sol["vel"] = rng.normal(0.0, 0.1, size=(ns, 3))
+    sol["omega"] = rng.normal(0.0, 1.0, size=(ns, 3))
+    U = fourier_velocity(mesh_cell_centres(mesh), seed + 1) if with_velocity else None
+    vol = float(table["volume"].sum() + (4.0 / 3.0 * math.pi * r_small ** 3).sum())
+    info = dict(n=n, n_small=n_small, n_big=n_big, seed=seed, solid_volume_fraction=vol,
+                d_big_cells=(float(Db.min() / h), float(Db.max() / h)))
+    return Scene(f"polydisperse_csg_{n}_{n_small}_{n_big}", mesh, sol, U, False, 1.0, 1e-3, info, table)
 
 
 def mesh_cell_centres(mesh: M.StructuredHexMesh) -> np.ndarray:
