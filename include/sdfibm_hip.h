@@ -160,6 +160,18 @@ int sdfibm_set_csg_shapes(sdfibm_ctx* ctx, const sdfibm_csg_shape* shapes, int64
  * converted to the resident SoA layout in the context's precision. */
 int sdfibm_set_velocity(sdfibm_ctx* ctx, const double* U_aos);
 
+/* Fluid velocity of cells [first, first + count), host AoS FP64 [count*3]: what
+ * sdfibm_set_velocity does for a range of cells, without allocating device
+ * memory.  The values pass through a fixed staging area of
+ * SDFIBM_NOALLOC_MAX_CELLS cells (24 MiB) that each GPU the library runs on holds
+ * until the process ends; a larger count is SDFIBM_ERR_ARG (upload in pieces).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (solver loops and serving paths that forbid hipMalloc).  All contexts
+ * on one GPU share the staging area: calls that overlap in time on one GPU (from
+ * several host threads) race; the library does not serialise them. */
+#define SDFIBM_NOALLOC_MAX_CELLS 1048576
+int sdfibm_set_velocity_cells_noalloc(sdfibm_ctx* ctx, int64_t first, int64_t count, const double* U_aos);
+
 /* SolidCloud::interact(time, dt) (solidcloud.cpp:435-464).  force_torque, if
  * not NULL, receives [n_solids * 6] FP64 (fx fy fz tx ty tz) per solid, the
  * values the reference stores with setFluidForceAndTorque (solid.h:136-140).

@@ -20,9 +20,11 @@ EXPORTS = (
     "sdfibm_set_dynamics", "sdfibm_evolve", "sdfibm_get_solids", "sdfibm_set_contact", "sdfibm_run_steps", "sdfibm_pair_counts",
     "sdfibm_surrogate_train", "sdfibm_surrogate_samples", "sdfibm_surrogate_eval", "sdfibm_surrogate_grad",
     "sdfibm_set_surrogate", "sdfibm_surrogate_count", "sdfibm_buffer_capacity", "sdfibm_set_csg_shapes",
+    "sdfibm_set_velocity_cells_noalloc",
 )
 
 SDFIBM_OK = 0
+NOALLOC_MAX_CELLS = 1048576  # SDFIBM_NOALLOC_MAX_CELLS
 STATUS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_SHAPE", -5: "ERR_TILE",
           -6: "ERR_STATE", -7: "ERR_MESH", -8: "ERR_CAPACITY", -9: "ERR_NODEVICE"}
 MESH_RECTILINEAR, MESH_STRUCTURED = 0, 1
@@ -95,6 +97,7 @@ def load():
         "sdfibm_surrogate_count": (i32, [P, P]),
         "sdfibm_buffer_capacity": (i32, [P, P, i32]),
         "sdfibm_set_csg_shapes": (i32, [P, P, i64]),
+        "sdfibm_set_velocity_cells_noalloc": (i32, [P, i64, i64, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -120,6 +120,13 @@ class SolidCloud:
         U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.n_cells, 3)
         _lib.check(_lib.load().sdfibm_set_velocity(self._h, _p(U)), self._h)
 
+    def set_velocity_cells_noalloc(self, first: int, U_part: np.ndarray):
+        """U of cells [first, first + len(U_part)) without any device allocation (at most
+        _lib.NOALLOC_MAX_CELLS cells per call)."""
+        U_part = np.ascontiguousarray(U_part, dtype=np.float64).reshape(-1, 3)
+        _lib.check(_lib.load().sdfibm_set_velocity_cells_noalloc(self._h, int(first), U_part.shape[0], _p(U_part)),
+                   self._h)
+
     # -- the coupling step -------------------------------------------------------------------
     def interact(self, time: float, dt: float) -> np.ndarray:
         ft = np.zeros((self.n_solids, 6))

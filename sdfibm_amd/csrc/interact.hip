@@ -38,6 +38,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/sdfibm_hip.h"
@@ -58,6 +59,14 @@ constexpr int kFillRep = SDFIBM_FILL_REP;  // row updates per flood-fill pass
 constexpr int kMaxPlanes = 32;  // plane and oversized solids (unbounded path)
 constexpr int kContactBlockSolids = 1024;   // evolve with contacts in one workgroup up to this many solids
 constexpr int kContactBlockCells = 1 << 16;  // ... and contact grid cells
+
+// Staging area of the allocation-free velocity upload
+// (sdfibm_set_velocity_cells_noalloc): kStageCells cells of FP64 AoS velocity,
+// 24 MiB, held by every GPU the library runs on until the process ends.  Host
+// code never passes this name to a kernel or copy: it uses the device address
+// that hipGetSymbolAddress gives for the current device (sdfibm_ctx::stage).
+constexpr long long kStageCells = SDFIBM_NOALLOC_MAX_CELLS;
+static __device__ __align__(256) double scratch[3 * kStageCells];
 
 enum ErrFlags { EF_TILE = 1, EF_CODECAP = 2, EF_BRICK = 8, EF_PLANE_MOVED = 16, EF_LABEL = 32 };
 
@@ -1732,6 +1741,7 @@ struct sdfibm_ctx {
     int* d_pseed = nullptr;
     long long member_words = 0;
     double* d_ustage = nullptr;  // set_velocity's AoS staging buffer, N * 3 doubles (first call)
+    double* stage = nullptr;     // this device's address of `scratch` (resolved by sdfibm_create)
     // CSG shapes (set_csg_shapes): the caller's records and their derived constants
     std::vector<sdfibm_csg_shape> h_csg_raw;
     std::vector<CsgK> h_csg;
@@ -2223,6 +2233,7 @@ int dispatch_interact(sdfibm_ctx* c, double dt, bool graph = false) {
 bool contact_on(const sdfibm_ctx* c);
 int contact_prepare(sdfibm_ctx* c);
 int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n);
+int resolve_stage(sdfibm_ctx* c);
 
 // Drop the captured step (its launches hold the buffers and settings of capture time).
 void graph_reset(sdfibm_ctx* c) {
@@ -2415,6 +2426,7 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
         return bail(SDFIBM_ERR_HIP);
     }
     c->own_stream = true;
+    if ((rc = resolve_stage(c))) return bail(rc);
     if (hipStreamCreate(&c->aux) != hipSuccess) {
         c->msg = "hipStreamCreate failed";
         return bail(SDFIBM_ERR_HIP);
@@ -2534,6 +2546,27 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
 }
 
 namespace {
+
+// The device address of `scratch` on the context's device, looked up once per
+// device (each GPU holds its own copy of the array) and kept for every context.
+int resolve_stage(sdfibm_ctx* c) {
+    static std::mutex mu;
+    static std::vector<double*> per_device;
+    std::lock_guard<std::mutex> lock(mu);
+    if (c->device < 0) return fail(c, SDFIBM_ERR_NODEVICE, "no current HIP device");
+    if ((int)per_device.size() <= c->device) per_device.resize(c->device + 1, nullptr);
+    if (!per_device[c->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(c, SDFIBM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        per_device[c->device] = (double*)p;
+    }
+    c->stage = per_device[c->device];
+    return SDFIBM_OK;
+}
 
 // Finest cell spacing per axis: the rectilinear coordinates' smallest step; a
 // structured block's nominal spacing less twice the largest point displacement.
@@ -2751,6 +2784,25 @@ int sdfibm_set_velocity(sdfibm_ctx* c, const double* U) {
     if (!c->d_ustage && (rc = dalloc(c, (void**)&c->d_ustage, c->N * 24))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_ustage, U, c->N * 24, hipMemcpyHostToDevice, c->stream));
     launch_aos_to_soa(c, c->d_ustage, 0, c->N);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SDFIBM_OK;
+}
+
+int sdfibm_set_velocity_cells_noalloc(sdfibm_ctx* c, int64_t first, int64_t count, const double* U) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (first < 0 || count < 0 || first > c->N - count)
+        return fail(c, SDFIBM_ERR_ARG, "cells [first, first + count) must lie in the mesh");
+    if (count > kStageCells)
+        return fail(c, SDFIBM_ERR_ARG,
+                    "count " + std::to_string(count) + " exceeds the staging area of " + std::to_string(kStageCells) +
+                        " cells: upload in pieces");
+    if (count > 0 && !U) return fail(c, SDFIBM_ERR_ARG, "U is NULL");
+    if (!c->stage) return fail(c, SDFIBM_ERR_STATE, "no staging area on this device");
+    if (count == 0) return SDFIBM_OK;
+    // the copy overwrites the `count` staged values the kernel reads: nothing to clear
+    HIPCHK(c, hipMemcpyAsync(c->stage, U, count * 24, hipMemcpyHostToDevice, c->stream));
+    launch_aos_to_soa(c, c->stage, first, count);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SDFIBM_OK;
