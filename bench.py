@@ -79,16 +79,29 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def cpu_baseline(scene, n_sample, budget_s):
-    """Reference-cost CPU oracle on the first solids in index order until the sample or the
-    time budget is used; returns cell-updates/s and the sample actually timed."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # the CPU restatement: timed here as the baseline, never shipped
+    import oracle  # the CPU restatement: the checker and the timed CPU baseline, never shipped
 
+    return oracle
+
+
+def oracle_mesh(scene):
+    """The oracle's restatement of the scene's mesh (and CSG table), built once per run."""
+    oracle = _oracle()
+    if getattr(scene, "csg", None) is not None:
+        oracle.set_csg(scene.csg)
     m = scene.mesh
     t0 = time.time()
     om = oracle.OracleMesh(structured=(m.nx, m.ny, m.nz, m.points))
     log(f"[cpu] oracle mesh built in {time.time() - t0:.1f}s")
+    return om
+
+
+def cpu_baseline(scene, n_sample, budget_s, om=None):
+    """Reference-cost CPU oracle on the first solids in index order until the sample or the
+    time budget is used; returns cell-updates/s and the sample actually timed."""
+    om = om or oracle_mesh(scene)
     U = scene.U
     fields = None
     done = 0
@@ -130,18 +143,14 @@ def copy_bandwidth(nbytes=2 << 30, reps=10):
     return 2.0 * nbytes / sec
 
 
-def cpu_baseline_fast(scene, per_solid_pairs, threads, budget_s):
+def cpu_baseline_fast(scene, per_solid_pairs, threads, budget_s, om=None):
     """The oracle with fast bookkeeping (stamped visited array, shared vertex cache), the solids
     split over `threads` host threads (the C++ runs outside the GIL), each with its own field
     arrays: an optimised CPU reference point beside the reference-cost one.  Times the first
     solids in index order, as many as fit the budget (estimated from a one-thread probe)."""
     import threading
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # the CPU restatement: timed here as a baseline, never shipped
-
-    m = scene.mesh
-    om = oracle.OracleMesh(structured=(m.nx, m.ny, m.nz, m.points))
+    om = om or oracle_mesh(scene)
     U = scene.U
     nsol = scene.solids.size
 
@@ -171,6 +180,101 @@ def cpu_baseline_fast(scene, per_solid_pairs, threads, budget_s):
     return pairs / el, n_take, pairs, el
 
 
+def cpu_baseline_slabs(scene, per_solid_pairs, ranks, n_sample, budget_s):
+    """The reference's own parallel mode, emulated on this host: the mesh decomposed into `ranks`
+    z-slabs (decomposePar's `simple` split along z), one thread per slab, each running the
+    reference-cost serial algorithm over every solid on its own sub-mesh -- a per-rank seed (the
+    nearest slab cell), the per-solid slab-sized visited array and the seed-fallback scan of the
+    whole slab for solids that do not reach it (src/cellenumerator.cpp:47-63) -- with the
+    per-solid force/torque partials summed at the end (src/solidcloud.cpp:427-431).  Times the
+    first solids in index order; the wall time is the slowest rank's."""
+    import threading
+
+    from sdfibm_amd import mesh as M
+
+    oracle = _oracle()
+    if getattr(scene, "csg", None) is not None:
+        oracle.set_csg(scene.csg)
+    m = scene.mesh
+    nx, ny, nz = m.nx, m.ny, m.nz
+    bounds = np.linspace(0, nz, ranks + 1).astype(int)
+    P = m.points.reshape(nz + 1, ny + 1, nx + 1, 3)
+    U3 = scene.U.reshape(nz, ny, nx, 3)
+    meshes, Us = [], []
+    for r in range(ranks):
+        k0, k1 = int(bounds[r]), int(bounds[r + 1])
+        if k1 <= k0:
+            continue
+        pts = np.ascontiguousarray(P[k0:k1 + 1].reshape(-1, 3))
+        meshes.append(oracle.OracleMesh(structured=(nx, ny, k1 - k0, pts)))
+        Us.append(np.ascontiguousarray(U3[k0:k1].reshape(-1, 3)))
+    nsol = scene.solids.size
+    # sample size from a probe of one rank (the most expensive: the slab with the most solids)
+    probe = min(nsol, 4)
+    fields = lambda om: dict(As=np.zeros(om.n_cells), Fs=np.zeros((om.n_cells, 3)), Ts=np.zeros(om.n_cells),
+                             Ct=np.zeros(om.n_cells), ft=np.zeros((nsol, 6)))
+    t = time.perf_counter()
+    meshes[0].interact_range(scene.solids, Us[0], 0, probe, rho_f=scene.rho_f, dt=scene.dt, mode=1,
+                             fields=fields(meshes[0]))
+    per_solid = (time.perf_counter() - t) / probe
+    n_take = int(min(n_sample, max(probe, budget_s / max(per_solid, 1e-9))))
+    fs = [fields(om) for om in meshes]
+    ths = [threading.Thread(target=om.interact_range, args=(scene.solids, Us[k], 0, n_take),
+                            kwargs=dict(rho_f=scene.rho_f, dt=scene.dt, mode=1, fields=fs[k]))
+           for k, om in enumerate(meshes)]
+    t = time.perf_counter()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    el = time.perf_counter() - t
+    ft = sum(f["ft"] for f in fs)  # the per-solid reductions across ranks
+    pairs = int(per_solid_pairs[:n_take].sum())
+    return pairs / el, n_take, pairs, el, len(meshes), ft
+
+
+def fp32_error_statement(cloud, scene, om, n_sample=16):
+    """Errors of the FP32 step against the FP64 oracle on the same full mesh, on sampled solids
+    (spread over the index range; the solids do not overlap, so each sampled solid's cells carry
+    only its own contributions): fractions (absolute; relative where alpha >= 1e-3), Fs per
+    element (relative where |Fs| >= 1e-3 of its largest value) and force/torque (relative to
+    max(|F|, 1e-3 max|F|))."""
+    ns = scene.solids.size
+    pick = np.unique(np.linspace(0, ns - 1, n_sample).astype(int))
+    ref = om.interact(scene.solids[pick], scene.U, rho_f=scene.rho_f, dt=scene.dt, capture=True,
+                      capacity=4000000)
+    ft = cloud.interact(0.0, scene.dt)[pick]
+    g = cloud.fields()
+    k = 0
+    da_abs, da_rel, cells = [], [], []
+    for t, s in enumerate(pick):
+        counts, c_cells, alpha = cloud.candidates(int(s))
+        nref = int(ref["counts"][t, :3].sum())
+        r_cells, r_alpha = ref["cells"][k:k + nref], ref["alpha"][k:k + nref]
+        k += nref
+        same = np.array_equal(c_cells, r_cells)
+        if not same:
+            return {"error": f"candidate cells of solid {s} differ"}
+        d = np.abs(alpha - r_alpha)
+        da_abs.append(d.max(initial=0.0))
+        big = r_alpha >= 1e-3
+        if big.any():
+            da_rel.append((d[big] / r_alpha[big]).max())
+        cells.append(r_cells)
+    cells = np.concatenate(cells)
+    Fg, Fr = g["Fs"][cells], ref["Fs"][cells]
+    fl = 1e-3 * np.abs(Fr).max()
+    m = np.abs(Fr) >= fl
+    rel = np.abs(Fg - Fr)[m] / np.abs(Fr)[m]
+    scale = np.maximum(np.abs(ref["ft"]), 1e-3 * np.abs(ref["ft"]).max())
+    return {"solids": int(pick.size), "cells": int(cells.size), "candidate_lists_bit_exact": True,
+            "As_max_abs": float(max(da_abs)), "As_max_rel_alpha_ge_1e-3": float(max(da_rel)),
+            "As_abs_p99": float(np.quantile(np.abs(g["As"][cells] - ref["As"][cells]), 0.99)),
+            "Fs_max_rel_above_1e-3_of_max": float(rel.max()), "Fs_p99_rel": float(np.quantile(rel, 0.99)),
+            "Fs_max_abs_over_max": float(np.abs(Fg - Fr).max() / np.abs(Fr).max()),
+            "force_torque_max_rel": float(np.max(np.abs(ft - ref["ft"]) / scale))}
+
+
 def surrogate_comparison(cloud, sc, ft0, As0, plain_ms, plain_cells_ms, steps):
     """The same step with the learned fraction (MLP 11-32-32-1, BF16 MFMA) serving the cut cells
     of spheres and ellipsoids, at the prescribed fallback rule (prediction in [0.02, 0.98] and a
@@ -189,7 +293,7 @@ def surrogate_comparison(cloud, sc, ft0, As0, plain_ms, plain_cells_ms, steps):
         cloud.interact_device(0.0, sc.dt)
     torch.cuda.synchronize()
     k = max(5, min(steps, 20))
-    ph = np.zeros(6)
+    ph = np.zeros(7)
     t0 = time.perf_counter()
     for _ in range(k):
         cloud.interact_device(0.0, sc.dt)
@@ -249,7 +353,7 @@ def bench_ellipsoids_loop(args):
     log(f"[bench] uploaded in {time.time() - t0:.1f}s")
     # per-kernel phases of one interact of the initial state
     cloud.interact(0.0, dt)
-    ph = np.zeros(6)
+    ph = np.zeros(7)
     for _ in range(5):
         cloud.interact_device(0.0, dt)
         ph += cloud.timing()
@@ -318,6 +422,129 @@ def bench_ellipsoids_loop(args):
                                           f"({pairs_t} cell-updates, {el:.1f}s), reference-cost bookkeeping, "
                                           f"FP64, 1 thread",
                                 "speedup": line["value"] / rate}
+    return line
+
+
+def host_info():
+    """Cores and model of this host's CPU (the CPU baselines run here)."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return {"nproc": cores, "cpu_model": model}
+
+
+def bench_polydisperse(args):
+    """Configuration C4: small spheres (2-4 cells) and large CSG solids (40-80 cells) on a 384^3
+    mesh: one interact per step, the size bins (the on-chip tile pass and the big-tile pass)
+    timed apart, the autotuner's choice, and full-size checks of the result."""
+    import torch
+
+    from sdfibm_amd import scenes
+    from sdfibm_amd.cloud import SolidCloud
+
+    n = args.n if args.n != 256 else 384
+    n_small = args.solids if args.solids != 10000 else 100000
+    n_big = 50
+    t0 = time.time()
+    sc = None
+    if args.scene_cache and os.path.exists(args.scene_cache):
+        z = np.load(args.scene_cache, allow_pickle=False)
+        if "csg" in z.files and int(z["n"]) == n and int(z["ns"]) == n_small:
+            from sdfibm_amd import mesh as M
+            sc = scenes.Scene(f"polydisperse_csg_{n}_{n_small}_{n_big}", M.uniform_box((n,) * 3, (0, 0, 0), (1, 1, 1)),
+                              z["solids"], z["U"], False, 1.0, 1e-3, json.loads(str(z["info"])), z["csg"])
+    if sc is None:
+        sc = scenes.polydisperse_csg(n, n_small, n_big)
+        if args.scene_cache:
+            np.savez(args.scene_cache, n=n, ns=n_small, solids=sc.solids, U=sc.U, csg=sc.csg,
+                     info=json.dumps(sc.info))
+    log(f"[bench] scene {sc.name} ready in {time.time() - t0:.1f}s: {sc.info}")
+    N = sc.mesh.n_cells
+    ns = sc.solids.size
+    cloud = SolidCloud(sc.mesh, rho_f=sc.rho_f, precision=args.precision)
+    cloud.set_csg_shapes(sc.csg)
+    cloud.set_solids(sc.solids)
+    cloud.set_velocity(sc.U)
+    tune = cloud.autotune(sc.dt, args.tune_cache or None)
+    log(f"[bench] autotune: {tune}")
+    for _ in range(args.warmup):
+        cloud.interact_device(0.0, sc.dt)
+    torch.cuda.synchronize()
+    ph = np.zeros(7)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        cloud.interact_device(0.0, sc.dt)
+        ph += cloud.timing()
+    torch.cuda.synchronize()
+    ms_per_step = 1e3 * (time.perf_counter() - t) / args.steps
+    ph /= args.steps
+    cloud.interact_steps(0.0, sc.dt, args.steps)
+    graph_ms = cloud.timing()[3] / args.steps
+    ft = cloud.interact(0.0, sc.dt)
+    f = cloud.fields()
+    pairs_in, pairs_cut = cloud.pair_counts()
+    big_pairs = [int(cloud.candidates(s)[0][:3].sum()) for s in range(n_big)]
+    n_pairs = pairs_in + pairs_cut
+    nc, nv = touched_stats(sc.mesh, f["Ct"])
+    rb = 4 if args.precision == "fp32" else 8
+    kernels = {"k_fields": {"ms": ph[2], "bytes": (5 * rb + 4) * N + 3 * rb * nc}}
+    achieved = kernels["k_fields"]["bytes"] / (ph[2] * 1e-3)
+    # full-size checks: force balance (the solids do not overlap), As range, volume
+    V = (1.0 / n) ** 3
+    F_cells = sc.rho_f * f["Fs"].sum(axis=0) * V
+    F_solids = ft[:, :3].sum(axis=0)
+    vol_solids = float(sc.csg["volume"].sum() + (4.0 / 3.0 * math.pi * sc.solids["prm"][n_big:, 0] ** 3).sum())
+    small_ms = ph[4] + ph[5]
+    line = {
+        "metric": METRIC, "value": n_pairs / (ms_per_step * 1e-3), "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic (seeded generator)",
+        "config": {"workload": f"polydisperse_csg {n}^3 hex, {n_small} spheres (d ~ U[2,4] cells) + {n_big} CSG "
+                               f"solids (unions/differences/intersections of boxes, cylinders, ellipsoids, spheres; "
+                               f"40-80 cells), plain method, {args.precision} fields, fp64 reductions",
+                   "cells": N, "solids": int(ns), "candidate_pairs": n_pairs, "inside_pairs": pairs_in,
+                   "cut_pairs": pairs_cut, "touched_cells": nc,
+                   "solid_volume_fraction": float(sc.info["solid_volume_fraction"]), "parallelism": "single GPU"},
+        "graph_ms_per_step": graph_ms,
+        "phases_ms": {"setup_bin": ph[0], "k_classify": ph[4], "k_cells": ph[5], "big_tile_pass": ph[6],
+                      "k_fields": ph[2]},
+        "size_bins": {
+            "small": {"solids": int(ns - n_big), "candidate_pairs": int(n_pairs - sum(big_pairs)),
+                      "ms": small_ms, "kernels": "k_classify (LDS bit rows) + k_cells, main stream"},
+            "big": {"solids": n_big, "candidate_pairs": int(sum(big_pairs)), "ms": ph[6],
+                    "kernels": "k_big_* (global-memory labelling, chunked cut cells), second stream",
+                    "largest_solid_pairs": int(max(big_pairs))},
+            "load_balance": {"big_over_small_ms": ph[6] / small_ms if small_ms > 0 else None,
+                             "tile_phase_ms": ph[1], "critical_path": "big" if ph[6] > small_ms else "small",
+                             "pairs_per_us": {"small": (n_pairs - sum(big_pairs)) / (small_ms * 1e3),
+                                              "big": sum(big_pairs) / (ph[6] * 1e3) if ph[6] > 0 else None}},
+        },
+        "autotune": tune,
+        "roofline": {"bound": "hbm", "kernel": "k_fields", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic("k_fields", f"polydisperse_csg_{n}_{n_small}_{n_big}_{args.precision}"),
+                     "algorithmic_bytes": kernels["k_fields"]["bytes"]},
+        "checks": {"force_balance_rel": float(np.abs(F_cells - F_solids).max() / max(np.abs(F_solids).max(), 1e-300)),
+                   "As_min": float(f["As"].min()), "As_max": float(f["As"].max()),
+                   "sum_As_V_over_solid_volume": float(f["As"].sum() * V / vol_solids)},
+    }
+    cloud.close()
+    if not args.no_cpu:
+        rate, ns_t, pairs_t, el = cpu_baseline(sc, args.cpu_sample, args.cpu_budget)
+        line["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                                "sample": f"first {ns_t} of {ns} solids in index order (the {n_big} CSG solids "
+                                          f"first) on the full {n}^3 mesh ({pairs_t} cell-updates, {el:.1f}s), "
+                                          f"reference-cost bookkeeping, FP64, 1 thread",
+                                "speedup": line["value"] / rate, "host": host_info()}
     return line
 
 
@@ -410,9 +637,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=8, help="threads of the fast CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the fast and z-slab CPU baselines (0: this host's cores, at most 16)")
+    ap.add_argument("--no-fp64", action="store_true", help="cloud3d fp32: skip the FP64 point")
     ap.add_argument("--scene-cache", default="", help="npz file to reuse the generated scene")
-    ap.add_argument("--workload", default="cloud3d", choices=["cloud3d", "sedimentation2d", "ellipsoids3d_jittered"])
+    ap.add_argument("--workload", default="cloud3d",
+                    choices=["cloud3d", "sedimentation2d", "ellipsoids3d_jittered", "polydisperse_csg"])
+    ap.add_argument("--tune-cache", default="", help="autotuner cache file (JSON lines); default: time every run")
     ap.add_argument("--loop-steps", type=int, default=None,
                     help="coupled steps timed (sedimentation2d: 200, ellipsoids3d_jittered: 500)")
     ap.add_argument("--contact", type=float, nargs=2, default=None, metavar=("K", "C"),
@@ -453,6 +684,9 @@ def main():
         args.loop_steps = args.loop_steps or 200
         print(json.dumps(bench_sedimentation(args)), flush=True)
         return
+    if args.workload == "polydisperse_csg":
+        print(json.dumps(bench_polydisperse(args)), flush=True)
+        return
     if args.workload == "ellipsoids3d_jittered":
         if args.n == 256 and args.solids == 10000:  # the cloud3d defaults
             args.n, args.solids = 200, 2000
@@ -483,10 +717,12 @@ def main():
     cloud = SolidCloud(sc.mesh, two_d=sc.two_d, rho_f=sc.rho_f, precision=args.precision)
     cloud.set_solids(sc.solids)
     cloud.set_velocity(sc.U)
+    tune = cloud.autotune(sc.dt, args.tune_cache or None)
+    log(f"[bench] autotune: {tune}")
     for _ in range(args.warmup):
         cloud.interact_device(0.0, sc.dt)
     torch.cuda.synchronize()
-    phases = np.zeros(6)
+    phases = np.zeros(7)
     t_start = time.perf_counter()
     for _ in range(args.steps):
         cloud.interact_device(0.0, sc.dt)
@@ -497,6 +733,24 @@ def main():
     phases /= args.steps
     log(f"[bench] {ms_per_step:.3f} ms/step; phases (setup+bin, tiles, fields, sum, classify, cells) ms = {phases}")
 
+    # the same interact as replays of one captured HIP graph (no host synchronisation between steps)
+    cloud.interact_steps(0.0, sc.dt, args.steps)
+    graph_ms = cloud.timing()[3] / args.steps
+    # a host-coupled caller's step: the solids' new poses (set_solids, same shapes) and the
+    # interact with the force/torque copy back; and with U handed over from the host as well
+    k_h = 10
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(k_h):
+        cloud.set_solids(sc.solids)
+        cloud.interact(0.0, sc.dt)
+    host_ms = 1e3 * (time.perf_counter() - t) / k_h
+    t = time.perf_counter()
+    for _ in range(k_h):
+        cloud.set_solids(sc.solids)
+        cloud.set_velocity(sc.U)
+        cloud.interact(0.0, sc.dt)
+    host_u_ms = 1e3 * (time.perf_counter() - t) / k_h
     ft = cloud.interact(0.0, sc.dt)
     f = cloud.fields()
     n_pairs, n_inside, n_cut, per_solid = candidate_stats(cloud, sc.solids.size)
@@ -541,6 +795,10 @@ def main():
                    "parallelism": "single GPU"},
         "phases_ms": {"setup_bin": phases[0], "k_classify": phases[4], "k_cells": phases[5],
                       "k_fields": phases[2]},
+        "graph_ms_per_step": graph_ms,
+        "host_coupled_ms_per_step": {"set_solids+interact+ft_copy": host_ms,
+                                     "set_solids+set_velocity+interact+ft_copy": host_u_ms},
+        "autotune": tune,
         "mesh_cells_per_s": N / (ms_per_step * 1e-3),
         "hbm_frac_whole_step": B_total / (ms_per_step * 1e-3) / HBM_PEAK,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
@@ -561,20 +819,52 @@ def main():
     line["roofline"]["per_kernel"] = per_kernel
     if args.precision == "fp32" and not args.no_surrogate:
         line["surrogate"] = surrogate_comparison(cloud, sc, ft, f["As"], ms_per_step, phases[5], args.steps)
+    if args.precision == "fp32" and not args.no_fp64:
+        # the FP64 point of the same scene (FP64 fractions and fields)
+        with SolidCloud(sc.mesh, two_d=sc.two_d, rho_f=sc.rho_f, precision="fp64") as c64:
+            c64.set_solids(sc.solids)
+            c64.set_velocity(sc.U)
+            c64.autotune(sc.dt)
+            for _ in range(2):
+                c64.interact_device(0.0, sc.dt)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                c64.interact_device(0.0, sc.dt)
+            torch.cuda.synchronize()
+            ms64 = 1e3 * (time.perf_counter() - t) / args.steps
+        line["fp64_point"] = {"ms_per_step": ms64, "value": n_pairs / (ms64 * 1e-3), "unit": "cell-updates/s"}
     if not args.no_cpu:
-        rate, ns_t, pairs_t, el = cpu_baseline(sc, args.cpu_sample, args.cpu_budget)
+        host = host_info()
+        om = oracle_mesh(sc)
+        if args.precision == "fp32":
+            line["fp32_vs_fp64_oracle"] = fp32_error_statement(cloud, sc, om)
+            log(f"[bench] fp32 errors: {line['fp32_vs_fp64_oracle']}")
+        rate, ns_t, pairs_t, el = cpu_baseline(sc, args.cpu_sample, args.cpu_budget, om=om)
         line["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                                 "sample": f"first {ns_t} of {sc.solids.size} solids in index order on the full "
                                           f"{args.n}^3 mesh ({pairs_t} cell-updates, {el:.1f}s), reference-cost "
                                           f"bookkeeping, FP64, 1 thread",
-                                "speedup": value / rate}
-        threads = int(args.cpu_threads)
-        rate_f, ns_f, pairs_f, el_f = cpu_baseline_fast(sc, per_solid, threads, args.cpu_budget / 2)
+                                "speedup": value / rate, "host": host}
+        threads = int(args.cpu_threads) if args.cpu_threads > 0 else min(host["nproc"], 16)
+        rate_f, ns_f, pairs_f, el_f = cpu_baseline_fast(sc, per_solid, threads, args.cpu_budget / 2, om=om)
         line["cpu_baseline_fast"] = {"value": rate_f, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                      "sample": f"first {ns_f} of {sc.solids.size} solids ({pairs_f} cell-updates, "
                                                f"{el_f:.1f}s), fast bookkeeping, FP64, solids split over "
                                                f"{threads} threads",
-                                     "speedup": value / rate_f}
+                                     "speedup": value / rate_f, "host": host}
+        del om
+        rate_s, ns_s, pairs_s, el_s, nr, _ = cpu_baseline_slabs(sc, per_solid, threads, args.cpu_sample,
+                                                                  args.cpu_budget)
+        line["cpu_baseline_slabs"] = {"value": rate_s, "unit": "cell-updates/s", "cores": nr, "kind": "port",
+                                      "sample": f"first {ns_s} of {sc.solids.size} solids on {nr} z-slab sub-meshes "
+                                                f"run concurrently ({pairs_s} cell-updates, {el_s:.1f}s): the "
+                                                f"reference's MPI decomposition emulated, reference-cost "
+                                                f"bookkeeping per rank, FP64",
+                                      "speedup": value / rate_s, "host": host}
+        best = max(rate, rate_s)
+        line["speedup_vs_reference_cpu"] = {"value": value / best, "basis": "best of the one-thread and the "
+                                            "z-slab reference-cost runs", "gate": 30.0, "meets": value / best >= 30.0}
     cloud.close()
     print(json.dumps(line), flush=True)
 

@@ -246,6 +246,15 @@ int sdfibm_set_contact(sdfibm_ctx* ctx, double stiffness, double damping);
  * motions or forcers).  sdfibm_timing's total is the replays' device time. */
 int sdfibm_run_steps(sdfibm_ctx* ctx, double time, double dt, int64_t n_steps, const double* gravity, double* ft);
 
+/* n_steps interacts of the current state (time, dt) as replays of one captured
+ * HIP graph (captured at the first call, again when dt or any setting changes),
+ * without host synchronisation between them: what a caller that keeps U and the
+ * solids on the device issues per step.  One eager interact first computes the
+ * plane solids' cell sets and the buffer sizes.  Errors of any replay are
+ * reported after the last.  ft: [ns*6] force/torque of the last, or NULL.
+ * sdfibm_timing's total is the replays' device time. */
+int sdfibm_interact_steps(sdfibm_ctx* ctx, double time, double dt, int64_t n_steps, double* ft);
+
 /* Device -> host copy of the solids' current records (after evolve). */
 int sdfibm_get_solids(sdfibm_ctx* ctx, sdfibm_solid* out, int64_t n);
 
@@ -336,9 +345,29 @@ int sdfibm_buffer_capacity(sdfibm_ctx* ctx, int64_t* out, int n);
 /* Timing of the phases of the last interact (milliseconds, HIP events on the
  * context's stream): [0] setup+binning, [1] per-solid tiles (classification,
  * cut cells and forcing, plane labelling), [2] field pass, [3] total,
- * [4] tile classification alone, [5] cut cells + forcing + plane labelling.
- * Returns the number of values written (<= n, at most 6). */
+ * [4] tile classification alone, [5] cut cells + forcing + plane labelling,
+ * [6] the big-tile pass (on the second stream, beside [4] and [5]; 0 without big
+ * tiles).  Returns the number of values written (<= n, at most 7). */
 int sdfibm_last_timing(sdfibm_ctx* ctx, double* ms, int n);
+
+/* Run-time variants of the step (no counterpart in the reference): the cut-cell
+ * pass of the tiles of the on-chip tile pass with one wave per solid
+ * (cells_wave 1), 256 threads per solid (0) or by the mean tile size (-1, the
+ * default); the big tiles' cut-cell pass in chunks of big_chunk candidates
+ * (64..16384, default 1024) over big_grid workgroups per CU (1..32, default 4).
+ * Results are the same up to the order of the FP64 force/torque sums. */
+int sdfibm_set_tuning(sdfibm_ctx* ctx, int32_t cells_wave, int32_t big_chunk, int32_t big_grid);
+
+/* Autotuner (no counterpart in the reference): times the variants above on the
+ * context's solids and U (interact runs several times; the fields then hold the
+ * current state's step) and keeps the fastest, searching one knob at a time --
+ * per size bin: the small tiles' cut-cell pass, then, when big tiles can occur,
+ * the big pass's chunk size and grid.  cache_path (may be NULL): a JSON-lines
+ * file keyed by the mesh, precision and the histogram of the solids' tile sizes;
+ * a matching line is applied without timing, a miss appends one.  chosen (may be
+ * NULL) receives the choice as JSON text (cells_wave, big_chunk, big_grid, ms,
+ * source "cache" or "timed", variants_timed). */
+int sdfibm_autotune(sdfibm_ctx* ctx, double dt, const char* cache_path, char* chosen, int64_t chosen_cap);
 
 /* Launch the work of interact on a caller-provided hipStream_t (NULL = the
  * context's own stream). */

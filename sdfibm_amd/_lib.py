@@ -20,7 +20,8 @@ EXPORTS = (
     "sdfibm_set_dynamics", "sdfibm_evolve", "sdfibm_get_solids", "sdfibm_set_contact", "sdfibm_run_steps", "sdfibm_pair_counts",
     "sdfibm_surrogate_train", "sdfibm_surrogate_samples", "sdfibm_surrogate_eval", "sdfibm_surrogate_grad",
     "sdfibm_set_surrogate", "sdfibm_surrogate_count", "sdfibm_buffer_capacity", "sdfibm_set_csg_shapes",
-    "sdfibm_set_velocity_cells_noalloc",
+    "sdfibm_set_velocity_cells_noalloc", "sdfibm_set_tuning", "sdfibm_autotune",
+    "sdfibm_interact_steps",
 )
 
 SDFIBM_OK = 0
@@ -98,6 +99,9 @@ def load():
         "sdfibm_buffer_capacity": (i32, [P, P, i32]),
         "sdfibm_set_csg_shapes": (i32, [P, P, i64]),
         "sdfibm_set_velocity_cells_noalloc": (i32, [P, i64, i64, P]),
+        "sdfibm_set_tuning": (i32, [P, i32, i32, i32]),
+        "sdfibm_autotune": (i32, [P, d, C.c_char_p, C.c_char_p, i64]),
+        "sdfibm_interact_steps": (i32, [P, d, d, i64, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -137,6 +137,14 @@ class SolidCloud:
         """interact without copying force/torque back (it still waits for the step)."""
         _lib.check(_lib.load().sdfibm_interact(self._h, float(time), float(dt), None), self._h)
 
+    def interact_steps(self, time: float, dt: float, n_steps: int) -> np.ndarray:
+        """n_steps interacts of the current state as replays of one captured HIP graph; returns
+        the last one's force/torque (timing()[3] is the replays' device time)."""
+        ft = np.zeros((self.n_solids, 6))
+        _lib.check(_lib.load().sdfibm_interact_steps(self._h, float(time), float(dt), int(n_steps), _p(ft)),
+                   self._h)
+        return ft
+
     def update_velocity(self, dt: float) -> None:
         """U -= Fs*dt on the device (the solver step after interact, src/main.cpp:70)."""
         _lib.check(_lib.load().sdfibm_update_velocity(self._h, float(dt)), self._h)
@@ -224,6 +232,20 @@ class SolidCloud:
         _lib.check(_lib.load().sdfibm_pair_counts(self._h, _p(out)), self._h)
         return int(out[0]), int(out[1])
 
+    def set_tuning(self, cells_wave: int = -1, big_chunk: int = 1024, big_grid: int = 4):
+        """Run-time variants of the step (see sdfibm_set_tuning)."""
+        _lib.check(_lib.load().sdfibm_set_tuning(self._h, int(cells_wave), int(big_chunk), int(big_grid)), self._h)
+
+    def autotune(self, dt: float, cache_path: str | None = None) -> dict:
+        """Time the step's variants on the current solids and keep the fastest (see
+        sdfibm_autotune); returns the choice."""
+        import json
+
+        buf = C.create_string_buffer(512)
+        path = cache_path.encode() if cache_path else None
+        _lib.check(_lib.load().sdfibm_autotune(self._h, float(dt), path, buf, 512), self._h)
+        return json.loads(buf.value.decode())
+
     def buffer_capacity(self) -> tuple:
         """(cells the tile-code buffer holds, solids the per-solid buffers hold)."""
         out = np.zeros(2, dtype=np.int64)
@@ -232,9 +254,10 @@ class SolidCloud:
 
     def timing(self) -> np.ndarray:
         """Phase times of the last interact in ms: setup+binning, tile pass, field pass, total,
-        tile classification, cut cells + forcing (+ plane labelling)."""
-        ms = np.zeros(6)
-        _lib.load().sdfibm_last_timing(self._h, _p(ms), 6)
+        tile classification, cut cells + forcing (+ plane labelling), big-tile pass (second
+        stream)."""
+        ms = np.zeros(7)
+        _lib.load().sdfibm_last_timing(self._h, _p(ms), 7)
         return ms
 
     def device_pointers(self) -> dict:

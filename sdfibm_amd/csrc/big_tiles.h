@@ -28,13 +28,13 @@
 //                   0 cut (placeholder); a second look at every edge checks the
 //                   union converged (EF_LABEL otherwise)
 //   k_big_list      candidate lists and counts, one workgroup per solid
-//   k_big_chunks    chunks of kBigChunk candidates, their offsets
+//   k_big_chunks    chunks of candidates (the chunk size is a run-time choice), their offsets
 //   k_big_force     fractions, codes and forcing per chunk; FP64 partial sums
 //   k_big_ft        partials summed in chunk order (x rho_f) -> force/torque
 #pragma once
 
 constexpr int kBigMaxSolids = 4096;  // big tiles per step (SDFIBM_ERR_CAPACITY beyond)
-constexpr int kBigChunk = 1024;      // candidate entries per workgroup of k_big_force
+constexpr int kBigChunk = 1024;      // default candidate entries per chunk of k_big_force (the autotuner's choice)
 
 struct BigMeta {
     int solid;
@@ -289,7 +289,7 @@ template <class R>
 __global__ void __launch_bounds__(1024) k_big_list(const TileInfo* __restrict__ Ts, BigMeta* __restrict__ meta,
                                                    const long long* __restrict__ tot, const R* __restrict__ codes,
                                                    int* __restrict__ cand, int2* __restrict__ ncand,
-                                                   long long code_cap) {
+                                                   long long code_cap, int chunk_entries) {
     __shared__ int scan_lds[32];
     const int nbig = (int)tot[0];
     const int tid = threadIdx.x;
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(1024) k_big_list(const TileInfo* __restrict__ 
         }
         if (tid == 0) {
             ncand[s] = make_int2(tin, tcut);
-            meta[b].nchunk = (tin + tcut + kBigChunk - 1) / kBigChunk;
+            meta[b].nchunk = (tin + tcut + chunk_entries - 1) / chunk_entries;
         }
         __syncthreads();
     }
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(NT) k_big_force(MeshDev m, const SolidK* __res
                                                   const int2* __restrict__ ncand, const R* __restrict__ Ux,
                                                   const R* __restrict__ Uy, const R* __restrict__ Uz,
                                                   R* __restrict__ codes, long long code_cap, double dt,
-                                                  double* __restrict__ part) {
+                                                  double* __restrict__ part, int chunk_entries) {
     __shared__ double red[6 * 16];
     const int nbig = (int)tot[0];
     const long long nch = tot[4];
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(NT) k_big_force(MeshDev m, const SolidK* __res
         const SolidK K = Ks[s];
         const int2 cnt = ncand[s];
         const int nin = cnt.x, ntot = cnt.x + cnt.y;
-        const int e0 = (int)(ch - meta[b].chunk0) * kBigChunk, e1 = min(e0 + kBigChunk, ntot);
+        const int e0 = (int)(ch - meta[b].chunk0) * chunk_entries, e1 = min(e0 + chunk_entries, ntot);
         const int* tl = cand + T.off;
         R* tcodes = codes + T.off;
         const float rn0 = 1.0f / (float)T.n[0], rn1 = 1.0f / (float)T.n[1];

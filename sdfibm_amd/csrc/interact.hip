@@ -1723,6 +1723,9 @@ struct sdfibm_ctx {
     // the coupled step captured as a HIP graph (run_steps)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph_i = nullptr;  // interact alone, captured (interact_steps)
+    hipGraphExec_t gexec_i = nullptr;
+    double graph_i_dt = 0.0;
     double graph_dt = 0.0, graph_g[3] = {0, 0, 0};
     int* d_sticky = nullptr;
     double* d_plane_pose = nullptr;  // center + q of each plane at capture
@@ -1753,8 +1756,13 @@ struct sdfibm_ctx {
     unsigned long long* d_big_vbits = nullptr;
     double* d_big_part = nullptr;
     long long big_word_cap = 0, big_chunk_cap = 0;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    double timing[6] = {0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // [5], [6]: big-tile pass
+    double timing[7] = {0, 0, 0, 0, 0, 0, 0};
+    // run-time variants (the autotuner's choices): candidates per chunk and workgroups per CU
+    // of the big tiles' cut-cell pass; the small tiles' cut-cell pass one wave per solid
+    // (1), 256 threads (0), or by the mean tile size (-1)
+    int big_chunk = kBigChunk, big_grid = 4, cells_wave = -1;
+    std::string tile_key;  // histogram of the solids' tile sizes at set_solids (the autotuner's cache key)
     std::string msg;
 };
 
@@ -1890,7 +1898,7 @@ int validate_csg(const sdfibm_csg_shape& g, long long idx, std::string& why) {
 template <class R, int MK>
 void launch_big(sdfibm_ctx* c, double dt, hipStream_t st) {
     const int ns = (int)c->ns;
-    const unsigned G = (unsigned)(4 * c->n_cu);
+    const unsigned G = (unsigned)(4 * c->n_cu), GF = (unsigned)(c->big_grid * c->n_cu);
     R* codes = (R*)c->d_codes;
     k_big_layout<<<1, 1024, 0, st>>>(ns, c->d_T, c->qmeta, c->qlist, c->d_big_meta, c->d_big_tot, c->big_word_cap,
                                      c->err);
@@ -1901,11 +1909,12 @@ void launch_big(sdfibm_ctx* c, double dt, hipStream_t st) {
     k_big_member<R><<<G, 256, 0, st>>>(c->m, c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, codes,
                                        c->code_cap, c->err);
     k_big_list<R><<<(unsigned)c->n_cu, 1024, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, codes, c->d_cand,
-                                                      c->d_ncand, c->code_cap);
+                                                      c->d_ncand, c->code_cap, c->big_chunk);
     k_big_chunks<<<1, 1024, 0, st>>>(c->d_big_meta, c->d_big_tot, c->big_chunk_cap, c->err);
-    k_big_force<R, MK, kBlock><<<G, kBlock, 0, st>>>(c->m, c->d_K, c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand,
+    k_big_force<R, MK, kBlock><<<GF, kBlock, 0, st>>>(c->m, c->d_K, c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand,
                                                      c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
-                                                     (const R*)c->U[2], codes, c->code_cap, dt, c->d_big_part);
+                                                     (const R*)c->U[2], codes, c->code_cap, dt, c->d_big_part,
+                                                     c->big_chunk);
     k_big_ft<<<4, 256, 0, st>>>(c->d_big_meta, c->d_big_tot, c->d_big_part, c->rho_f, c->d_ft);
 }
 
@@ -1952,7 +1961,11 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
             c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
         k_classify<R, MK, kRowsMedium, 1, 1024><<<std::min(g, 4 * cus), 1024, 0, c->aux>>>(
             c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
-        if (c->big_on) launch_big<R, MK>(c, dt, c->aux);
+        if (c->big_on) {
+            if (!graph) (void)hipEventRecord(c->ev[5], c->aux);
+            launch_big<R, MK>(c, dt, c->aux);
+            if (!graph) (void)hipEventRecord(c->ev[6], c->aux);
+        }
         if (hipEventRecord(c->join, c->aux) != hipSuccess) return -1;
         k_classify<R, MK, kRowsSmall, 0, kBlock><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
                                                                c->d_cand, c->code_cap, c->d_ncand);
@@ -1964,7 +1977,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
         if (kSgOk && c->d_sg != nullptr) {
             // same workgroup size as the plain pass: identical reduction order when every cell falls back
             k_fill_i32<<<1, 64, 0, st>>>(c->d_sg_count, 1, 0);
-            if (c->small_tiles)
+            if (c->cells_wave == 1 || (c->cells_wave < 0 && c->small_tiles))
                 k_cells<R, MK, kSgOk, 64><<<g, 64, 0, st>>>(
                     c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand, c->d_ncand, (const R*)c->U[0],
                     (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft, c->d_sg,
@@ -1974,7 +1987,8 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
                     c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand, c->d_ncand, (const R*)c->U[0],
                     (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft, c->d_sg,
                     c->sg_lo, c->sg_hi, c->sg_cos, c->d_sg_count);
-        } else if (c->small_tiles) {  // a few dozen candidates per solid: one wave each
+        } else if (c->cells_wave == 1 || (c->cells_wave < 0 && c->small_tiles)) {
+            // a few dozen candidates per solid: one wave each
             k_cells<R, MK, false, 64><<<g, 64, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
                                                         c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
                                                         (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft,
@@ -2241,6 +2255,10 @@ void graph_reset(sdfibm_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->gexec = nullptr;
     c->graph = nullptr;
+    if (c->gexec_i) (void)hipGraphExecDestroy(c->gexec_i);
+    if (c->graph_i) (void)hipGraphDestroy(c->graph_i);
+    c->gexec_i = nullptr;
+    c->graph_i = nullptr;
 }
 
 // Launch-only parts of the steps around interact (no synchronisation; also
@@ -2617,6 +2635,7 @@ int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     std::vector<int> planes;
     long long cells = 0, tile_sum = 0, n_tiles = 0, big_words = 0, big_chunks = 0;
     bool big_on = false;
+    int hist[34] = {0};  // tiles per power of two of their cell count; [32] planes and oversized, [33] big
     double hmin[3];
     min_spacing(c, hmin);
     const long long nn[3] = {c->nx, c->ny, c->nz};
@@ -2629,7 +2648,12 @@ int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         TileInfo t = c->kind == SDFIBM_MESH_RECTILINEAR ? make_tile<MK_RECT>(c->hm, K) : make_tile<MK_STRUCT>(c->hm, K);
         if (t.cls == TC_TOOBIG) {  // oversized: the unbounded path, like a plane
             planes.push_back((int)s);
+            ++hist[32];
             continue;
+        }
+        if (t.cls != TC_EMPTY) {
+            const long long nt = (long long)t.n[0] * t.n[1] * t.n[2];
+            ++hist[t.cls == TC_BIG ? 33 : std::min(31, 63 - __builtin_clzll((unsigned long long)nt))];
         }
         // can the solid's tile, in any pose, outgrow the on-chip tile pass?  (its bounding
         // sphere in cells of the finest spacing)
@@ -2730,6 +2754,14 @@ int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     c->small_tiles = n_tiles > 0 && tile_sum < 512 * n_tiles;
     c->big_on = big_on;
     if (big_on && (rc = grow_big(c, big_words, big_chunks))) return rc;
+    {
+        std::string key = std::to_string(c->nx) + "x" + std::to_string(c->ny) + "x" + std::to_string(c->nz) +
+                          (c->kind == SDFIBM_MESH_RECTILINEAR ? ":rect:" : ":struct:") +
+                          (c->prec == SDFIBM_FP64 ? "fp64:" : "fp32:") + std::to_string(n) + ":";
+        for (int b = 0; b < 34; ++b)
+            if (hist[b]) key += std::to_string(b) + "=" + std::to_string(hist[b]) + ",";
+        c->tile_key = key;
+    }
     if (n > 0) {
         std::vector<unsigned char> unb(n, 0);
         for (int p : planes) unb[p] = 1;
@@ -2874,6 +2906,9 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     c->timing[3] = (double)ms[0] + ms[1] + ms[2];
     c->timing[4] = ms[3];
     c->timing[5] = ms[4];
+    float msb = 0.f;
+    if (c->big_on) (void)hipEventElapsedTime(&msb, c->ev[5], c->ev[6]);
+    c->timing[6] = msb;
     if (flags) {
         c->have_step = false;
         c->have_ct = false;
@@ -3141,6 +3176,62 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
     return SDFIBM_OK;
 }
 
+int sdfibm_interact_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, double* ft) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "interact_steps before set_solids");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
+    if (n_steps < 0) return fail(c, SDFIBM_ERR_ARG, "n_steps must be >= 0");
+    if (n_steps == 0) return SDFIBM_OK;
+    // an eager pass: plane memberships (the replays keep them: nothing moves between
+    // interacts) and buffer sizes
+    int rc = sdfibm_interact(c, time, dt, nullptr);
+    if (rc) return rc;
+    if (!c->d_sticky && (rc = dalloc(c, (void**)&c->d_sticky, 64))) return rc;
+    if (c->gexec_i && c->graph_i_dt != dt) {
+        (void)hipGraphExecDestroy(c->gexec_i);
+        (void)hipGraphDestroy(c->graph_i);
+        c->gexec_i = nullptr;
+        c->graph_i = nullptr;
+    }
+    if (!c->gexec_i) {
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int r = dispatch_interact(c, dt, true);
+        if (r == SDFIBM_OK) k_flag_or<<<1, 64, 0, c->stream>>>(c->err, c->d_sticky);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        if (r != SDFIBM_OK) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return r;
+        }
+        if (e != hipSuccess) return fail(c, SDFIBM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&c->gexec_i, gr, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(gr);
+            c->gexec_i = nullptr;
+            return fail(c, SDFIBM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+        c->graph_i = gr;
+        c->graph_i_dt = dt;
+    }
+    k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_sticky, 1, 0);
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int64_t t = 0; t < n_steps; ++t) HIPCHK(c, hipGraphLaunch(c->gexec_i, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    int flags = 0;
+    HIPCHK(c, hipMemcpyAsync(&flags, c->d_sticky, 4, hipMemcpyDeviceToHost, c->stream));
+    if (ft && c->ns > 0) HIPCHK(c, hipMemcpyAsync(ft, c->d_ft, c->ns * 48, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[3]);
+    for (double& t : c->timing) t = 0.0;
+    c->timing[3] = ms;
+    c->have_step = flags == 0;
+    c->have_ct = flags == 0;
+    if (flags)
+        return fail(c, (flags & EF_TILE) ? SDFIBM_ERR_TILE : SDFIBM_ERR_CAPACITY, err_text(flags));
+    return SDFIBM_OK;
+}
+
 int sdfibm_set_surrogate(sdfibm_ctx* c, const float* params, double band_lo, double band_hi, double max_angle_deg) {
     if (!c) return SDFIBM_ERR_ARG;
     graph_reset(c);
@@ -3212,7 +3303,7 @@ int sdfibm_buffer_capacity(sdfibm_ctx* c, int64_t* out, int n) {
 
 int sdfibm_last_timing(sdfibm_ctx* c, double* ms, int n) {
     if (!c || !ms || n < 0) return SDFIBM_ERR_ARG;
-    const int k = n < 6 ? n : 6;
+    const int k = n < 7 ? n : 7;
     for (int t = 0; t < k; ++t) ms[t] = c->timing[t];
     return k;
 }
@@ -3388,6 +3479,136 @@ int sdfibm_pair_counts(sdfibm_ctx* c, int64_t* out) {
         out[1] += cnt[1] + cnt[2];
     }
     c->have_step = had;
+    return SDFIBM_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// run-time variants and their autotuner (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+int sdfibm_set_tuning(sdfibm_ctx* c, int32_t cells_wave, int32_t big_chunk, int32_t big_grid) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (cells_wave < -1 || cells_wave > 1 || big_chunk < 64 || big_chunk > 16384 || big_grid < 1 || big_grid > 32)
+        return fail(c, SDFIBM_ERR_ARG, "tuning: cells_wave -1..1, big_chunk 64..16384, big_grid 1..32");
+    graph_reset(c);
+    c->cells_wave = cells_wave;
+    c->big_chunk = big_chunk;
+    c->big_grid = big_grid;
+    return SDFIBM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct Tuning {
+    int cells_wave, big_chunk, big_grid;
+    double ms;
+};
+
+// Device time of the step (sum of its phases on the main stream) under the context's
+// current variant: the median of `reps` interacts after two untimed ones.
+int time_step(sdfibm_ctx* c, double dt, int reps, double& ms) {
+    std::vector<double> t;
+    for (int r = 0; r < reps + 2; ++r) {
+        const int rc = sdfibm_interact(c, 0.0, dt, nullptr);
+        if (rc) return rc;
+        if (r >= 2) t.push_back(c->timing[3]);
+    }
+    std::sort(t.begin(), t.end());
+    ms = t[t.size() / 2];
+    return SDFIBM_OK;
+}
+
+// One line of the cache file for a key: {"key": "...", "cells_wave": a, "big_chunk": b, "big_grid": g, "ms": m}
+bool cache_lookup(const char* path, const std::string& key, Tuning& out) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return false;
+    char line[4096];
+    bool hit = false;
+    const std::string want = "\"key\": \"" + key + "\"";
+    while (std::fgets(line, sizeof(line), f)) {
+        const std::string l(line);
+        if (l.find(want) == std::string::npos) continue;
+        auto num = [&](const char* name, double& v) {
+            const std::string k = std::string("\"") + name + "\": ";
+            const size_t p = l.find(k);
+            if (p == std::string::npos) return false;
+            v = std::strtod(l.c_str() + p + k.size(), nullptr);
+            return true;
+        };
+        double a, b, g, m;
+        if (num("cells_wave", a) && num("big_chunk", b) && num("big_grid", g) && num("ms", m)) {
+            out = Tuning{(int)a, (int)b, (int)g, m};
+            hit = true;  // the last matching line wins
+        }
+    }
+    std::fclose(f);
+    return hit;
+}
+
+std::string tuning_json(const Tuning& t, const char* source, int tried) {
+    char buf[256];
+    std::snprintf(buf, sizeof(buf),
+                  "{\"cells_wave\": %d, \"big_chunk\": %d, \"big_grid\": %d, \"ms\": %.6f, \"source\": \"%s\", "
+                  "\"variants_timed\": %d}",
+                  t.cells_wave, t.big_chunk, t.big_grid, t.ms, source, tried);
+    return buf;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sdfibm_autotune(sdfibm_ctx* c, double dt, const char* cache_path, char* chosen, int64_t chosen_cap) {
+    if (!c) return SDFIBM_ERR_ARG;
+    if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "autotune before set_solids");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
+    Tuning best{c->cells_wave, c->big_chunk, c->big_grid, 0.0};
+    int tried = 0;
+    const char* source = "timed";
+    int rc;
+    if (cache_path && cache_lookup(cache_path, c->tile_key, best)) {
+        source = "cache";
+    } else {
+        // coordinate search, one knob at a time: the small tiles' cut-cell pass, then (with big
+        // tiles) the big pass's chunk size, then its grid
+        auto run = [&](Tuning t, double& ms) {
+            if ((rc = sdfibm_set_tuning(c, t.cells_wave, t.big_chunk, t.big_grid))) return rc;
+            ++tried;
+            return time_step(c, dt, 5, ms);
+        };
+        best.ms = 1e300;
+        for (int w : {0, 1}) {
+            Tuning t{w, kBigChunk, 4, 0.0};
+            if ((rc = run(t, t.ms))) return rc;
+            if (t.ms < best.ms) best = t;
+        }
+        if (c->big_on) {
+            for (int ch : {256, 512, 2048, 4096}) {
+                Tuning t{best.cells_wave, ch, best.big_grid, 0.0};
+                if ((rc = run(t, t.ms))) return rc;
+                if (t.ms < best.ms) best = t;
+            }
+            for (int g : {2, 8}) {
+                Tuning t{best.cells_wave, best.big_chunk, g, 0.0};
+                if ((rc = run(t, t.ms))) return rc;
+                if (t.ms < best.ms) best = t;
+            }
+        }
+        if (cache_path) {
+            if (FILE* f = std::fopen(cache_path, "a")) {
+                std::fprintf(f, "{\"key\": \"%s\", \"cells_wave\": %d, \"big_chunk\": %d, \"big_grid\": %d, \"ms\": %.6f}\n",
+                             c->tile_key.c_str(), best.cells_wave, best.big_chunk, best.big_grid, best.ms);
+                std::fclose(f);
+            }
+        }
+    }
+    if ((rc = sdfibm_set_tuning(c, best.cells_wave, best.big_chunk, best.big_grid))) return rc;
+    if (chosen && chosen_cap > 0) {
+        const std::string j = tuning_json(best, source, tried);
+        std::snprintf(chosen, (size_t)chosen_cap, "%s", j.c_str());
+    }
     return SDFIBM_OK;
 }
 
