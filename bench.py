@@ -233,43 +233,46 @@ def cpu_baseline_slabs(scene, per_solid_pairs, ranks, n_sample, budget_s):
     return pairs / el, n_take, pairs, el, len(meshes), ft
 
 
-def fp32_error_statement(cloud, scene, om, n_sample=16):
-    """Errors of the FP32 step against the FP64 oracle on the same full mesh, on sampled solids
-    (spread over the index range; the solids do not overlap, so each sampled solid's cells carry
-    only its own contributions): fractions (absolute; relative where alpha >= 1e-3), Fs per
-    element (relative where |Fs| >= 1e-3 of its largest value) and force/torque (relative to
-    max(|F|, 1e-3 max|F|))."""
+def fp32_error_statement(scene, om, n_sample=16):
+    """Errors of the FP32 step against the FP64 oracle on the same full mesh and velocity, for a
+    sample of the solids (spread over the index range) given alone to both: candidate lists,
+    fractions (absolute; relative where alpha >= 1e-3), As and Fs per element (Fs relative where
+    |Fs| >= 1e-3 of its largest value) and force/torque (relative to max(|F|, 1e-3 max|F|))."""
+    from sdfibm_amd.cloud import SolidCloud
+
     ns = scene.solids.size
     pick = np.unique(np.linspace(0, ns - 1, n_sample).astype(int))
-    ref = om.interact(scene.solids[pick], scene.U, rho_f=scene.rho_f, dt=scene.dt, capture=True,
-                      capacity=4000000)
-    ft = cloud.interact(0.0, scene.dt)[pick]
-    g = cloud.fields()
-    k = 0
-    da_abs, da_rel, cells = [], [], []
-    for t, s in enumerate(pick):
-        counts, c_cells, alpha = cloud.candidates(int(s))
-        nref = int(ref["counts"][t, :3].sum())
-        r_cells, r_alpha = ref["cells"][k:k + nref], ref["alpha"][k:k + nref]
-        k += nref
-        same = np.array_equal(c_cells, r_cells)
-        if not same:
-            return {"error": f"candidate cells of solid {s} differ"}
-        d = np.abs(alpha - r_alpha)
-        da_abs.append(d.max(initial=0.0))
-        big = r_alpha >= 1e-3
-        if big.any():
-            da_rel.append((d[big] / r_alpha[big]).max())
-        cells.append(r_cells)
-    cells = np.concatenate(cells)
-    Fg, Fr = g["Fs"][cells], ref["Fs"][cells]
+    sol = scene.solids[pick]
+    ref = om.interact(sol, scene.U, rho_f=scene.rho_f, dt=scene.dt, capture=True, capacity=4000000)
+    with SolidCloud(scene.mesh, two_d=scene.two_d, rho_f=scene.rho_f, precision="fp32") as c:
+        c.set_solids(sol)
+        c.set_velocity(scene.U)
+        ft = c.interact(0.0, scene.dt)
+        g = c.fields()
+        k = 0
+        da_abs, da_rel = [], []
+        for t in range(pick.size):
+            counts, c_cells, alpha = c.candidates(t)
+            nref = int(ref["counts"][t, :3].sum())
+            r_cells, r_alpha = ref["cells"][k:k + nref], ref["alpha"][k:k + nref]
+            k += nref
+            if not (np.array_equal(counts[:3], ref["counts"][t, :3]) and np.array_equal(c_cells, r_cells)):
+                return {"error": f"candidate cells of solid {pick[t]} differ"}
+            d = np.abs(alpha - r_alpha)
+            da_abs.append(d.max(initial=0.0))
+            big = r_alpha >= 1e-3
+            if big.any():
+                da_rel.append((d[big] / r_alpha[big]).max())
+    touched = ref["Ct"] != 0
+    Fg, Fr = g["Fs"][touched], ref["Fs"][touched]
     fl = 1e-3 * np.abs(Fr).max()
     m = np.abs(Fr) >= fl
     rel = np.abs(Fg - Fr)[m] / np.abs(Fr)[m]
     scale = np.maximum(np.abs(ref["ft"]), 1e-3 * np.abs(ref["ft"]).max())
-    return {"solids": int(pick.size), "cells": int(cells.size), "candidate_lists_bit_exact": True,
-            "As_max_abs": float(max(da_abs)), "As_max_rel_alpha_ge_1e-3": float(max(da_rel)),
-            "As_abs_p99": float(np.quantile(np.abs(g["As"][cells] - ref["As"][cells]), 0.99)),
+    dAs = np.abs(g["As"] - ref["As"])
+    return {"solids": int(pick.size), "cells": int(touched.sum()), "candidate_lists_bit_exact": True,
+            "alpha_max_abs": float(max(da_abs)), "alpha_max_rel_alpha_ge_1e-3": float(max(da_rel)),
+            "As_max_abs": float(dAs.max()),
             "Fs_max_rel_above_1e-3_of_max": float(rel.max()), "Fs_p99_rel": float(np.quantile(rel, 0.99)),
             "Fs_max_abs_over_max": float(np.abs(Fg - Fr).max() / np.abs(Fr).max()),
             "force_torque_max_rel": float(np.max(np.abs(ft - ref["ft"]) / scale))}
@@ -474,7 +477,7 @@ def bench_polydisperse(args):
     cloud.set_csg_shapes(sc.csg)
     cloud.set_solids(sc.solids)
     cloud.set_velocity(sc.U)
-    tune = cloud.autotune(sc.dt, args.tune_cache or None)
+    tune = None if args.no_autotune else cloud.autotune(sc.dt, args.tune_cache or None)
     log(f"[bench] autotune: {tune}")
     for _ in range(args.warmup):
         cloud.interact_device(0.0, sc.dt)
@@ -644,6 +647,7 @@ def main():
     ap.add_argument("--workload", default="cloud3d",
                     choices=["cloud3d", "sedimentation2d", "ellipsoids3d_jittered", "polydisperse_csg"])
     ap.add_argument("--tune-cache", default="", help="autotuner cache file (JSON lines); default: time every run")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default variants (profiling runs)")
     ap.add_argument("--loop-steps", type=int, default=None,
                     help="coupled steps timed (sedimentation2d: 200, ellipsoids3d_jittered: 500)")
     ap.add_argument("--contact", type=float, nargs=2, default=None, metavar=("K", "C"),
@@ -717,7 +721,7 @@ def main():
     cloud = SolidCloud(sc.mesh, two_d=sc.two_d, rho_f=sc.rho_f, precision=args.precision)
     cloud.set_solids(sc.solids)
     cloud.set_velocity(sc.U)
-    tune = cloud.autotune(sc.dt, args.tune_cache or None)
+    tune = None if args.no_autotune else cloud.autotune(sc.dt, args.tune_cache or None)
     log(f"[bench] autotune: {tune}")
     for _ in range(args.warmup):
         cloud.interact_device(0.0, sc.dt)
@@ -838,7 +842,7 @@ def main():
         host = host_info()
         om = oracle_mesh(sc)
         if args.precision == "fp32":
-            line["fp32_vs_fp64_oracle"] = fp32_error_statement(cloud, sc, om)
+            line["fp32_vs_fp64_oracle"] = fp32_error_statement(sc, om)
             log(f"[bench] fp32 errors: {line['fp32_vs_fp64_oracle']}")
         rate, ns_t, pairs_t, el = cpu_baseline(sc, args.cpu_sample, args.cpu_budget, om=om)
         line["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",

@@ -105,3 +105,48 @@ def test_coupled_loop_sedimentation(prec):
         # the discs sank, the walls stayed
         assert np.all(got["center"][:16, 1] < sc.solids["center"][:16, 1])
         np.testing.assert_array_equal(got["center"][16:], sc.solids["center"][16:])
+
+
+def test_coupled_loop_3d_jittered_graph():
+    """The 3-D coupled loop as replays of the captured graph (sdfibm_run_steps) on a jittered
+    24^3 mesh (non-planar faces): spheres, ellipsoids and boxes between four fixed planes, with
+    gravity, four steps against the oracle's interact -> U -= Fs dt -> evolve -> fixInternal.
+    Tolerances of the 2-D loop above (each side feeds its own FP64 force sums forward)."""
+    from test_gpu_graph import _walls
+    sc = scenes.small_mixed_3d(24, seed=6, n_each=2, jitter=0.2)
+    keep = np.isin(sc.solids["shape"], [S.SPHERE, S.ELLIPSOID, S.BOX])
+    sol0 = np.concatenate([sc.solids[keep], _walls(0.06)])
+    ns = sol0.size
+    rho = np.linspace(1.4, 2.6, ns)
+    masks = np.full(ns, -1, dtype=np.int32)
+    masks[-4:] = motion_mask("b000000")
+    sol0["vel"][-4:] = 0.0
+    sol0["omega"][-4:] = 0.0
+    g = np.array([0.0, -9.81, 0.0])
+    dt = 1e-3
+    n = 4
+    om = oracle.OracleMesh(sc.mesh.to_polymesh())
+    sol = sol0.copy()
+    dyn = oracle.make_dyn(ns, rho, masks)
+    U = sc.U.copy()
+    for step in range(n):
+        t = step * dt
+        r = om.interact(sol, U, rho_f=sc.rho_f, dt=dt)
+        U = om.update_velocity(U, r["Fs"], dt)
+        sol, dyn = oracle.evolve(sol, dyn, r["ft"], t, dt, g, sc.rho_f)
+        U = om.fix_internal(U, r["Ct"], sol)
+    with SolidCloud(sc.mesh, rho_f=sc.rho_f, precision="fp64") as c:
+        c.set_solids(sol0)
+        c.set_velocity(sc.U)
+        c.set_dynamics(rho, masks)
+        ft = c.run_steps(0.0, dt, n, g)
+        got = c.solids()
+        Ug = c.velocity()
+        Ct = c.fields()["Ct"]
+    np.testing.assert_array_equal(Ct, r["Ct"])
+    scale = np.maximum(np.abs(r["ft"]), 1e-4 * np.abs(r["ft"]).max())
+    assert np.max(np.abs(ft - r["ft"]) / scale) <= 1e-9
+    np.testing.assert_allclose(_state(got), _state(sol), rtol=1e-9, atol=1e-12)
+    assert np.max(np.abs(Ug - U)) <= 1e-9 * max(np.abs(U).max(), 1.0)
+    assert np.any(got["center"][:-4] != sol0["center"][:-4])
+    np.testing.assert_array_equal(got["center"][-4:], sol0["center"][-4:])

@@ -72,7 +72,7 @@ def test_autotune_and_cache(tmp_path):
         ref_ft = c.interact(0.0, 1e-3)
         ref = c.fields()
         a = c.autotune(1e-3, path)
-        assert a["source"] == "timed" and a["variants_timed"] == 8 and a["ms"] > 0
+        assert a["source"] == "timed" and a["variants_timed"] == 9 and a["ms"] > 0
         b = c.autotune(1e-3, path)
         assert b["source"] == "cache" and b["variants_timed"] == 0
         assert (a["cells_wave"], a["big_chunk"], a["big_grid"]) == (b["cells_wave"], b["big_chunk"], b["big_grid"])
