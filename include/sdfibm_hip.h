@@ -344,10 +344,11 @@ int sdfibm_buffer_capacity(sdfibm_ctx* ctx, int64_t* out, int n);
 
 /* Timing of the phases of the last interact (milliseconds, HIP events on the
  * context's stream): [0] setup+binning, [1] per-solid tiles (classification,
- * cut cells and forcing, plane labelling), [2] field pass, [3] total,
- * [4] tile classification alone, [5] cut cells + forcing + plane labelling,
- * [6] the big-tile pass (on the second stream, beside [4] and [5]; 0 without big
- * tiles).  Returns the number of values written (<= n, at most 7). */
+ * cut cells and forcing, plane labelling, and the wait for the big-tile pass),
+ * [2] field pass, [3] total, [4] tile classification alone, [5] cut cells +
+ * forcing of the tiles of the on-chip tile pass, [6] the big-tile pass (on the
+ * second stream, beside [4] and [5]; 0 without big tiles).  Returns the number of
+ * values written (<= n, at most 7). */
 int sdfibm_last_timing(sdfibm_ctx* ctx, double* ms, int n);
 
 /* Run-time variants of the step (no counterpart in the reference): the cut-cell

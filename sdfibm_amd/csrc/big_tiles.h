@@ -16,18 +16,23 @@
 // small tiles' packed (i, j, k) does not fit 80-cell tiles).
 //
 // All big solids of a step are processed together, over flat index spaces
-// (their tile vertices, their tile cells, their candidate chunks) laid out by
-// one scan kernel; no host round trip, so the step stays capturable.
-//   k_big_layout    offsets of each big solid's vertex bits, cells, totals
+// (their tile vertices; their tile cells in blocks of kBigCPB, one workgroup
+// iteration each, so a workgroup finds its solid once; their candidate chunks)
+// laid out by one scan kernel; no host round trip, so the step stays capturable.
+//   k_big_layout    offsets of each big solid's vertex bits, cells, cell blocks, totals
 //   k_big_vertices  strict inside test per tile vertex (FP64) -> bit words
 //   k_big_cells     cells with >= 1 inside vertex -> union-find labels (in the
-//                   candidate-list buffer), lowest such cell
-//   k_big_hook      union over +x, +y, +z neighbours of active cells
+//                   candidate-list buffer): the first cell of their run along x;
+//                   lowest such cell
+//   k_big_hook      union of the runs over +y, +z neighbours (once per overlap)
 //   k_big_compress  labels -> roots
 //   k_big_member    codes: NaN outside the seed's component, +inf ALL_INSIDE,
-//                   0 cut (placeholder); a second look at every edge checks the
-//                   union converged (EF_LABEL otherwise)
-//   k_big_list      candidate lists and counts, one workgroup per solid
+//                   0 cut (placeholder), and each cell block's two counts; a
+//                   second look at every edge checks the union converged
+//                   (EF_LABEL otherwise)
+//   k_big_scan      per solid, the blocks' counts scanned in block order: list
+//                   offsets, the solid's counts and candidate chunks
+//   k_big_write     candidate lists (stable compaction of each block's cells)
 //   k_big_chunks    chunks of candidates (the chunk size is a run-time choice), their offsets
 //   k_big_force     fractions, codes and forcing per chunk; FP64 partial sums
 //   k_big_ft        partials summed in chunk order (x rho_f) -> force/torque
@@ -35,6 +40,7 @@
 
 constexpr int kBigMaxSolids = 4096;  // big tiles per step (SDFIBM_ERR_CAPACITY beyond)
 constexpr int kBigChunk = 1024;      // default candidate entries per chunk of k_big_force (the autotuner's choice)
+constexpr int kBigCPB = 1024;        // tile cells per block of the flat-cell kernels (4 per thread of 256)
 
 struct BigMeta {
     int solid;
@@ -44,11 +50,13 @@ struct BigMeta {
     long long cbase; // first flat cell index
     int chunk0;      // first chunk of its candidate entries
     int nchunk;
+    int blk0;        // first cell block (kBigCPB tile cells each)
+    int nblk;
 };
 
 // Totals of the step's big tiles (BigTot fields as long long):
 // [0] big solids, [1] flat vertices (padded), [2] flat cells, [3] bit words, [4] chunks,
-// [5] bit words needed (for the host's buffer growth), [6] chunks needed
+// [5] bit words needed (for the host's buffer growth), [6] chunks needed, [7] cell blocks
 constexpr int kBigTotN = 8;
 
 // b with base[b] <= t < base[b + 1] (base ascending, n entries, base[0] = 0)
@@ -71,38 +79,41 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
                                                      const int* __restrict__ qmeta, const int* __restrict__ qlist,
                                                      BigMeta* __restrict__ meta, long long* __restrict__ tot,
                                                      long long word_cap, int* __restrict__ err) {
-    __shared__ long long part[2][1024];
+    __shared__ long long part[3][1024];
     const int tid = threadIdx.x;
     int nbig = qmeta[4];
     if (nbig > kBigMaxSolids) {
         if (tid == 0) atomicOr(err, EF_CODECAP);
         nbig = 0;
     }
-    long long w[4], c[4];
-    long long sw = 0, sc = 0;
+    long long w[4], c[4], k[4];
+    long long sw = 0, sc = 0, sk = 0;
     for (int u = 0; u < 4; ++u) {
         const int b = tid * 4 + u;
-        w[u] = c[u] = 0;
+        w[u] = c[u] = k[u] = 0;
         if (b < nbig) {
             const TileInfo T = Ts[qlist[4LL * ns + b]];
             const long long V = (long long)(T.n[0] + 1) * (T.n[1] + 1) * (T.n[2] + 1);
             w[u] = (V + 63) / 64;
             c[u] = (long long)T.n[0] * T.n[1] * T.n[2];
+            k[u] = (c[u] + kBigCPB - 1) / kBigCPB;
         }
         sw += w[u];
         sc += c[u];
+        sk += k[u];
     }
     part[0][tid] = sw;
     part[1][tid] = sc;
+    part[2][tid] = sk;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan of the thread sums
-        const long long a = tid >= off ? part[0][tid - off] : 0, b2 = tid >= off ? part[1][tid - off] : 0;
+        long long a[3];
+        for (int r = 0; r < 3; ++r) a[r] = tid >= off ? part[r][tid - off] : 0;
         __syncthreads();
-        part[0][tid] += a;
-        part[1][tid] += b2;
+        for (int r = 0; r < 3; ++r) part[r][tid] += a[r];
         __syncthreads();
     }
-    long long ow = part[0][tid] - sw, oc = part[1][tid] - sc;
+    long long ow = part[0][tid] - sw, oc = part[1][tid] - sc, ok = part[2][tid] - sk;
     for (int u = 0; u < 4; ++u) {
         const int b = tid * 4 + u;
         if (b < nbig) {
@@ -114,20 +125,26 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
             M.cbase = oc;
             M.chunk0 = 0;
             M.nchunk = 0;
+            M.blk0 = (int)ok;
+            M.nblk = (int)k[u];
             meta[b] = M;
         }
         ow += w[u];
         oc += c[u];
+        ok += k[u];
     }
     if (tid == 1023) {
-        const long long words = part[0][1023], cells = part[1][1023];
-        const bool fits = words <= word_cap;
+        const long long words = part[0][1023], cells = part[1][1023], blocks = part[2][1023];
+        // the block counts live in a buffer of words / 16 + kBigMaxSolids entries (a tile has at
+        // least as many vertices as cells, so blocks <= words * 64 / kBigCPB + big solids)
+        const bool fits = words + 1 <= word_cap && blocks <= word_cap / 16 + kBigMaxSolids;  // + the guard word
         if (!fits) atomicOr(err, EF_CODECAP);
         tot[0] = fits ? nbig : 0;
         tot[1] = fits ? words * 64 : 0;
         tot[2] = fits ? cells : 0;
         tot[3] = words;
         tot[5] = words;
+        tot[7] = fits ? blocks : 0;
     }
 }
 
@@ -171,42 +188,98 @@ __device__ inline int big_nin(const unsigned long long* w, int i, int j, int k, 
            (int)big_vbit(w, v0 + dz + dy + 1);
 }
 
-// Iterates the flat cells of the step's big tiles: f(meta, tile, q, i, j, k).
+// Iterates the cell blocks of the step's big tiles, one workgroup iteration per
+// block (the block's solid found once, uniform): f(b, meta, tile, block, q0, q1)
+// for the tile-local cells [q0, q1).
 template <class F>
-__device__ inline void big_cells_loop(const BigMeta* meta, const TileInfo* Ts, const long long* tot, long long code_cap,
-                                      F f) {
+__device__ inline void big_blocks_loop(const BigMeta* meta, const TileInfo* Ts, const long long* tot, long long code_cap,
+                                       F f) {
     const int nbig = (int)tot[0];
-    const long long nc = tot[2];
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += (long long)gridDim.x * blockDim.x) {
-        const int b = big_find(nbig, t, [&](int x) { return meta[x].cbase; });
-        const BigMeta& M = meta[b];
+    const long long nblk = tot[7];
+    for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int b = big_find(nbig, blk, [&](int x) { return (long long)meta[x].blk0; });
+        const BigMeta M = meta[b];
         const TileInfo T = Ts[M.solid];
-        if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) continue;  // flagged by the tile offsets
-        const int q = (int)(t - M.cbase);
-        int r, i, j, k;
-        divmod_small(q, T.n[0], 1.0f / (float)T.n[0], r, i);
-        divmod_small(r, T.n[1], 1.0f / (float)T.n[1], k, j);
-        f(b, M, T, q, i, j, k);
+        const int C = T.n[0] * T.n[1] * T.n[2];
+        if (T.off + (long long)C > code_cap) continue;  // flagged by the tile offsets
+        const int q0 = (int)(blk - M.blk0) * kBigCPB;
+        f(b, M, T, blk, q0, min(q0 + kBigCPB, C));
     }
 }
 
-// Cells with >= 1 inside vertex: label = own index, else -1; the lowest one.
+// Iterates the flat cells of the step's big tiles: f(b, meta, tile, q, i, j, k).
+template <class F>
+__device__ inline void big_cells_loop(const BigMeta* meta, const TileInfo* Ts, const long long* tot, long long code_cap,
+                                      F f) {
+    big_blocks_loop(meta, Ts, tot, code_cap, [&](int b, const BigMeta& M, const TileInfo& T, long long, int q0, int q1) {
+        const float r0 = 1.0f / (float)T.n[0], r1 = 1.0f / (float)T.n[1];
+        for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+            int r, i, j, k;
+            divmod_small(q, T.n[0], r0, r, i);
+            divmod_small(r, T.n[1], r1, k, j);
+            f(b, M, T, q, i, j, k);
+        }
+    });
+}
+
+// 64 bits of a bit array starting at bit pos (the word after pos's word is read:
+// the vertex-bit buffer keeps one guard word after the last solid's words).
+__device__ inline unsigned long long big_bits64(const unsigned long long* w, long long pos) {
+    const long long wi = pos >> 6;
+    const int sh = (int)(pos & 63);
+    const unsigned long long lo = w[wi] >> sh;
+    return sh == 0 ? lo : (lo | (w[wi + 1] << (64 - sh)));
+}
+
+// First cell of the run of active cells along x that holds active cell i of row
+// (j, k): one past the highest inactive cell below i (0 if none).  Cell p is
+// inactive when none of its 8 vertices is inside: with R(p) the OR of the row's
+// four vertex rows at vertex p, when R(p) = R(p + 1) = 0.  Windows of 63 cells.
+__device__ inline int big_run_start(const unsigned long long* w, int i, int j, int k, int VX, int VY) {
+    const long long r00 = (long long)VX * (j + (long long)VY * k), r10 = r00 + VX;
+    const long long r01 = r00 + (long long)VX * VY, r11 = r01 + VX;
+    int pos = i - 1;  // highest candidate cell
+    while (pos >= 0) {
+        const int lo = pos >= 62 ? pos - 62 : 0;
+        const unsigned long long R = big_bits64(w, r00 + lo) | big_bits64(w, r10 + lo) | big_bits64(w, r01 + lo) |
+                                     big_bits64(w, r11 + lo);
+        // cells lo .. pos inactive where R(p) = R(p + 1) = 0
+        const unsigned long long Z = ~R & ~(R >> 1) & ((2ull << (pos - lo)) - 1ull);
+        if (Z) return lo + (63 - __builtin_clzll(Z)) + 1;
+        if (lo == 0) return 0;
+        pos = lo - 1;
+    }
+    return 0;
+}
+
+// Cells with >= 1 inside vertex: label = the first cell of their run along x
+// (so that the runs start out united), else -1; the lowest such cell (one atomic
+// per wave and block).
 __global__ void __launch_bounds__(256) k_big_cells(const TileInfo* __restrict__ Ts, BigMeta* __restrict__ meta,
                                                    const long long* __restrict__ tot,
                                                    const unsigned long long* __restrict__ vbits, int* __restrict__ cand,
                                                    long long code_cap) {
-    big_cells_loop(meta, Ts, tot, code_cap, [&](int b, const BigMeta& M, const TileInfo& T, int q, int i, int j, int k) {
-        const int nin = big_nin(vbits + M.woff, i, j, k, T.n[0] + 1, T.n[1] + 1);
-        cand[T.off + q] = nin > 0 ? q : -1;
-        // the lowest active cell: the grid walks the cells in ascending order, so after the
-        // first few waves of a solid the plain read already rules out almost every atomic
-        if (nin > 0 && q < __hip_atomic_load(&meta[b].first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMin(&meta[b].first, q);
+    big_blocks_loop(meta, Ts, tot, code_cap, [&](int b, const BigMeta& M, const TileInfo& T, long long, int q0, int q1) {
+        const float r0 = 1.0f / (float)T.n[0], r1 = 1.0f / (float)T.n[1];
+        const unsigned long long* w = vbits + M.woff;
+        int lowest = 0x7fffffff;
+        for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+            int r, i, j, k;
+            divmod_small(q, T.n[0], r0, r, i);
+            divmod_small(r, T.n[1], r1, k, j);
+            const int nin = big_nin(w, i, j, k, T.n[0] + 1, T.n[1] + 1);
+            cand[T.off + q] = nin > 0 ? q - i + big_run_start(w, i, j, k, T.n[0] + 1, T.n[1] + 1) : -1;
+            if (nin > 0 && q < lowest) lowest = q;
+        }
+        for (int off = 32; off >= 1; off >>= 1) lowest = min(lowest, __shfl_xor(lowest, off, 64));
+        if ((threadIdx.x & 63) == 0 && lowest != 0x7fffffff) atomicMin(&meta[b].first, lowest);
     });
 }
 
-// Union over the +x, +y, +z neighbours (hook the larger root under the smaller:
-// the same concurrent union as the planes' labelling).
+// Union of the runs over the +y and +z neighbours (hook the larger root under
+// the smaller: the same concurrent union as the planes' labelling), once per
+// overlap of two runs -- at its first cell, where the cells before the pair are
+// not both active.
 __global__ void __launch_bounds__(256) k_big_hook(const TileInfo* __restrict__ Ts, const BigMeta* __restrict__ meta,
                                                   const long long* __restrict__ tot, int* __restrict__ cand,
                                                   long long code_cap) {
@@ -214,12 +287,11 @@ __global__ void __launch_bounds__(256) k_big_hook(const TileInfo* __restrict__ T
         int* L = cand + T.off;
         if (L[q] < 0) return;
         const int nx = T.n[0], nxy = T.n[0] * T.n[1];
-        int nb[3], nn = 0;
-        if (i + 1 < T.n[0]) nb[nn++] = q + 1;
-        if (j + 1 < T.n[1]) nb[nn++] = q + nx;
-        if (k + 1 < T.n[2]) nb[nn++] = q + nxy;
+        const bool left = i > 0 && L[q - 1] >= 0;
+        int nb[2], nn = 0;
+        if (j + 1 < T.n[1] && L[q + nx] >= 0 && !(left && L[q - 1 + nx] >= 0)) nb[nn++] = q + nx;
+        if (k + 1 < T.n[2] && L[q + nxy] >= 0 && !(left && L[q - 1 + nxy] >= 0)) nb[nn++] = q + nxy;
         for (int u = 0; u < nn; ++u) {
-            if (L[nb[u]] < 0) continue;
             int a = uf_find(L, q), b2 = uf_find(L, nb[u]);
             while (a != b2) {
                 const int hi = a > b2 ? a : b2, lo = a > b2 ? b2 : a;
@@ -242,21 +314,19 @@ __global__ void __launch_bounds__(256) k_big_compress(const TileInfo* __restrict
 }
 
 // Codes of the seed's component; every active cell's label must equal its active
-// neighbours' (a union that did not converge raises EF_LABEL).
+// neighbours' (a union that did not converge raises EF_LABEL).  bcnt[block] =
+// the block's ALL_INSIDE and cut counts.
 template <class R>
 __global__ void __launch_bounds__(256) k_big_member(const MeshDev m, const TileInfo* __restrict__ Ts,
                                                     const BigMeta* __restrict__ meta, const long long* __restrict__ tot,
                                                     const unsigned long long* __restrict__ vbits,
                                                     const int* __restrict__ cand, R* __restrict__ codes,
-                                                    long long code_cap, int* __restrict__ err) {
-    big_cells_loop(meta, Ts, tot, code_cap, [&](int, const BigMeta& M, const TileInfo& T, int q, int i, int j, int k) {
+                                                    int2* __restrict__ bcnt, long long code_cap, int* __restrict__ err) {
+    __shared__ int scan_lds[32];
+    big_blocks_loop(meta, Ts, tot, code_cap, [&](int, const BigMeta& M, const TileInfo& T, long long blk, int q0, int q1) {
         const int* L = cand + T.off;
         R* tc = codes + T.off;
-        const int lq = L[q];
-        if (lq < 0) {
-            tc[q] = code_none<R>();
-            return;
-        }
+        // the seed: the nearest cell when it has an inside vertex, else the lowest such cell
         int seed = -1;
         if (T.seed >= 0) {
             const int si = T.seed % m.nx, sj = (T.seed / m.nx) % m.ny, sk = T.seed / (m.nx * m.ny);
@@ -266,64 +336,106 @@ __global__ void __launch_bounds__(256) k_big_member(const MeshDev m, const TileI
                 if (L[sq] >= 0) seed = sq;
             }
         }
-        if (seed < 0) seed = M.first;  // an active cell exists: this one
-        const bool member = L[seed] == lq;
+        if (seed < 0) seed = M.first;
+        const int lseed = seed >= 0 && seed != 0x7fffffff ? L[seed] : -2;
         const int nx = T.n[0], nxy = T.n[0] * T.n[1];
-        const bool bad = (i + 1 < T.n[0] && L[q + 1] >= 0 && L[q + 1] != lq) ||
-                         (j + 1 < T.n[1] && L[q + nx] >= 0 && L[q + nx] != lq) ||
-                         (k + 1 < T.n[2] && L[q + nxy] >= 0 && L[q + nxy] != lq);
-        if (bad) atomicOr(err, EF_LABEL);
-        if (!member) {
-            tc[q] = code_none<R>();
-        } else {
-            const int nin = big_nin(vbits + M.woff, i, j, k, T.n[0] + 1, T.n[1] + 1);
-            tc[q] = nin == 8 ? R(__builtin_huge_val()) : R(0);
+        const float r0 = 1.0f / (float)T.n[0], r1 = 1.0f / (float)T.n[1];
+        int nall = 0, ncut = 0;
+        for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+            const int lq = L[q];
+            if (lq < 0) {
+                tc[q] = code_none<R>();
+                continue;
+            }
+            int r, i, j, k;
+            divmod_small(q, T.n[0], r0, r, i);
+            divmod_small(r, T.n[1], r1, k, j);
+            const bool bad = (i + 1 < T.n[0] && L[q + 1] >= 0 && L[q + 1] != lq) ||
+                             (j + 1 < T.n[1] && L[q + nx] >= 0 && L[q + nx] != lq) ||
+                             (k + 1 < T.n[2] && L[q + nxy] >= 0 && L[q + nxy] != lq);
+            if (bad) atomicOr(err, EF_LABEL);
+            if (lq != lseed) {
+                tc[q] = code_none<R>();
+            } else {
+                const bool all8 = big_nin(vbits + M.woff, i, j, k, T.n[0] + 1, T.n[1] + 1) == 8;
+                tc[q] = all8 ? R(__builtin_huge_val()) : R(0);
+                nall += all8 ? 1 : 0;
+                ncut += all8 ? 0 : 1;
+            }
         }
+        int pa, pc, ta, tcut;
+        block_exscan2<256>(nall, ncut, pa, pc, ta, tcut, scan_lds);
+        if (threadIdx.x == 0) bcnt[blk] = make_int2(ta, tcut);
+        __syncthreads();  // scan_lds is reused by the next block
     });
 }
 
-// Candidate lists of the big tiles: one workgroup per solid; each thread a
-// contiguous run of the tile's cells; ALL_INSIDE cells first, then cut cells,
-// each ascending (tile-local linear indices).
-template <class R>
-__global__ void __launch_bounds__(1024) k_big_list(const TileInfo* __restrict__ Ts, BigMeta* __restrict__ meta,
-                                                   const long long* __restrict__ tot, const R* __restrict__ codes,
-                                                   int* __restrict__ cand, int2* __restrict__ ncand,
+// Per big solid (one workgroup each): its blocks' counts scanned in block order
+// -> each block's list offsets (ALL_INSIDE, cut), the solid's two counts and
+// its number of candidate chunks.
+__global__ void __launch_bounds__(1024) k_big_scan(const TileInfo* __restrict__ Ts, BigMeta* __restrict__ meta,
+                                                   const long long* __restrict__ tot, const int2* __restrict__ bcnt,
+                                                   int2* __restrict__ boff, int2* __restrict__ ncand,
                                                    long long code_cap, int chunk_entries) {
     __shared__ int scan_lds[32];
     const int nbig = (int)tot[0];
-    const int tid = threadIdx.x;
     for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
-        const int s = meta[b].solid;
-        const TileInfo T = Ts[s];
-        const int C = T.n[0] * T.n[1] * T.n[2];
-        if (T.off + (long long)C > code_cap) continue;
+        const BigMeta M = meta[b];
+        const TileInfo T = Ts[M.solid];
+        if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) continue;  // flagged; uniform
+        int ca = 0, cc = 0;  // carries over rounds of 1024 blocks
+        for (int base = 0; base < M.nblk; base += 1024) {
+            const int t = base + (int)threadIdx.x;
+            const int2 v = t < M.nblk ? bcnt[M.blk0 + t] : make_int2(0, 0);
+            int pa, pc, ta, tc;
+            block_exscan2<1024>(v.x, v.y, pa, pc, ta, tc, scan_lds);
+            if (t < M.nblk) boff[M.blk0 + t] = make_int2(ca + pa, cc + pc);
+            ca += ta;
+            cc += tc;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            ncand[M.solid] = make_int2(ca, cc);
+            meta[b].nchunk = (ca + cc + chunk_entries - 1) / chunk_entries;
+        }
+    }
+}
+
+// Candidate lists of the big tiles: ALL_INSIDE cells first, then cut cells,
+// each ascending (tile-local linear indices); per block a stable compaction of
+// its cells (4 consecutive per thread) at the offsets k_big_scan gave it.
+template <class R>
+__global__ void __launch_bounds__(256) k_big_write(const TileInfo* __restrict__ Ts, const BigMeta* __restrict__ meta,
+                                                   const long long* __restrict__ tot, const R* __restrict__ codes,
+                                                   const int2* __restrict__ boff, const int2* __restrict__ ncand,
+                                                   int* __restrict__ cand, long long code_cap) {
+    static_assert(kBigCPB == 4 * 256, "4 cells per thread");
+    __shared__ int scan_lds[32];
+    big_blocks_loop(meta, Ts, tot, code_cap, [&](int, const BigMeta& M, const TileInfo& T, long long blk, int q0, int q1) {
         const R* tc = codes + T.off;
-        const int chunk = (C + 1023) / 1024;
-        const int q0 = min(tid * chunk, C), q1 = min(q0 + chunk, C);
-        int nin = 0, ncut = 0;
-        for (int q = q0; q < q1; ++q) {
-            const R v = tc[q];
-            nin += (v == R(__builtin_huge_val())) ? 1 : 0;
-            ncut += (v == R(0)) ? 1 : 0;
+        const int qa = q0 + 4 * (int)threadIdx.x;
+        int cls[4];
+        int na = 0, nc = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = qa + u;
+            const R v = q < q1 ? tc[q] : code_none<R>();
+            cls[u] = v == R(__builtin_huge_val()) ? 1 : (v == R(0) ? 2 : 0);
+            na += cls[u] == 1 ? 1 : 0;
+            nc += cls[u] == 2 ? 1 : 0;
         }
-        int pi, pc, tin, tcut;
-        block_exscan2<1024>(nin, ncut, pi, pc, tin, tcut, scan_lds);
-        pc += tin;
+        int pa, pc, ta, tcut;
+        block_exscan2<256>(na, nc, pa, pc, ta, tcut, scan_lds);
+        const int2 o = boff[blk];
         int* tl = cand + T.off;
-        for (int q = q0; q < q1; ++q) {
-            const R v = tc[q];
-            if (v == R(__builtin_huge_val()))
-                tl[pi++] = q;
-            else if (v == R(0))
-                tl[pc++] = q;
-        }
-        if (tid == 0) {
-            ncand[s] = make_int2(tin, tcut);
-            meta[b].nchunk = (tin + tcut + chunk_entries - 1) / chunk_entries;
+        int wa = o.x + pa, wc = ncand[M.solid].x + o.y + pc;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (cls[u] == 1) tl[wa++] = qa + u;
+            if (cls[u] == 2) tl[wc++] = qa + u;
         }
         __syncthreads();
-    }
+    });
 }
 
 // Chunk offsets (one workgroup, exclusive scan over the big solids).

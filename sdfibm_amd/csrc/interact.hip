@@ -802,13 +802,18 @@ __global__ void __launch_bounds__(NT) k_cells(MeshDev m, int ns, const SolidK* _
                                                   const R* __restrict__ Uy, const R* __restrict__ Uz,
                                                   R* __restrict__ codes, long long code_cap, double dt,
                                                   double rho_f, double* __restrict__ ft, const SgNet* __restrict__ sgnet,
-                                                  float sg_lo, float sg_hi, float sg_cos, int* __restrict__ sg_count) {
+                                                  float sg_lo, float sg_hi, float sg_cos, int* __restrict__ sg_count,
+                                                  int part, int lm_cap) {
     __shared__ double red[6 * 16];
     constexpr int NSG = SG ? 1 : 0;
     __shared__ SgNet sn[NSG ? 1 : 1];
     __shared__ unsigned short feat[SG ? NT * kSgPad : 1];
     const int tid = threadIdx.x;
-    const int item = blockIdx.x;
+    // part 0: every item; 1: the large and medium tiles' items below lm_cap; 2: the rest
+    const int n_lm = qmeta[2] + qmeta[1];
+    int item = blockIdx.x;
+    if (part == 1 && item >= min(n_lm, lm_cap)) return;
+    if (part == 2) item += min(n_lm, lm_cap);
     if (item >= qmeta[0] + qmeta[1] + qmeta[2] + qmeta[3]) return;
     const double dtINV = 1.0 / dt;
     const int s = __builtin_amdgcn_readfirstlane(queue_item(qmeta, qlist, ns, item));
@@ -1659,6 +1664,7 @@ struct sdfibm_ctx {
     bool own_stream = false;
     hipStream_t aux = nullptr;                   // second stream of the tile classification
     hipEvent_t fork = nullptr, join = nullptr;  // its dependencies on the main stream
+    hipEvent_t joinc = nullptr;                 // ... the classification of its tiles done
     int* h_flags = nullptr;  // pinned, mapped host words written by k_flags_out: [0] error flags, [1] moved planes
     int* d_hflags = nullptr;  // its device address
     int kind = 0, prec = 0, two_d = 0;
@@ -1753,10 +1759,14 @@ struct sdfibm_ctx {
     bool big_on = false;
     BigMeta* d_big_meta = nullptr;  // kBigMaxSolids
     long long* d_big_tot = nullptr;  // kBigTotN
+    int lm_grid = 0;  // workgroups of the cut-cell pass's 256-thread part (solids whose tile can be medium or large)
     unsigned long long* d_big_vbits = nullptr;
+    int2* d_big_bcnt = nullptr;  // per cell block: ALL_INSIDE and cut counts
+    int2* d_big_boff = nullptr;  // per cell block: its list offsets
     double* d_big_part = nullptr;
     long long big_word_cap = 0, big_chunk_cap = 0;
-    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // [5], [6]: big-tile pass
+    // [0..4] phases of the main stream, [5], [6] the big-tile pass, [7] end of the cut-cell pass
+    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double timing[7] = {0, 0, 0, 0, 0, 0, 0};
     // run-time variants (the autotuner's choices): candidates per chunk and workgroups per CU
     // of the big tiles' cut-cell pass; the small tiles' cut-cell pass one wave per solid
@@ -1907,9 +1917,11 @@ void launch_big(sdfibm_ctx* c, double dt, hipStream_t st) {
     k_big_hook<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand, c->code_cap);
     k_big_compress<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand, c->code_cap);
     k_big_member<R><<<G, 256, 0, st>>>(c->m, c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, codes,
-                                       c->code_cap, c->err);
-    k_big_list<R><<<(unsigned)c->n_cu, 1024, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, codes, c->d_cand,
-                                                      c->d_ncand, c->code_cap, c->big_chunk);
+                                       c->d_big_bcnt, c->code_cap, c->err);
+    k_big_scan<<<(unsigned)c->n_cu, 1024, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_bcnt, c->d_big_boff,
+                                                   c->d_ncand, c->code_cap, c->big_chunk);
+    k_big_write<R><<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, codes, c->d_big_boff, c->d_ncand, c->d_cand,
+                                      c->code_cap);
     k_big_chunks<<<1, 1024, 0, st>>>(c->d_big_meta, c->d_big_tot, c->big_chunk_cap, c->err);
     k_big_force<R, MK, kBlock><<<GF, kBlock, 0, st>>>(c->m, c->d_K, c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand,
                                                      c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
@@ -1961,6 +1973,8 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
             c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
         k_classify<R, MK, kRowsMedium, 1, 1024><<<std::min(g, 4 * cus), 1024, 0, c->aux>>>(
             c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
+        // the cut-cell pass needs these tiles, the field pass also the big ones
+        if (hipEventRecord(c->joinc, c->aux) != hipSuccess) return -1;
         if (c->big_on) {
             if (!graph) (void)hipEventRecord(c->ev[5], c->aux);
             launch_big<R, MK>(c, dt, c->aux);
@@ -1971,36 +1985,42 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
                                                                c->d_cand, c->code_cap, c->d_ncand);
         k_classify<R, MK, kRowsTiny, 3, 64><<<g, 64, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
                                                               c->d_cand, c->code_cap, c->d_ncand);
-        if (hipStreamWaitEvent(st, c->join, 0) != hipSuccess) return -1;
+        if (hipStreamWaitEvent(st, c->joinc, 0) != hipSuccess) return -1;
         if (!graph) (void)hipEventRecord(c->ev[4], st);
         constexpr bool kSgOk = MK == MK_RECT && sizeof(R) == sizeof(float);
-        if (kSgOk && c->d_sg != nullptr) {
-            // same workgroup size as the plain pass: identical reduction order when every cell falls back
-            k_fill_i32<<<1, 64, 0, st>>>(c->d_sg_count, 1, 0);
-            if (c->cells_wave == 1 || (c->cells_wave < 0 && c->small_tiles))
-                k_cells<R, MK, kSgOk, 64><<<g, 64, 0, st>>>(
-                    c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand, c->d_ncand, (const R*)c->U[0],
-                    (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft, c->d_sg,
-                    c->sg_lo, c->sg_hi, c->sg_cos, c->d_sg_count);
-            else
-                k_cells<R, MK, kSgOk, kBlock><<<g, kBlock, 0, st>>>(
-                    c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand, c->d_ncand, (const R*)c->U[0],
-                    (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft, c->d_sg,
-                    c->sg_lo, c->sg_hi, c->sg_cos, c->d_sg_count);
-        } else if (c->cells_wave == 1 || (c->cells_wave < 0 && c->small_tiles)) {
-            // a few dozen candidates per solid: one wave each
-            k_cells<R, MK, false, 64><<<g, 64, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand,
-                                                        c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
-                                                        (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft,
-                                                        nullptr, 0.f, 0.f, 0.f, nullptr);
+        const bool sg = kSgOk && c->d_sg != nullptr;
+        // same workgroup sizes with and without the surrogate: identical reduction order when
+        // every cell falls back
+        if (sg) k_fill_i32<<<1, 64, 0, st>>>(c->d_sg_count, 1, 0);
+        const bool one_wave = c->cells_wave == 1 || (c->cells_wave < 0 && c->small_tiles);
+#define SDFIBM_CELLS(SG_, NT_, GRID_, PART_)                                                                    \
+    k_cells<R, MK, SG_, NT_><<<GRID_, NT_, 0, st>>>(                                                           \
+        c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand, c->d_ncand, (const R*)c->U[0],               \
+        (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft, SG_ ? c->d_sg : nullptr, \
+        c->sg_lo, c->sg_hi, c->sg_cos, SG_ ? c->d_sg_count : nullptr, PART_, c->lm_grid)
+        // one wave per small or tiny tile (a few dozen cut cells each); the large and medium
+        // tiles -- up to lm_grid of them, the number set_solids allows for (beyond it, one
+        // wave each) -- keep 256 threads
+        if (sg) {
+            if (one_wave) {
+                if (c->lm_grid > 0) SDFIBM_CELLS(kSgOk, kBlock, (unsigned)c->lm_grid, 1);
+                SDFIBM_CELLS(kSgOk, 64, g, 2);
+            } else {
+                SDFIBM_CELLS(kSgOk, kBlock, g, 0);
+            }
         } else {
-            k_cells<R, MK, false, kBlock><<<g, kBlock, 0, st>>>(
-                c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, c->d_cand, c->d_ncand, (const R*)c->U[0],
-                (const R*)c->U[1], (const R*)c->U[2], codes, c->code_cap, dt, c->rho_f, c->d_ft, nullptr, 0.f, 0.f,
-                0.f, nullptr);
+            if (one_wave) {
+                if (c->lm_grid > 0) SDFIBM_CELLS(false, kBlock, (unsigned)c->lm_grid, 1);
+                SDFIBM_CELLS(false, 64, g, 2);
+            } else {
+                SDFIBM_CELLS(false, kBlock, g, 0);
+            }
         }
+#undef SDFIBM_CELLS
+        if (!graph) (void)hipEventRecord(c->ev[7], st);
     } else {
         if (!graph) (void)hipEventRecord(c->ev[4], st);
+        if (!graph) (void)hipEventRecord(c->ev[7], st);
     }
     const int np = (int)c->h_planes.size();
     unsigned dirty = 0;
@@ -2049,6 +2069,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
         k_plane_alpha<R, MK><<<g, 256, 0, st>>>(c->m, c->d_plist, c->d_pscalars, c->d_K, sp, member,
                                                 (R*)c->d_pcode + (long long)p * c->N);
     }
+    if (ns > 0 && hipStreamWaitEvent(st, c->join, 0) != hipSuccess) return -1;  // the big-tile pass
     if (!graph) (void)hipEventRecord(c->ev[2], st);
 #define SDFIBM_FIELDS(BX_, BY_, BZ_, CPT_, HP_)                                                                 \
     k_fields<R, MK, BX_, BY_, BZ_, CPT_, HP_><<<dim3(c->nbx, c->nby, c->nbz), BX_ * BY_ * BZ_ / CPT_, 0, st>>>(     \
@@ -2455,7 +2476,8 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
             return bail(SDFIBM_ERR_HIP);
         }
     if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->joinc, hipEventDisableTiming) != hipSuccess) {
         c->msg = "hipEventCreate failed";
         return bail(SDFIBM_ERR_HIP);
     }
@@ -2509,6 +2531,7 @@ int sdfibm_destroy(sdfibm_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
+    if (c->joinc) (void)hipEventDestroy(c->joinc);
     if (c->h_flags) (void)hipHostFree(c->h_flags);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -2609,9 +2632,15 @@ int grow_big(sdfibm_ctx* c, long long words, long long chunks) {
     if (words > c->big_word_cap) {
         graph_reset(c);
         dfree(c, c->d_big_vbits);
+        dfree(c, c->d_big_bcnt);
+        dfree(c, c->d_big_boff);
         c->big_word_cap = 0;
         const long long want = words + words / 4 + 1024;
         if ((rc = dalloc(c, (void**)&c->d_big_vbits, want * 8))) return rc;
+        // cell blocks: at most want / 16 + kBigMaxSolids (k_big_layout checks)
+        const long long blocks = want / 16 + kBigMaxSolids;
+        if ((rc = dalloc(c, (void**)&c->d_big_bcnt, blocks * sizeof(int2)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_big_boff, blocks * sizeof(int2)))) return rc;
         c->big_word_cap = want;
     }
     if (chunks > c->big_chunk_cap) {
@@ -2633,7 +2662,7 @@ int grow_big(sdfibm_ctx* c, long long words, long long chunks) {
 int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     graph_reset(c);
     std::vector<int> planes;
-    long long cells = 0, tile_sum = 0, n_tiles = 0, big_words = 0, big_chunks = 0;
+    long long cells = 0, tile_sum = 0, n_tiles = 0, big_words = 0, big_chunks = 0, n_lm = 0;
     bool big_on = false;
     int hist[34] = {0};  // tiles per power of two of their cell count; [32] planes and oversized, [33] big
     double hmin[3];
@@ -2665,6 +2694,9 @@ int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
                 for (int d = 0; d < 3; ++d)
                     if (std::isfinite(R)) e[d] = std::min<long long>(nn[d], (long long)std::ceil(2.0 * R / hmin[d]) + 2);
             }
+            if (e[0] <= kMaxTileX && (e[1] + 1) * (e[2] + 1) > kRowsSmall &&
+                (e[1] + 1) * (e[2] + 1) <= kRowsLarge)
+                ++n_lm;  // a medium or large tile in some pose
             if (e[0] > kMaxTileX || (e[1] + 1) * (e[2] + 1) > kRowsLarge) {
                 big_on = true;
                 big_words += (e[0] + 2) * (e[1] + 2) * (e[2] + 2) / 64 + 2;
@@ -2752,6 +2784,7 @@ int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     }
     c->h_planes = planes;
     c->small_tiles = n_tiles > 0 && tile_sum < 512 * n_tiles;
+    c->lm_grid = (int)std::min<long long>(n_lm, n);
     c->big_on = big_on;
     if (big_on && (rc = grow_big(c, big_words, big_chunks))) return rc;
     {
@@ -2899,7 +2932,7 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     (void)hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]);
     (void)hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]);
     (void)hipEventElapsedTime(&ms[3], c->ev[1], c->ev[4]);
-    (void)hipEventElapsedTime(&ms[4], c->ev[4], c->ev[2]);
+    (void)hipEventElapsedTime(&ms[4], c->ev[4], c->ev[7]);
     c->timing[0] = ms[0];
     c->timing[1] = ms[1];
     c->timing[2] = ms[2];
@@ -3506,14 +3539,14 @@ struct Tuning {
     double ms;
 };
 
-// Device time of the step (sum of its phases on the main stream) under the context's
-// current variant: the median of `reps` interacts after two untimed ones.
-int time_step(sdfibm_ctx* c, double dt, int reps, double& ms) {
+// Device time of one phase of the step (an index of sdfibm_ctx::timing) under the
+// context's current variant: the median of `reps` interacts after two untimed ones.
+int time_step(sdfibm_ctx* c, double dt, int reps, int phase, double& ms) {
     std::vector<double> t;
     for (int r = 0; r < reps + 2; ++r) {
         const int rc = sdfibm_interact(c, 0.0, dt, nullptr);
         if (rc) return rc;
-        if (r >= 2) t.push_back(c->timing[3]);
+        if (r >= 2) t.push_back(c->timing[phase]);
     }
     std::sort(t.begin(), t.end());
     ms = t[t.size() / 2];
@@ -3571,31 +3604,40 @@ int sdfibm_autotune(sdfibm_ctx* c, double dt, const char* cache_path, char* chos
     if (cache_path && cache_lookup(cache_path, c->tile_key, best)) {
         source = "cache";
     } else {
-        // coordinate search, one knob at a time: the small tiles' cut-cell pass, then (with big
-        // tiles) the big pass's chunk size, then its grid
-        auto run = [&](Tuning t, double& ms) {
+        // per size bin, one knob at a time, each timed by its own phase: the small tiles'
+        // cut-cell pass by its phase (timing[5]), then (with big tiles) the big pass's chunk
+        // size and grid by the big pass (timing[6], on the second stream); the choice's step
+        // time is measured last
+        auto run = [&](Tuning t, int phase, double& ms) {
             if ((rc = sdfibm_set_tuning(c, t.cells_wave, t.big_chunk, t.big_grid))) return rc;
             ++tried;
-            return time_step(c, dt, 5, ms);
+            return time_step(c, dt, 5, phase, ms);
         };
-        best.ms = 1e300;
+        double bin = 1e300;
+        best = Tuning{0, kBigChunk, 4, 0.0};
         for (int w : {0, 1}) {
             Tuning t{w, kBigChunk, 4, 0.0};
-            if ((rc = run(t, t.ms))) return rc;
-            if (t.ms < best.ms) best = t;
+            double ms;
+            if ((rc = run(t, 5, ms))) return rc;
+            if (ms < bin) bin = ms, best = t;
         }
         if (c->big_on) {
-            for (int ch : {256, 512, 2048, 4096}) {
+            bin = 1e300;
+            for (int ch : {256, 512, 1024, 2048, 4096}) {
                 Tuning t{best.cells_wave, ch, best.big_grid, 0.0};
-                if ((rc = run(t, t.ms))) return rc;
-                if (t.ms < best.ms) best = t;
+                double ms;
+                if ((rc = run(t, 6, ms))) return rc;
+                if (ms < bin) bin = ms, best = t;
             }
             for (int g : {2, 8}) {
                 Tuning t{best.cells_wave, best.big_chunk, g, 0.0};
-                if ((rc = run(t, t.ms))) return rc;
-                if (t.ms < best.ms) best = t;
+                double ms;
+                if ((rc = run(t, 6, ms))) return rc;
+                if (ms < bin) bin = ms, best = t;
             }
         }
+        if ((rc = sdfibm_set_tuning(c, best.cells_wave, best.big_chunk, best.big_grid))) return rc;
+        if ((rc = time_step(c, dt, 5, 3, best.ms))) return rc;
         if (cache_path) {
             if (FILE* f = std::fopen(cache_path, "a")) {
                 std::fprintf(f, "{\"key\": \"%s\", \"cells_wave\": %d, \"big_chunk\": %d, \"big_grid\": %d, \"ms\": %.6f}\n",
