@@ -104,7 +104,8 @@ typedef struct {
 
 typedef enum {
     SDFIBM_MESH_RECTILINEAR = 0, /* points = xs x ys x zs tensor product; geometry in closed form */
-    SDFIBM_MESH_STRUCTURED = 1   /* one hex block with arbitrary point positions; geometry given */
+    SDFIBM_MESH_STRUCTURED = 1,  /* one hex block with arbitrary point positions; geometry given */
+    SDFIBM_MESH_POLY = 2         /* a general polyMesh (sdfibm_create_polymesh) */
 } sdfibm_mesh_kind;
 
 /* A logically structured hex mesh (one blockMesh block): points numbered
@@ -127,6 +128,29 @@ typedef struct {
     const double* face_areas[3];  /* same layout; area vectors (mesh.Sf()), sign ignored */
 } sdfibm_mesh_desc;
 
+/* A general polyhedral mesh: OpenFOAM's polyMesh addressing (points, faces,
+ * owner, neighbour) and the geometry OpenFOAM derives from it.  Replaces
+ * MeshInfo (src/meshinfo.h:11-29) for any mesh the reference runs on: several
+ * blockMesh blocks, O-grids, snappyHexMesh output.  The library derives cells()
+ * (each cell's owned faces ascending, then its neighbour faces ascending, as
+ * primitiveMesh::calcCells lists them), cellPoints (ascending point labels) and
+ * cellCells from the addressing.  The four geometry arrays may be NULL: they are
+ * then computed as primitiveMesh does (face centres and area vectors from the
+ * triangles fanned about the face's point average; cell centres and volumes from
+ * the face pyramids about the average of the cell's face centres). */
+typedef struct {
+    int64_t n_points, n_faces, n_internal_faces, n_cells;
+    const double* points;         /* [n_points*3]                                    (mesh.points()) */
+    const int64_t* face_offsets;  /* [n_faces+1]: face f's points are                              */
+    const int32_t* face_points;   /*   face_points[face_offsets[f] .. face_offsets[f+1])  (mesh.faces()) */
+    const int32_t* owner;         /* [n_faces]                                       (mesh.faceOwner()) */
+    const int32_t* neighbour;     /* [n_internal_faces]                              (mesh.faceNeighbour()) */
+    const double* cell_centres;   /* [n_cells*3] or NULL                             (mesh.C()) */
+    const double* cell_volumes;   /* [n_cells] or NULL                               (mesh.V()) */
+    const double* face_centres;   /* [n_faces*3] or NULL                             (mesh.Cf()) */
+    const double* face_areas;     /* [n_faces*3] or NULL                             (mesh.Sf()) */
+} sdfibm_polymesh_desc;
+
 typedef enum { SDFIBM_FP32 = 0, SDFIBM_FP64 = 1 } sdfibm_precision;
 
 typedef struct {
@@ -143,13 +167,22 @@ typedef struct sdfibm_ctx sdfibm_ctx;
  * once and allocates U, As, Fs, Ts, Ct.  Replaces the SolidCloud constructor's
  * field binding (solidcloud.cpp:209-217) and MeshInfo (meshinfo.h:22-29). */
 int sdfibm_create(const sdfibm_mesh_desc* mesh, const sdfibm_config* cfg, sdfibm_ctx** out);
+
+/* The same for a general polyhedral mesh (the constructor's MeshInfo binding,
+ * meshinfo.h:22-29, for any OpenFOAM mesh).  Every call below works on such a
+ * context except sdfibm_run_steps, sdfibm_interact_steps, sdfibm_autotune and the
+ * surrogate (SDFIBM_ERR_STATE): its step reads two counts back to size its
+ * buffers, so it is not captured as a graph.  sdfibm_last_timing: [0] the
+ * (solid, cell) pairs, their labels and force/torque, [1] the per-cell lists,
+ * [2] the field pass, [3] total. */
+int sdfibm_create_polymesh(const sdfibm_polymesh_desc* mesh, const sdfibm_config* cfg, sdfibm_ctx** out);
 int sdfibm_destroy(sdfibm_ctx* ctx);
 
 /* Upload the solids (index order = the reference's m_solids order,
  * solidcloud.h:49).  Validates shape ids and parameters.  Solids of any size:
- * tiles beyond the on-chip tile pass (more than 63 cells along x or 4096 vertex
- * rows) take the big-tile pass; beyond 2^22 tile vertices a solid takes the
- * planes' path (<= 32 such plus planes). */
+ * tiles beyond the on-chip tile pass (more than 63 cells along x or 1024 vertex
+ * rows; at most 16384 such solids) take the big-tile pass; beyond 2^22 tile
+ * vertices a solid takes the planes' path (<= 32 such plus planes). */
 int sdfibm_set_solids(sdfibm_ctx* ctx, const sdfibm_solid* solids, int64_t n);
 
 /* The CSG shape table the context's CSG solids refer to (copied).  The solids

@@ -21,14 +21,14 @@ EXPORTS = (
     "sdfibm_surrogate_train", "sdfibm_surrogate_samples", "sdfibm_surrogate_eval", "sdfibm_surrogate_grad",
     "sdfibm_set_surrogate", "sdfibm_surrogate_count", "sdfibm_buffer_capacity", "sdfibm_set_csg_shapes",
     "sdfibm_set_velocity_cells_noalloc", "sdfibm_set_tuning", "sdfibm_autotune",
-    "sdfibm_interact_steps",
+    "sdfibm_interact_steps", "sdfibm_create_polymesh",
 )
 
 SDFIBM_OK = 0
 NOALLOC_MAX_CELLS = 1048576  # SDFIBM_NOALLOC_MAX_CELLS
 STATUS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_SHAPE", -5: "ERR_TILE",
           -6: "ERR_STATE", -7: "ERR_MESH", -8: "ERR_CAPACITY", -9: "ERR_NODEVICE"}
-MESH_RECTILINEAR, MESH_STRUCTURED = 0, 1
+MESH_RECTILINEAR, MESH_STRUCTURED, MESH_POLY = 0, 1, 2
 FP32, FP64 = 0, 1
 
 
@@ -37,6 +37,14 @@ class MeshDesc(C.Structure):
                 ("xs", C.c_void_p), ("ys", C.c_void_p), ("zs", C.c_void_p),
                 ("points", C.c_void_p), ("cell_centres", C.c_void_p), ("cell_volumes", C.c_void_p),
                 ("face_centres", C.c_void_p * 3), ("face_areas", C.c_void_p * 3)]
+
+
+class PolyMeshDesc(C.Structure):
+    _fields_ = [("n_points", C.c_int64), ("n_faces", C.c_int64), ("n_internal_faces", C.c_int64),
+                ("n_cells", C.c_int64), ("points", C.c_void_p), ("face_offsets", C.c_void_p),
+                ("face_points", C.c_void_p), ("owner", C.c_void_p), ("neighbour", C.c_void_p),
+                ("cell_centres", C.c_void_p), ("cell_volumes", C.c_void_p), ("face_centres", C.c_void_p),
+                ("face_areas", C.c_void_p)]
 
 
 class Config(C.Structure):
@@ -102,6 +110,7 @@ def load():
         "sdfibm_set_tuning": (i32, [P, i32, i32, i32]),
         "sdfibm_autotune": (i32, [P, d, C.c_char_p, C.c_char_p, i64]),
         "sdfibm_interact_steps": (i32, [P, d, d, i64, P]),
+        "sdfibm_create_polymesh": (i32, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

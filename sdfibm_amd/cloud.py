@@ -14,6 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
+from .foam_io import PolyMesh
 from .mesh import StructuredHexMesh
 from .solids import CSG_DTYPE, SOLID_DTYPE
 
@@ -39,19 +40,30 @@ class SolidCloud:
     """Coupling-step context on the current GPU.
 
     mesh       StructuredHexMesh (rectilinear → closed-form geometry on the device; otherwise
-               the host geometry arrays are uploaded once)
+               the host geometry arrays are uploaded once), or a PolyMesh (foam_io: a general
+               polyMesh, e.g. several blockMesh blocks; the library derives its addressing and
+               geometry)
     two_d      meta.on_twod
     rho_f      fluid density (0 reproduces on_fluid = 0)
     precision  "fp64" (parity mode) or "fp32" (resident fields and fractions in FP32; the
                inside tests and the force/torque reduction stay FP64)
     """
 
-    def __init__(self, mesh: StructuredHexMesh, *, two_d: bool = False, rho_f: float = 1.0,
+    def __init__(self, mesh, *, two_d: bool = False, rho_f: float = 1.0,
                  precision: str = "fp64", force_structured: bool = False):
         L = _lib.load()
         self.mesh = mesh
         self.n_cells = mesh.n_cells
         self._keep = []
+        cfg = _lib.Config()
+        cfg.two_d = 1 if two_d else 0
+        cfg.precision = _lib.FP64 if precision == "fp64" else _lib.FP32
+        cfg.rho_f = float(rho_f)
+        self.precision = precision
+        self.n_solids = 0
+        if isinstance(mesh, PolyMesh):
+            self._create_poly(L, mesh, cfg)
+            return
         d = _lib.MeshDesc()
         d.nx, d.ny, d.nz = mesh.nx, mesh.ny, mesh.nz
         if mesh.rectilinear and not force_structured:
@@ -74,17 +86,26 @@ class SolidCloud:
             for f, key in enumerate("xyz"):
                 d.face_centres[f] = arrs[f"f{key}c"].ctypes.data
                 d.face_areas[f] = arrs[f"f{key}s"].ctypes.data
-        cfg = _lib.Config()
-        cfg.two_d = 1 if two_d else 0
-        cfg.precision = _lib.FP64 if precision == "fp64" else _lib.FP32
-        cfg.rho_f = float(rho_f)
-        self.precision = precision
         h = C.c_void_p()
         st = L.sdfibm_create(C.byref(d), C.byref(cfg), C.byref(h))
         _lib.check(st, None)
         self._h = h
         self._keep = []  # the library copied the mesh
-        self.n_solids = 0
+
+    def _create_poly(self, L, pm: PolyMesh, cfg):
+        pts = np.ascontiguousarray(pm.points, dtype=np.float64)
+        fo = np.ascontiguousarray(pm.face_offsets, dtype=np.int64)
+        fp = np.ascontiguousarray(pm.face_points, dtype=np.int32)
+        own = np.ascontiguousarray(pm.owner, dtype=np.int32)
+        nei = np.ascontiguousarray(pm.neighbour, dtype=np.int32)
+        d = _lib.PolyMeshDesc()
+        d.n_points, d.n_faces, d.n_internal_faces, d.n_cells = pts.shape[0], own.size, nei.size, pm.n_cells
+        d.points, d.face_offsets, d.face_points = pts.ctypes.data, fo.ctypes.data, fp.ctypes.data
+        d.owner, d.neighbour = own.ctypes.data, nei.ctypes.data if nei.size else None
+        h = C.c_void_p()
+        st = L.sdfibm_create_polymesh(C.byref(d), C.byref(cfg), C.byref(h))
+        _lib.check(st, None)
+        self._h = h
 
     # -- lifecycle ---------------------------------------------------------------------------
     def close(self):

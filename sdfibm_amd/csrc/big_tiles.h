@@ -1,6 +1,8 @@
 // big_tiles.h -- the tile pass for solids whose cell tile is too large for the
-// LDS bit rows of k_classify (more than 63 cells along x or more than 4096
-// vertex rows), e.g. the 40-80-cell CSG solids of a polydisperse cloud.
+// LDS bit rows of k_classify (more than 63 cells along x or more than 1024
+// vertex rows), e.g. the 40-80-cell CSG solids of a polydisperse cloud: one
+// workgroup per solid would make such a tile the step's long pole, so these
+// tiles are spread over the whole chip.
 //
 // Included by interact.hip, inside its kernel namespace, after the helpers it
 // uses (divmod_small, block_exscan2, block_reduce, uf_find, code_none,
@@ -38,9 +40,11 @@
 //   k_big_ft        partials summed in chunk order (x rho_f) -> force/torque
 #pragma once
 
-constexpr int kBigMaxSolids = 4096;  // big tiles per step (SDFIBM_ERR_CAPACITY beyond)
+constexpr int kBigPer = 16;                  // big solids per thread of the one-workgroup scans
+constexpr int kBigMaxSolids = 1024 * kBigPer;  // big tiles per step (SDFIBM_ERR_CAPACITY beyond)
 constexpr int kBigChunk = 1024;      // default candidate entries per chunk of k_big_force (the autotuner's choice)
 constexpr int kBigCPB = 1024;        // tile cells per block of the flat-cell kernels (4 per thread of 256)
+constexpr int kBigWPB = 32;          // vertex bit words per block of k_big_vertices (8 per wave of 4)
 
 struct BigMeta {
     int solid;
@@ -52,12 +56,15 @@ struct BigMeta {
     int nchunk;
     int blk0;        // first cell block (kBigCPB tile cells each)
     int nblk;
+    int vblk0;       // first vertex block (kBigWPB bit words each)
+    int pad;
 };
 
 // Totals of the step's big tiles (BigTot fields as long long):
 // [0] big solids, [1] flat vertices (padded), [2] flat cells, [3] bit words, [4] chunks,
-// [5] bit words needed (for the host's buffer growth), [6] chunks needed, [7] cell blocks
-constexpr int kBigTotN = 8;
+// [5] bit words needed (for the host's buffer growth), [6] chunks needed, [7] cell blocks,
+// [8] vertex blocks
+constexpr int kBigTotN = 10;
 
 // b with base[b] <= t < base[b + 1] (base ascending, n entries, base[0] = 0)
 template <class F>
@@ -73,49 +80,50 @@ __device__ inline int big_find(int n, long long t, F base) {
     return lo;
 }
 
-// Layout of the step's big tiles: one workgroup of 1024 threads, up to 4 big
-// solids per thread, exclusive scans of vertex words, cells.
+// Layout of the step's big tiles: one workgroup of 1024 threads, up to kBigPer
+// big solids per thread, exclusive scans of vertex words, cells, cell blocks.
 __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __restrict__ Ts,
                                                      const int* __restrict__ qmeta, const int* __restrict__ qlist,
                                                      BigMeta* __restrict__ meta, long long* __restrict__ tot,
                                                      long long word_cap, int* __restrict__ err) {
-    __shared__ long long part[3][1024];
+    __shared__ long long part[4][1024];
     const int tid = threadIdx.x;
     int nbig = qmeta[4];
     if (nbig > kBigMaxSolids) {
         if (tid == 0) atomicOr(err, EF_CODECAP);
         nbig = 0;
     }
-    long long w[4], c[4], k[4];
-    long long sw = 0, sc = 0, sk = 0;
-    for (int u = 0; u < 4; ++u) {
-        const int b = tid * 4 + u;
-        w[u] = c[u] = k[u] = 0;
+    long long w[kBigPer], c[kBigPer];
+    long long sw = 0, sc = 0, sk = 0, sv = 0;
+    for (int u = 0; u < kBigPer; ++u) {
+        const int b = tid * kBigPer + u;
+        w[u] = c[u] = 0;
         if (b < nbig) {
             const TileInfo T = Ts[qlist[4LL * ns + b]];
             const long long V = (long long)(T.n[0] + 1) * (T.n[1] + 1) * (T.n[2] + 1);
             w[u] = (V + 63) / 64;
             c[u] = (long long)T.n[0] * T.n[1] * T.n[2];
-            k[u] = (c[u] + kBigCPB - 1) / kBigCPB;
         }
         sw += w[u];
         sc += c[u];
-        sk += k[u];
+        sk += (c[u] + kBigCPB - 1) / kBigCPB;
+        sv += (w[u] + kBigWPB - 1) / kBigWPB;
     }
     part[0][tid] = sw;
     part[1][tid] = sc;
     part[2][tid] = sk;
+    part[3][tid] = sv;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan of the thread sums
-        long long a[3];
-        for (int r = 0; r < 3; ++r) a[r] = tid >= off ? part[r][tid - off] : 0;
+        long long a[4];
+        for (int r = 0; r < 4; ++r) a[r] = tid >= off ? part[r][tid - off] : 0;
         __syncthreads();
-        for (int r = 0; r < 3; ++r) part[r][tid] += a[r];
+        for (int r = 0; r < 4; ++r) part[r][tid] += a[r];
         __syncthreads();
     }
-    long long ow = part[0][tid] - sw, oc = part[1][tid] - sc, ok = part[2][tid] - sk;
-    for (int u = 0; u < 4; ++u) {
-        const int b = tid * 4 + u;
+    long long ow = part[0][tid] - sw, oc = part[1][tid] - sc, ok = part[2][tid] - sk, ov = part[3][tid] - sv;
+    for (int u = 0; u < kBigPer; ++u) {
+        const int b = tid * kBigPer + u;
         if (b < nbig) {
             BigMeta M;
             M.solid = qlist[4LL * ns + b];
@@ -126,15 +134,18 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
             M.chunk0 = 0;
             M.nchunk = 0;
             M.blk0 = (int)ok;
-            M.nblk = (int)k[u];
+            M.nblk = (int)((c[u] + kBigCPB - 1) / kBigCPB);
+            M.vblk0 = (int)ov;
+            M.pad = 0;
             meta[b] = M;
         }
         ow += w[u];
         oc += c[u];
-        ok += k[u];
+        ok += (c[u] + kBigCPB - 1) / kBigCPB;
+        ov += (w[u] + kBigWPB - 1) / kBigWPB;
     }
     if (tid == 1023) {
-        const long long words = part[0][1023], cells = part[1][1023], blocks = part[2][1023];
+        const long long words = part[0][1023], cells = part[1][1023], blocks = part[2][1023], vblocks = part[3][1023];
         // the block counts live in a buffer of words / 16 + kBigMaxSolids entries (a tile has at
         // least as many vertices as cells, so blocks <= words * 64 / kBigCPB + big solids)
         const bool fits = words + 1 <= word_cap && blocks <= word_cap / 16 + kBigMaxSolids;  // + the guard word
@@ -145,35 +156,43 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
         tot[3] = words;
         tot[5] = words;
         tot[7] = fits ? blocks : 0;
+        tot[8] = fits ? vblocks : 0;
     }
 }
 
 // Strict inside test at every big-tile vertex (FP64, the reference's operation
-// order, as k_classify); a wave covers one 64-bit word of one solid.
+// order, as k_classify): blocks of kBigWPB bit words of one solid per workgroup
+// iteration (the solid found once), a wave per word (one ballot).
 template <int MK>
 __global__ void __launch_bounds__(256) k_big_vertices(MeshDev m, const SolidK* __restrict__ Ks,
                                                       const TileInfo* __restrict__ Ts, const BigMeta* __restrict__ meta,
                                                       const long long* __restrict__ tot,
                                                       unsigned long long* __restrict__ vbits) {
     const int nbig = (int)tot[0];
-    const long long nv = tot[1];
-    const int lane = threadIdx.x & 63;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t - lane < nv;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int b = big_find(nbig, t, [&](int x) { return meta[x].vbase; });  // wave-uniform
+    const long long nvb = tot[8];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (long long blk = blockIdx.x; blk < nvb; blk += gridDim.x) {
+        const int b = big_find(nbig, blk, [&](int x) { return (long long)meta[x].vblk0; });  // uniform
         const BigMeta M = meta[b];
         const TileInfo T = Ts[M.solid];
+        const SolidK K = Ks[M.solid];
         const int VX = T.n[0] + 1, VY = T.n[1] + 1;
-        const long long v = t - M.vbase;
-        bool in = false;
-        if (v < (long long)VX * VY * (T.n[2] + 1)) {
-            int r, vi, vj, vk;
-            divmod_small((int)v, VX, 1.0f / (float)VX, r, vi);
-            divmod_small(r, VY, 1.0f / (float)VY, vk, vj);
-            in = inside_world(Ks[M.solid], vertex_pos<MK>(m, T.lo[0] + vi, T.lo[1] + vj, T.lo[2] + vk));
+        const long long V = (long long)VX * VY * (T.n[2] + 1), words = (V + 63) / 64;
+        const float rvx = 1.0f / (float)VX, rvy = 1.0f / (float)VY;
+        for (int r = 0; r < kBigWPB / 4; ++r) {
+            const long long word = (blk - M.vblk0) * kBigWPB + 4 * r + wv;
+            if (word >= words) break;  // wave-uniform
+            const long long v = word * 64 + lane;
+            bool in = false;
+            if (v < V) {
+                int q, vi, vj, vk;
+                divmod_small((int)v, VX, rvx, q, vi);
+                divmod_small(q, VY, rvy, vk, vj);
+                in = inside_world(K, vertex_pos<MK>(m, T.lo[0] + vi, T.lo[1] + vj, T.lo[2] + vk));
+            }
+            const unsigned long long bal = __ballot(in);
+            if (lane == 0) vbits[M.woff + word] = bal;
         }
-        const unsigned long long bal = __ballot(in);
-        if (lane == 0) vbits[M.woff + v / 64] = bal;
     }
 }
 
@@ -254,7 +273,10 @@ __device__ inline int big_run_start(const unsigned long long* w, int i, int j, i
 
 // Cells with >= 1 inside vertex: label = the first cell of their run along x
 // (so that the runs start out united), else -1; the lowest such cell (one atomic
-// per wave and block).
+// per wave and block).  A wave holds 64 consecutive cells: a run that starts in
+// the wave is found from the wave's ballot of active cells and of row starts; the
+// one run that may come in from before the wave is traced back from the vertex
+// bits once (big_run_start).
 __global__ void __launch_bounds__(256) k_big_cells(const TileInfo* __restrict__ Ts, BigMeta* __restrict__ meta,
                                                    const long long* __restrict__ tot,
                                                    const unsigned long long* __restrict__ vbits, int* __restrict__ cand,
@@ -262,17 +284,33 @@ __global__ void __launch_bounds__(256) k_big_cells(const TileInfo* __restrict__ 
     big_blocks_loop(meta, Ts, tot, code_cap, [&](int b, const BigMeta& M, const TileInfo& T, long long, int q0, int q1) {
         const float r0 = 1.0f / (float)T.n[0], r1 = 1.0f / (float)T.n[1];
         const unsigned long long* w = vbits + M.woff;
+        const int lane = threadIdx.x & 63;
         int lowest = 0x7fffffff;
-        for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
-            int r, i, j, k;
-            divmod_small(q, T.n[0], r0, r, i);
-            divmod_small(r, T.n[1], r1, k, j);
-            const int nin = big_nin(w, i, j, k, T.n[0] + 1, T.n[1] + 1);
-            cand[T.off + q] = nin > 0 ? q - i + big_run_start(w, i, j, k, T.n[0] + 1, T.n[1] + 1) : -1;
-            if (nin > 0 && q < lowest) lowest = q;
+        for (int base = q0; base < q1; base += blockDim.x) {  // the same trip count for every thread
+            const int q = base + (int)threadIdx.x;
+            const bool live = q < q1;
+            int i = 0, j = 0, k = 0, nin = 0;
+            if (live) {
+                int r;
+                divmod_small(q, T.n[0], r0, r, i);
+                divmod_small(r, T.n[1], r1, k, j);
+                nin = big_nin(w, i, j, k, T.n[0] + 1, T.n[1] + 1);
+            }
+            const bool act = nin > 0;
+            const unsigned long long am = __ballot(act), rs = __ballot(live && i == 0);
+            // lanes where a run starts: active, and the cell before is inactive or in the row
+            // before (lane 0's cell before lies outside the wave: a start only at a row start)
+            const unsigned long long starts = am & ((~(am << 1) & ~1ull) | rs);
+            const unsigned long long mine = starts & ((2ull << lane) - 1ull);
+            // the run entering the wave at lane 0, traced back once
+            int enter = 0;
+            if (act && lane == 0 && !mine) enter = q - i + big_run_start(w, i, j, k, T.n[0] + 1, T.n[1] + 1);
+            enter = __shfl(enter, 0, 64);
+            if (live) cand[T.off + q] = !act ? -1 : (mine ? base + (int)(threadIdx.x & ~63u) + (63 - __builtin_clzll(mine)) : enter);
+            if (act && q < lowest) lowest = q;
         }
         for (int off = 32; off >= 1; off >>= 1) lowest = min(lowest, __shfl_xor(lowest, off, 64));
-        if ((threadIdx.x & 63) == 0 && lowest != 0x7fffffff) atomicMin(&meta[b].first, lowest);
+        if (lane == 0 && lowest != 0x7fffffff) atomicMin(&meta[b].first, lowest);
     });
 }
 
@@ -444,9 +482,9 @@ __global__ void __launch_bounds__(1024) k_big_chunks(BigMeta* __restrict__ meta,
     __shared__ int part[1024];
     const int nbig = (int)tot[0];
     const int tid = threadIdx.x;
-    int c[4], sum = 0;
-    for (int u = 0; u < 4; ++u) {
-        const int b = tid * 4 + u;
+    int c[kBigPer], sum = 0;
+    for (int u = 0; u < kBigPer; ++u) {
+        const int b = tid * kBigPer + u;
         c[u] = b < nbig ? meta[b].nchunk : 0;
         sum += c[u];
     }
@@ -459,8 +497,8 @@ __global__ void __launch_bounds__(1024) k_big_chunks(BigMeta* __restrict__ meta,
         __syncthreads();
     }
     int o = part[tid] - sum;
-    for (int u = 0; u < 4; ++u) {
-        const int b = tid * 4 + u;
+    for (int u = 0; u < kBigPer; ++u) {
+        const int b = tid * kBigPer + u;
         if (b < nbig) meta[b].chunk0 = o;
         o += c[u];
     }

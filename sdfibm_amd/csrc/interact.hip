@@ -150,12 +150,12 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
 }
 
 // Work queues of the per-solid tile pass, one per tile class (0 small, 1 medium,
-// 2 large; k_setup moves small tiles of at most kRowsTiny vertex rows to 3, tiny;
+// 2 unused; k_setup moves small tiles of at most kRowsTiny vertex rows to 3, tiny;
 // 4 big, the big-tile pass): qmeta[q] = number of solids queued, qlist[q * ns + t]
 // = the t-th of them.
 constexpr int kQueues = 5;
 __host__ __device__ inline int tile_queue(int cls) {
-    return cls == TC_SMALL ? 0 : (cls == TC_MEDIUM ? 1 : (cls == TC_LARGE ? 2 : (cls == TC_BIG ? 4 : -1)));
+    return cls == TC_SMALL ? 0 : (cls == TC_MEDIUM ? 1 : (cls == TC_BIG ? 4 : -1));
 }
 
 template <int MK>
@@ -476,7 +476,7 @@ __device__ inline int queue_item(const int* qmeta, const int* qlist, int ns, int
 }
 
 // Candidate entry: the cell's tile-local (i, j, k) packed as i | j << 6 | k << 19
-// (i < 64, j and k < 2^13: a tile has at most 4096 vertex rows); the ALL_INSIDE
+// (i < 64, j and k < 2^13: a tile has at most 1024 vertex rows); the ALL_INSIDE
 // entries come first in the list, the count of each part is in ncand[s] =
 // {inside, cut}.
 constexpr int kStageYZ = 256;  // tile coordinates staged in LDS up to this many per axis
@@ -1094,6 +1094,7 @@ __global__ void __launch_bounds__(256) k_plane_alpha(MeshDev m, const int* __res
 }
 
 #include "big_tiles.h"
+#include "poly_mesh.h"
 
 // Field pass: one workgroup per brick of BX*BY*BZ cells, CPT cells per thread
 // (the thread's cells are CPT layers apart along z: lane (ti, tj) of layer tk
@@ -1759,7 +1760,39 @@ struct sdfibm_ctx {
     bool big_on = false;
     BigMeta* d_big_meta = nullptr;  // kBigMaxSolids
     long long* d_big_tot = nullptr;  // kBigTotN
-    int lm_grid = 0;  // workgroups of the cut-cell pass's 256-thread part (solids whose tile can be medium or large)
+    int lm_grid = 0;  // workgroups of the cut-cell pass's 256-thread part (solids whose tile can be medium or big)
+    // general polyhedral meshes (SDFIBM_MESH_POLY): the mesh, per-solid, per-visit, per-pair
+    // and per-cell buffers of the step
+    PolyDev pm{};
+    int poly_reach = 0;  // bins a cell's points reach beyond its own
+    double* g_box = nullptr;
+    int* g_seed = nullptr;
+    int* g_first = nullptr;
+    long long* g_vcnt = nullptr;
+    long long* g_voff = nullptr;
+    long long* g_prange = nullptr;
+    int* g_vcell = nullptr;
+    int* g_vsolid = nullptr;
+    unsigned char* g_act = nullptr;
+    unsigned char* g_all = nullptr;
+    long long* g_vpos = nullptr;
+    long long g_vcap = 0;
+    int* g_pcell = nullptr;
+    int* g_psolid = nullptr;
+    int* g_label = nullptr;
+    unsigned char* g_pall = nullptr;
+    void* g_pcode = nullptr;
+    double* g_pf = nullptr;
+    int* g_clist = nullptr;
+    long long g_pcap = 0;
+    unsigned long long* g_hk = nullptr;
+    int* g_hv = nullptr;
+    int g_bits = 0;
+    int* g_ccnt = nullptr;
+    long long* g_coff = nullptr;
+    long long* g_tmp = nullptr;
+    long long g_tmpcap = 0;
+    long long g_np = 0;
     unsigned long long* d_big_vbits = nullptr;
     int2* d_big_bcnt = nullptr;  // per cell block: ALL_INSIDE and cut counts
     int2* d_big_boff = nullptr;  // per cell block: its list offsets
@@ -1908,7 +1941,8 @@ int validate_csg(const sdfibm_csg_shape& g, long long idx, std::string& why) {
 template <class R, int MK>
 void launch_big(sdfibm_ctx* c, double dt, hipStream_t st) {
     const int ns = (int)c->ns;
-    const unsigned G = (unsigned)(4 * c->n_cu), GF = (unsigned)(c->big_grid * c->n_cu);
+    // flat kernels: 8 workgroups of 256 per CU (32 waves: their loads are latency-bound)
+    const unsigned G = (unsigned)(8 * c->n_cu), GF = (unsigned)(c->big_grid * c->n_cu);
     R* codes = (R*)c->d_codes;
     k_big_layout<<<1, 1024, 0, st>>>(ns, c->d_T, c->qmeta, c->qlist, c->d_big_meta, c->d_big_tot, c->big_word_cap,
                                      c->err);
@@ -1961,16 +1995,13 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     if (!graph) (void)hipEventRecord(c->ev[1], st);
     R* codes = (R*)c->d_codes;
     if (ns > 0) {
-        // persistent launches: a few workgroups per CU take solids from the class queues
         // the class sizes live on the device: the small-tile grid covers every solid (one
-        // each), the medium and large grids (big LDS, few solids) loop over their queues
-        // the few large and medium tiles (long per-solid latency) run beside the small
-        // ones on a second stream
+        // each), the medium grid (big LDS, few solids) loops over its queue; the medium
+        // tiles (long per-solid latency) and the big-tile pass run beside the small ones
+        // on a second stream
         const unsigned g = (unsigned)ns, cus = (unsigned)c->n_cu;
         if (hipEventRecord(c->fork, st) != hipSuccess || hipStreamWaitEvent(c->aux, c->fork, 0) != hipSuccess)
             return -1;
-        k_classify<R, MK, kRowsLarge, 2, 1024><<<std::min(g, cus), 1024, 0, c->aux>>>(
-            c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
         k_classify<R, MK, kRowsMedium, 1, 1024><<<std::min(g, 4 * cus), 1024, 0, c->aux>>>(
             c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes, c->d_cand, c->code_cap, c->d_ncand);
         // the cut-cell pass needs these tiles, the field pass also the big ones
@@ -2392,12 +2423,598 @@ void launch_aos_to_soa(sdfibm_ctx* c, const double* staged, long long first, lon
 
 std::string err_text(int flags) {
     std::string s;
-    if (flags & EF_TILE) s += "a solid's cell tile exceeds 63 cells along x or 4096 vertex rows; ";
-    if (flags & EF_CODECAP) s += "tile code buffer too small (or more than 4096 big tiles); ";
+    if (flags & EF_TILE) s += "a solid's cell tile exceeds 63 cells along x or 1024 vertex rows; ";
+    if (flags & EF_CODECAP) s += "tile code buffer too small (or more than 16384 big tiles); ";
     if (flags & EF_BRICK) s += "more than 128 solids overlap one brick; ";
     if (flags & EF_PLANE_MOVED) s += "a plane solid moved during a graph replay; ";
     if (flags & EF_LABEL) s += "the big-tile component labelling did not converge; ";
     return s;
+}
+
+// ---------------------------------------------------------------------------
+// General polyhedral meshes (SDFIBM_MESH_POLY): set-up and the step
+// ---------------------------------------------------------------------------
+// Face centre and area vector of a polygon as OpenFOAM's primitiveMesh forms
+// them: triangles fanned about the average of the face's points, the centre
+// their centroids weighted by the triangle areas projected on the face normal.
+void poly_face_geometry(const double* P, const int32_t* lab, int n, double* fc, double* sf) {
+    auto pt = [&](int t, int d) { return P[3 * (long long)lab[t] + d]; };
+    if (n == 3) {
+        double e1[3], e2[3];
+        for (int d = 0; d < 3; ++d) {
+            fc[d] = (1.0 / 3.0) * ((pt(0, d) + pt(1, d)) + pt(2, d));
+            e1[d] = pt(1, d) - pt(0, d);
+            e2[d] = pt(2, d) - pt(0, d);
+        }
+        sf[0] = 0.5 * (e1[1] * e2[2] - e1[2] * e2[1]);
+        sf[1] = 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]);
+        sf[2] = 0.5 * (e1[0] * e2[1] - e1[1] * e2[0]);
+        return;
+    }
+    double m[3] = {0, 0, 0};
+    for (int t = 0; t < n; ++t)
+        for (int d = 0; d < 3; ++d) m[d] += pt(t, d);
+    for (int d = 0; d < 3; ++d) m[d] /= (double)n;
+    // triangle (p_t, p_t+1, m): normal (p_t+1 - p_t) x (m - p_t)
+    auto tri_normal = [&](int t, double* nrm) {
+        const int u = (t + 1) % n;
+        double a[3], b[3];
+        for (int d = 0; d < 3; ++d) {
+            a[d] = pt(u, d) - pt(t, d);
+            b[d] = m[d] - pt(t, d);
+        }
+        nrm[0] = a[1] * b[2] - a[2] * b[1];
+        nrm[1] = a[2] * b[0] - a[0] * b[2];
+        nrm[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double sum_n[3] = {0, 0, 0};
+    for (int t = 0; t < n; ++t) {
+        double nrm[3];
+        tri_normal(t, nrm);
+        for (int d = 0; d < 3; ++d) sum_n[d] += nrm[d];
+    }
+    const double mag = std::sqrt(sum_n[0] * sum_n[0] + sum_n[1] * sum_n[1] + sum_n[2] * sum_n[2]);
+    for (int d = 0; d < 3; ++d) sf[d] = 0.5 * sum_n[d];
+    if (!(mag > 1e-300)) {
+        for (int d = 0; d < 3; ++d) fc[d] = m[d];
+        return;
+    }
+    double w_sum = 0.0, c_sum[3] = {0, 0, 0};
+    for (int t = 0; t < n; ++t) {
+        double nrm[3];
+        tri_normal(t, nrm);
+        const double w = (nrm[0] * sum_n[0] + nrm[1] * sum_n[1] + nrm[2] * sum_n[2]) / mag;
+        const int u = (t + 1) % n;
+        w_sum += w;
+        for (int d = 0; d < 3; ++d) c_sum[d] += w * ((pt(t, d) + pt(u, d)) + m[d]);
+    }
+    for (int d = 0; d < 3; ++d) fc[d] = ((1.0 / 3.0) * c_sum[d]) / w_sum;
+}
+
+// The mesh's addressing and geometry on the device; the cell bins.
+int poly_setup(sdfibm_ctx* c, const sdfibm_polymesh_desc* d) {
+    const long long nP = d->n_points, nF = d->n_faces, nI = d->n_internal_faces, nC = d->n_cells;
+    if (nP < 4 || nF < 4 || nC < 1 || nI < 0 || nI > nF)
+        return fail(c, SDFIBM_ERR_MESH, "polyMesh sizes are inconsistent");
+    if (nP > INT_MAX || nF > INT_MAX || nC > INT_MAX) return fail(c, SDFIBM_ERR_MESH, "more than 2^31-1 points, faces or cells");
+    if (!d->points || !d->face_offsets || !d->face_points || !d->owner || (nI > 0 && !d->neighbour))
+        return fail(c, SDFIBM_ERR_ARG, "polyMesh arrays missing");
+    const int64_t* fo = d->face_offsets;
+    if (fo[0] != 0) return fail(c, SDFIBM_ERR_MESH, "face_offsets[0] must be 0");
+    for (long long f = 0; f < nF; ++f)
+        if (fo[f + 1] - fo[f] < 3) return fail(c, SDFIBM_ERR_MESH, "face " + std::to_string(f) + " has fewer than 3 points");
+    if (fo[nF] > INT_MAX) return fail(c, SDFIBM_ERR_MESH, "more than 2^31-1 face points");
+    for (long long t = 0; t < fo[nF]; ++t)
+        if (d->face_points[t] < 0 || d->face_points[t] >= nP) return fail(c, SDFIBM_ERR_MESH, "face point label out of range");
+    for (long long f = 0; f < nF; ++f) {
+        if (d->owner[f] < 0 || d->owner[f] >= nC) return fail(c, SDFIBM_ERR_MESH, "owner label out of range");
+        if (f < nI && (d->neighbour[f] < 0 || d->neighbour[f] >= nC || d->neighbour[f] == d->owner[f]))
+            return fail(c, SDFIBM_ERR_MESH, "neighbour label out of range");
+    }
+    // cells(): owned faces ascending, then neighbour faces ascending (primitiveMesh::calcCells)
+    std::vector<int> cfo(nC + 1, 0), cf;
+    for (long long f = 0; f < nF; ++f) ++cfo[d->owner[f] + 1];
+    for (long long f = 0; f < nI; ++f) ++cfo[d->neighbour[f] + 1];
+    for (long long k = 0; k < nC; ++k) cfo[k + 1] += cfo[k];
+    cf.resize(cfo[nC]);
+    {
+        std::vector<int> at(cfo.begin(), cfo.end() - 1);
+        for (long long f = 0; f < nF; ++f) cf[at[d->owner[f]]++] = (int)f;
+        for (long long f = 0; f < nI; ++f) cf[at[d->neighbour[f]]++] = (int)f;
+    }
+    for (long long k = 0; k < nC; ++k)
+        if (cfo[k + 1] - cfo[k] < 4) return fail(c, SDFIBM_ERR_MESH, "cell " + std::to_string(k) + " has fewer than 4 faces");
+    // cellPoints (ascending labels) and cellCells
+    std::vector<int> cpo(nC + 1, 0), cp, cno(nC + 1, 0), cn;
+    {
+        std::vector<int> tmp;
+        for (long long k = 0; k < nC; ++k) {
+            tmp.clear();
+            for (int t = cfo[k]; t < cfo[k + 1]; ++t)
+                for (long long u = fo[cf[t]]; u < fo[cf[t] + 1]; ++u) tmp.push_back(d->face_points[u]);
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            cp.insert(cp.end(), tmp.begin(), tmp.end());
+            cpo[k + 1] = (int)cp.size();
+        }
+        for (long long f = 0; f < nI; ++f) {
+            ++cno[d->owner[f] + 1];
+            ++cno[d->neighbour[f] + 1];
+        }
+        for (long long k = 0; k < nC; ++k) cno[k + 1] += cno[k];
+        cn.resize(cno[nC]);
+        std::vector<int> at(cno.begin(), cno.end() - 1);
+        for (long long f = 0; f < nI; ++f) {
+            cn[at[d->owner[f]]++] = d->neighbour[f];
+            cn[at[d->neighbour[f]]++] = d->owner[f];
+        }
+    }
+    // geometry: the caller's, else primitiveMesh's (face fan; cell pyramids about the average
+    // of its face centres)
+    std::vector<double> fcv, sfv, ccv, volv;
+    const double *fcp = d->face_centres, *sfp = d->face_areas, *ccp = d->cell_centres, *vp = d->cell_volumes;
+    if (!fcp || !sfp) {
+        fcv.resize(3 * nF);
+        sfv.resize(3 * nF);
+        for (long long f = 0; f < nF; ++f)
+            poly_face_geometry(d->points, d->face_points + fo[f], (int)(fo[f + 1] - fo[f]), &fcv[3 * f], &sfv[3 * f]);
+        fcp = fcv.data();
+        sfp = sfv.data();
+    }
+    if (!ccp || !vp) {
+        ccv.assign(3 * nC, 0.0);
+        volv.assign(nC, 0.0);
+        for (long long k = 0; k < nC; ++k) {
+            double est[3] = {0, 0, 0};
+            const int nfc = cfo[k + 1] - cfo[k];
+            for (int t = cfo[k]; t < cfo[k + 1]; ++t)
+                for (int e = 0; e < 3; ++e) est[e] += fcp[3 * (long long)cf[t] + e];
+            for (int e = 0; e < 3; ++e) est[e] /= (double)nfc;
+            double vsum = 0.0, csum[3] = {0, 0, 0};
+            for (int t = cfo[k]; t < cfo[k + 1]; ++t) {
+                const long long f = cf[t];
+                const double sgn = d->owner[f] == k ? 1.0 : -1.0;  // outward from this cell
+                double h = 0.0;
+                for (int e = 0; e < 3; ++e) h += sfp[3 * f + e] * (sgn * (fcp[3 * f + e] - est[e]));
+                for (int e = 0; e < 3; ++e) csum[e] += h * ((3.0 / 4.0) * fcp[3 * f + e] + (1.0 / 4.0) * est[e]);
+                vsum += h;
+            }
+            for (int e = 0; e < 3; ++e) ccv[3 * k + e] = std::fabs(vsum) > 1e-300 ? csum[e] / vsum : est[e];
+            volv[k] = vsum * (1.0 / 3.0);
+        }
+        ccp = ccv.data();
+        vp = volv.data();
+    }
+    for (long long k = 0; k < nC; ++k)
+        if (!(vp[k] > 0)) return fail(c, SDFIBM_ERR_MESH, "cell " + std::to_string(k) + " has no positive volume");
+    // bins: a uniform grid over the cell centres' box, about 4 cells per bin; each bin's box
+    // holds its cells' point boxes
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (long long k = 0; k < nC; ++k)
+        for (int e = 0; e < 3; ++e) {
+            lo[e] = std::min(lo[e], ccp[3 * k + e]);
+            hi[e] = std::max(hi[e], ccp[3 * k + e]);
+        }
+    double ext[3], vol_box = 1.0;
+    int dims = 0;
+    for (int e = 0; e < 3; ++e) {
+        ext[e] = hi[e] - lo[e];
+        if (ext[e] > 0) {
+            vol_box *= ext[e];
+            ++dims;
+        }
+    }
+    const double want = std::max(1.0, (double)nC / 4.0);
+    const double side = dims > 0 ? std::pow(vol_box / want, 1.0 / dims) : 1.0;
+    PolyDev& P = c->pm;
+    long long nb = 1;
+    for (int e = 0; e < 3; ++e) {
+        P.bn[e] = ext[e] > 0 ? (int)std::max(1.0, std::min(4096.0, std::floor(ext[e] / side))) : 1;
+        P.bw[e] = ext[e] > 0 ? ext[e] / P.bn[e] : 1.0;
+        P.binv[e] = 1.0 / P.bw[e];
+        P.blo[e] = lo[e];
+        nb *= P.bn[e];
+    }
+    std::vector<int> bid(nC), bo(nb + 1, 0), bc(nC);
+    for (long long k = 0; k < nC; ++k) {
+        long long b = 0;
+        for (int e = 2; e >= 0; --e) {
+            int i = (int)std::floor((ccp[3 * k + e] - P.blo[e]) * P.binv[e]);
+            i = std::max(0, std::min(P.bn[e] - 1, i));
+            b = b * P.bn[e] + i;
+        }
+        bid[k] = (int)b;
+        ++bo[b + 1];
+    }
+    for (long long b = 0; b < nb; ++b) bo[b + 1] += bo[b];
+    std::vector<double> bbox(6 * nb);
+    for (long long b = 0; b < nb; ++b)
+        for (int e = 0; e < 3; ++e) {
+            bbox[6 * b + e] = 1e300;
+            bbox[6 * b + 3 + e] = -1e300;
+        }
+    {
+        std::vector<int> at(bo.begin(), bo.end() - 1);
+        for (long long k = 0; k < nC; ++k) bc[at[bid[k]]++] = (int)k;  // ascending within a bin
+    }
+    // a cell's point box, and how far it reaches beyond its bin (in bins)
+    int reach = 0;
+    for (long long k = 0; k < nC; ++k) {
+        double cl[3] = {1e300, 1e300, 1e300}, ch[3] = {-1e300, -1e300, -1e300};
+        for (int t = cpo[k]; t < cpo[k + 1]; ++t)
+            for (int e = 0; e < 3; ++e) {
+                const double x = d->points[3 * (long long)cp[t] + e];
+                cl[e] = std::min(cl[e], x);
+                ch[e] = std::max(ch[e], x);
+            }
+        const long long b = bid[k];
+        long long r = b;
+        int ijk[3];
+        for (int e = 0; e < 3; ++e) {
+            ijk[e] = (int)(r % P.bn[e]);
+            r /= P.bn[e];
+        }
+        for (int e = 0; e < 3; ++e) {
+            bbox[6 * b + e] = std::min(bbox[6 * b + e], cl[e]);
+            bbox[6 * b + 3 + e] = std::max(bbox[6 * b + 3 + e], ch[e]);
+            const double blo = P.blo[e] + ijk[e] * P.bw[e], bhi = blo + P.bw[e];
+            const double over = std::max(blo - cl[e], ch[e] - bhi);
+            if (over > 0) reach = std::max(reach, (int)std::ceil(over * P.binv[e]));
+        }
+    }
+    c->poly_reach = std::min(reach, 4096);
+    // the mesh box (contact grid)
+    for (int e = 0; e < 3; ++e) {
+        c->box_lo[e] = 1e300;
+        c->box_hi[e] = -1e300;
+    }
+    for (long long q = 0; q < nP; ++q)
+        for (int e = 0; e < 3; ++e) {
+            c->box_lo[e] = std::min(c->box_lo[e], d->points[3 * q + e]);
+            c->box_hi[e] = std::max(c->box_hi[e], d->points[3 * q + e]);
+        }
+    // uploads
+    int rc;
+    auto up = [&](const void* src, size_t bytes, void** dst) -> int {
+        int r;
+        if ((r = dalloc(c, dst, bytes))) return r;
+        if (bytes && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(c, SDFIBM_ERR_HIP, "mesh upload failed");
+        return SDFIBM_OK;
+    };
+    std::vector<double> soa(nP);
+    double* pxyz[3] = {nullptr, nullptr, nullptr};
+    for (int e = 0; e < 3; ++e) {
+        for (long long q = 0; q < nP; ++q) soa[q] = d->points[3 * q + e];
+        if ((rc = up(soa.data(), nP * 8, (void**)&pxyz[e]))) return rc;
+    }
+    P.px = pxyz[0];
+    P.py = pxyz[1];
+    P.pz = pxyz[2];
+    std::vector<int> fo32(nF + 1), fp32(d->face_points, d->face_points + fo[nF]);
+    for (long long f = 0; f <= nF; ++f) fo32[f] = (int)fo[f];
+    void* p;
+    if ((rc = up(fo32.data(), (nF + 1) * 4, &p))) return rc;
+    P.f_off = (const int*)p;
+    if ((rc = up(fp32.data(), fp32.size() * 4, &p))) return rc;
+    P.f_pts = (const int*)p;
+    if ((rc = up(fcp, nF * 24, &p))) return rc;
+    P.fc = (const double*)p;
+    if ((rc = up(sfp, nF * 24, &p))) return rc;
+    P.sf = (const double*)p;
+    if ((rc = up(cfo.data(), (nC + 1) * 4, &p))) return rc;
+    P.c_foff = (const int*)p;
+    if ((rc = up(cf.data(), cf.size() * 4, &p))) return rc;
+    P.c_faces = (const int*)p;
+    if ((rc = up(cpo.data(), (nC + 1) * 4, &p))) return rc;
+    P.c_poff = (const int*)p;
+    if ((rc = up(cp.data(), cp.size() * 4, &p))) return rc;
+    P.c_pts = (const int*)p;
+    if ((rc = up(cno.data(), (nC + 1) * 4, &p))) return rc;
+    P.c_noff = (const int*)p;
+    if ((rc = up(cn.data(), std::max<size_t>(cn.size(), 1) * 4, &p))) return rc;
+    P.c_nbs = (const int*)p;
+    if ((rc = up(ccp, nC * 24, &p))) return rc;
+    P.cc = (const double*)p;
+    if ((rc = up(vp, nC * 8, &p))) return rc;
+    P.vol = (const double*)p;
+    if ((rc = up(bo.data(), (nb + 1) * 4, &p))) return rc;
+    P.b_off = (const int*)p;
+    if ((rc = up(bc.data(), nC * 4, &p))) return rc;
+    P.b_cells = (const int*)p;
+    if ((rc = up(bbox.data(), nb * 48, &p))) return rc;
+    P.b_box = (const double*)p;
+    P.n_cells = (int)nC;
+    P.n_faces = (int)nF;
+    P.n_points = (int)nP;
+    P.two_d = c->two_d;
+    // per-cell work buffers of the step
+    if ((rc = dalloc(c, (void**)&c->g_ccnt, nC * 4))) return rc;
+    if ((rc = dalloc(c, (void**)&c->g_coff, (nC + 1) * 8))) return rc;
+    return SDFIBM_OK;
+}
+
+// A buffer grown to hold at least n elements of `bytes` each (with headroom).
+template <class T>
+int poly_grow(sdfibm_ctx* c, T*& buf, long long& cap, long long n, size_t bytes) {
+    if (n <= cap && buf) return SDFIBM_OK;
+    dfree(c, buf);
+    cap = 0;
+    const long long want = n + n / 4 + 1024;
+    int rc;
+    if ((rc = dalloc(c, (void**)&buf, want * bytes))) return rc;
+    cap = want;
+    return SDFIBM_OK;
+}
+
+// Per-solid buffers of the step for n solids.
+int poly_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
+    for (int64_t s = 0; s < n; ++s) {
+        std::string why;
+        const int v = validate_solid(solids[s], s, why, (long long)c->h_csg.size());
+        if (v) return fail(c, v, why);
+    }
+    c->have_solids = false;
+    c->have_step = false;
+    c->have_ct = false;
+    c->ns = 0;
+    c->contact_ready = false;
+    int rc;
+    if (n > c->solid_cap) {
+        for (void** b : {(void**)&c->d_solids, (void**)&c->d_K, (void**)&c->d_ft, (void**)&c->d_ncand,
+                         (void**)&c->g_box, (void**)&c->g_seed, (void**)&c->g_first, (void**)&c->g_vcnt,
+                         (void**)&c->g_voff, (void**)&c->g_prange}) {
+            void* q = *b;
+            dfree(c, q);
+            *b = nullptr;
+        }
+        c->solid_cap = 0;
+        const long long cap = std::max<long long>(n, 16);
+        if ((rc = dalloc(c, (void**)&c->d_solids, cap * sizeof(sdfibm_solid)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_K, cap * sizeof(SolidK)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_ft, cap * 48))) return rc;
+        if ((rc = dalloc(c, (void**)&c->d_ncand, cap * sizeof(int2)))) return rc;
+        if ((rc = dalloc(c, (void**)&c->g_box, cap * 48))) return rc;
+        if ((rc = dalloc(c, (void**)&c->g_seed, cap * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->g_first, cap * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->g_vcnt, cap * 8))) return rc;
+        if ((rc = dalloc(c, (void**)&c->g_voff, (cap + 1) * 8))) return rc;
+        if ((rc = dalloc(c, (void**)&c->g_prange, cap * 16))) return rc;
+        c->solid_cap = cap;
+    }
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(c->d_solids, solids, n * sizeof(sdfibm_solid), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemset(c->d_ft, 0, n * 48));
+    }
+    c->h_solids.assign(solids, solids + n);
+    c->ns = n;
+    c->have_solids = true;
+    return SDFIBM_OK;
+}
+
+// One step on a polyhedral mesh: pairs, labels, codes, force/torque, fields.
+template <class R>
+int poly_step(sdfibm_ctx* c, double dt) {
+    const int ns = (int)c->ns;
+    const long long nC = c->pm.n_cells;
+    hipStream_t st = c->stream;
+    const unsigned G = (unsigned)(4 * c->n_cu);
+    int rc;
+    (void)hipEventRecord(c->ev[0], st);
+    long long np = 0;
+    if (ns > 0) {
+        if ((rc = poly_grow(c, c->g_tmp, c->g_tmpcap, std::max<long long>(ns, nC) / kScanTile + 2, 8))) return rc;
+        k_poly_setup<<<grid_for(ns, 4), 256, 0, st>>>(c->d_solids, ns, c->pm, c->d_K, c->g_box, c->g_seed, c->g_vcnt,
+                                                      c->d_ft, c->g_first, c->d_csg, c->poly_reach);
+        scan<long long>(c->g_vcnt, ns, c->g_voff, c->g_tmp, st);
+        long long nv = 0;
+        HIPCHK(c, hipMemcpyAsync(&nv, c->g_voff + ns, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        if (nv > INT_MAX) return fail(c, SDFIBM_ERR_CAPACITY, "more than 2^31-1 (solid, cell) visits");
+        if (nv + 1 > c->g_vcap || !c->g_vcell) {  // the visit arrays, one capacity
+            for (void** b : {(void**)&c->g_vcell, (void**)&c->g_vsolid, (void**)&c->g_act, (void**)&c->g_all,
+                             (void**)&c->g_vpos}) {
+                void* q = *b;
+                dfree(c, q);
+                *b = nullptr;
+            }
+            c->g_vcap = 0;
+            const long long want = nv + 1 + (nv + 1) / 4 + 1024;
+            if ((rc = dalloc(c, (void**)&c->g_vcell, want * 4))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_vsolid, want * 4))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_act, want))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_all, want))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_vpos, want * 8))) return rc;
+            c->g_vcap = want;
+        }
+        if ((rc = poly_grow(c, c->g_tmp, c->g_tmpcap, nv / kScanTile + 2, 8))) return rc;
+        k_poly_visit<<<grid_for(ns, 4), 256, 0, st>>>(ns, c->pm, c->g_box, c->g_voff, c->g_vcell, c->g_vsolid,
+                                                      c->poly_reach);
+        if (nv > 0)
+            k_poly_test<<<G, 256, 0, st>>>(c->pm, c->d_K, nv, c->g_vcell, c->g_vsolid, c->g_act, c->g_all, c->g_first);
+        scan<unsigned char>(c->g_act, nv, c->g_vpos, c->g_tmp, st);
+        HIPCHK(c, hipMemcpyAsync(&np, c->g_vpos + nv, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        if (np > INT_MAX / 2) return fail(c, SDFIBM_ERR_CAPACITY, "more than 2^30 (solid, cell) pairs");
+        if (np > c->g_pcap || !c->g_pcell) {  // the pair arrays, one capacity
+            for (void** b : {(void**)&c->g_pcell, (void**)&c->g_psolid, (void**)&c->g_label, (void**)&c->g_pall,
+                             (void**)&c->g_pcode, (void**)&c->g_pf, (void**)&c->g_clist}) {
+                void* q = *b;
+                dfree(c, q);
+                *b = nullptr;
+            }
+            c->g_pcap = 0;
+            const long long want = np + np / 4 + 1024;
+            if ((rc = dalloc(c, (void**)&c->g_pcell, want * 4))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_psolid, want * 4))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_label, want * 4))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_pall, want))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_pcode, want * 8))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_pf, want * 48))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_clist, want * 4))) return rc;
+            c->g_pcap = want;
+        }
+        // hash table: a power of two of at least twice the pairs
+        int bits = 10;
+        while ((1LL << bits) < 2 * np) ++bits;
+        if (bits > c->g_bits) {
+            dfree(c, c->g_hk);
+            dfree(c, c->g_hv);
+            c->g_bits = 0;
+            if ((rc = dalloc(c, (void**)&c->g_hk, (1LL << bits) * 8))) return rc;
+            if ((rc = dalloc(c, (void**)&c->g_hv, (1LL << bits) * 4))) return rc;
+            c->g_bits = bits;
+        }
+        bits = c->g_bits;
+        k_fill_i32<<<grid_for(2LL << bits, 256), 256, 0, st>>>((int*)c->g_hk, 2LL << bits, -1);  // kPolyEmpty keys
+        if (np > 0) {
+            k_poly_compact<<<G, 256, 0, st>>>(nv, c->g_vcell, c->g_vsolid, c->g_act, c->g_all, c->g_vpos, c->g_pcell,
+                                              c->g_psolid, c->g_pall, c->g_label, c->g_hk, c->g_hv, bits);
+            k_poly_hook<<<G, 256, 0, st>>>(c->pm, np, c->g_pcell, c->g_psolid, c->g_label, c->g_hk, c->g_hv, bits);
+            k_poly_compress<<<G, 256, 0, st>>>(np, c->g_label);
+            k_poly_force<R><<<G, 256, 0, st>>>(c->pm, c->d_K, np, c->g_pcell, c->g_psolid, c->g_pall, c->g_label,
+                                               c->g_seed, c->g_first, c->g_hk, c->g_hv, bits, (const R*)c->U[0],
+                                               (const R*)c->U[1], (const R*)c->U[2], (R*)c->g_pcode, c->g_pf, dt);
+        }
+        k_poly_reduce<R><<<grid_for(ns, 4), 256, 0, st>>>(ns, c->g_voff, c->g_vpos, (const R*)c->g_pcode, c->g_pf,
+                                                          c->rho_f, c->d_ft, c->d_ncand, c->g_prange);
+    }
+    c->g_np = np;
+    (void)hipEventRecord(c->ev[1], st);
+    // the member pairs of each cell
+    if ((rc = poly_grow(c, c->g_tmp, c->g_tmpcap, nC / kScanTile + 2, 8))) return rc;
+    k_fill_i32<<<grid_for(nC, 256), 256, 0, st>>>(c->g_ccnt, nC, 0);
+    if (np > 0) k_poly_cells<R><<<G, 256, 0, st>>>(np, c->g_pcell, (const R*)c->g_pcode, c->g_ccnt, nullptr, nullptr);
+    scan<int>(c->g_ccnt, nC, c->g_coff, c->g_tmp, st);
+    k_fill_i32<<<grid_for(nC, 256), 256, 0, st>>>(c->g_ccnt, nC, 0);
+    if (np > 0)
+        k_poly_cells<R><<<G, 256, 0, st>>>(np, c->g_pcell, (const R*)c->g_pcode, c->g_ccnt, c->g_coff, c->g_clist);
+    (void)hipEventRecord(c->ev[2], st);
+    k_poly_fields<R><<<grid_for(nC, 256), 256, 0, st>>>(c->pm, c->d_K, c->g_coff, c->g_clist, c->g_psolid,
+                                                        (const R*)c->g_pcode, (const R*)c->U[0], (const R*)c->U[1],
+                                                        (const R*)c->U[2], (R*)c->As, (R*)c->Fs[0], (R*)c->Fs[1],
+                                                        (R*)c->Fs[2], (R*)c->Tsf, c->Ct, dt);
+    (void)hipEventRecord(c->ev[3], st);
+    HIPCHK(c, hipGetLastError());
+    return SDFIBM_OK;
+}
+
+int poly_interact(sdfibm_ctx* c, double dt, double* ft) {
+    const int rc = c->prec == SDFIBM_FP64 ? poly_step<double>(c, dt) : poly_step<float>(c, dt);
+    if (rc) return rc;
+    if (ft && c->ns > 0) HIPCHK(c, hipMemcpyAsync(ft, c->d_ft, c->ns * 48, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms[4] = {0, 0, 0, 0};
+    (void)hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]);
+    (void)hipEventElapsedTime(&ms[3], c->ev[0], c->ev[3]);
+    for (int t = 0; t < 7; ++t) c->timing[t] = 0.0;
+    c->timing[0] = ms[0];
+    c->timing[1] = ms[1];
+    c->timing[2] = ms[2];
+    c->timing[3] = ms[3];
+    c->have_step = true;
+    c->have_ct = true;
+    return SDFIBM_OK;
+}
+
+// Candidate cells of one solid (the ALL_INSIDE, centre-inside and centre-outside
+// sets, each ascending) from the last step's pairs.
+int poly_candidates(sdfibm_ctx* c, int64_t solid, int64_t* counts, int64_t* cells, double* alpha, int64_t capacity) {
+    long long pr[2] = {0, 0};
+    int seed = -1;
+    HIPCHK(c, hipMemcpy(pr, c->g_prange + 2 * solid, 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&seed, c->g_seed + solid, 4, hipMemcpyDeviceToHost));
+    const long long n = pr[1] - pr[0];
+    const size_t rs = rsize(c);
+    std::vector<int> pc(std::max<long long>(n, 1));
+    std::vector<unsigned char> raw(std::max<long long>(n, 1) * rs);
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(pc.data(), c->g_pcell + pr[0], n * 4, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(raw.data(), (unsigned char*)c->g_pcode + pr[0] * rs, n * rs, hipMemcpyDeviceToHost));
+    }
+    std::vector<std::pair<long long, double>> lists[3];
+    for (long long q = 0; q < n; ++q) {
+        const double v = rs == 8 ? ((const double*)raw.data())[q] : (double)((const float*)raw.data())[q];
+        if (std::isnan(v)) continue;
+        const int t = std::isinf(v) ? 0 : (std::signbit(v) ? 2 : 1);
+        lists[t].push_back({pc[q], std::isinf(v) ? 1.0 : std::fabs(v)});
+    }
+    long long first = -1;
+    bool near_in = false;
+    for (int t = 0; t < 3; ++t) {
+        std::sort(lists[t].begin(), lists[t].end());
+        counts[t] = (int64_t)lists[t].size();
+        for (auto& e : lists[t]) {
+            if (first < 0 || e.first < first) first = e.first;
+            if (e.first == seed) near_in = true;
+        }
+    }
+    counts[3] = counts[0] + counts[1] + counts[2] > 0 ? (near_in ? seed : first) : -1;
+    long long k = 0;
+    for (int t = 0; t < 3; ++t)
+        for (auto& e : lists[t]) {
+            if (k >= capacity) break;
+            if (cells) cells[k] = e.first;
+            if (alpha) alpha[k] = e.second;
+            ++k;
+        }
+    return SDFIBM_OK;
+}
+
+// Context set-up shared by both mesh kinds: the configuration and the device
+// (create_begin), then, once the mesh is on the device, the streams, events,
+// flag words, resident fields and step metadata (create_end).
+int create_begin(const sdfibm_config* cfg, sdfibm_ctx** out) {
+    *out = nullptr;
+    if (cfg->precision != SDFIBM_FP32 && cfg->precision != SDFIBM_FP64)
+        return fail(nullptr, SDFIBM_ERR_ARG, "precision must be SDFIBM_FP32 or SDFIBM_FP64");
+    if (!std::isfinite(cfg->rho_f)) return fail(nullptr, SDFIBM_ERR_ARG, "rho_f must be finite");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return fail(nullptr, SDFIBM_ERR_NODEVICE, "no HIP device available");
+    }
+    sdfibm_ctx* c = new sdfibm_ctx;
+    (void)hipGetDevice(&c->device);
+    c->prec = cfg->precision;
+    c->two_d = cfg->two_d ? 1 : 0;
+    c->rho_f = cfg->rho_f;
+    *out = c;
+    return SDFIBM_OK;
+}
+
+int create_end(sdfibm_ctx* c) {
+    int rc;
+    if (hipStreamCreate(&c->stream) != hipSuccess) return fail(c, SDFIBM_ERR_HIP, "hipStreamCreate failed");
+    c->own_stream = true;
+    if ((rc = resolve_stage(c))) return rc;
+    if (hipStreamCreate(&c->aux) != hipSuccess) return fail(c, SDFIBM_ERR_HIP, "hipStreamCreate failed");
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(c, SDFIBM_ERR_HIP, "hipEventCreate failed");
+    if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->joinc, hipEventDisableTiming) != hipSuccess)
+        return fail(c, SDFIBM_ERR_HIP, "hipEventCreate failed");
+    if (hipHostMalloc((void**)&c->h_flags, 64, hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->d_hflags, c->h_flags, 0) != hipSuccess)
+        return fail(c, SDFIBM_ERR_HIP, "pinned host buffer for the error flags failed");
+    const size_t rs = rsize(c);
+    for (int a = 0; a < 3; ++a) {
+        if ((rc = dalloc(c, &c->U[a], c->N * rs))) return rc;
+        if ((rc = dalloc(c, &c->Fs[a], c->N * rs))) return rc;
+        if (hipMemset(c->U[a], 0, c->N * rs) != hipSuccess) return fail(c, SDFIBM_ERR_HIP, "memset U");
+    }
+    if ((rc = dalloc(c, &c->As, c->N * rs))) return rc;
+    if ((rc = dalloc(c, &c->Tsf, c->N * rs))) return rc;
+    if ((rc = dalloc(c, (void**)&c->Ct, c->N * 4))) return rc;
+    if ((rc = dalloc(c, (void**)&c->err, 64))) return rc;
+    if ((rc = dalloc(c, (void**)&c->qmeta, 64))) return rc;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->n_cu = prop.multiProcessorCount;
+    return SDFIBM_OK;
 }
 
 }  // namespace
@@ -2431,70 +3048,23 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
     if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(nullptr, SDFIBM_ERR_ARG, "mesh dimensions must be >= 1");
     if ((long long)d->nx * d->ny * d->nz > (1LL << 31) - 1)
         return fail(nullptr, SDFIBM_ERR_MESH, "more than 2^31-1 cells");
-    if (cfg->precision != SDFIBM_FP32 && cfg->precision != SDFIBM_FP64)
-        return fail(nullptr, SDFIBM_ERR_ARG, "precision must be SDFIBM_FP32 or SDFIBM_FP64");
-    if (!std::isfinite(cfg->rho_f)) return fail(nullptr, SDFIBM_ERR_ARG, "rho_f must be finite");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        (void)hipGetLastError();
-        return fail(nullptr, SDFIBM_ERR_NODEVICE, "no HIP device available");
-    }
-    sdfibm_ctx* c = new sdfibm_ctx;
-    (void)hipGetDevice(&c->device);
-    c->kind = d->kind;
-    c->prec = cfg->precision;
-    c->two_d = cfg->two_d ? 1 : 0;
-    c->rho_f = cfg->rho_f;
-    c->nx = d->nx;
-    c->ny = d->ny;
-    c->nz = d->nz;
-    c->N = (long long)d->nx * d->ny * d->nz;
-    int rc = setup_mesh(c, d);
-    if (rc) {
-        g_noctx_msg = c->msg;
-        sdfibm_destroy(c);
-        return rc;
-    }
+    if (d->kind != SDFIBM_MESH_RECTILINEAR && d->kind != SDFIBM_MESH_STRUCTURED)
+        return fail(nullptr, SDFIBM_ERR_ARG, "mesh kind must be RECTILINEAR or STRUCTURED (sdfibm_create_polymesh: POLY)");
+    sdfibm_ctx* c = nullptr;
+    int rc = create_begin(cfg, &c);
+    if (rc) return rc;
     auto bail = [&](int code) {
         g_noctx_msg = c->msg;
         sdfibm_destroy(c);
         return code;
     };
-    if (hipStreamCreate(&c->stream) != hipSuccess) {
-        c->msg = "hipStreamCreate failed";
-        return bail(SDFIBM_ERR_HIP);
-    }
-    c->own_stream = true;
-    if ((rc = resolve_stage(c))) return bail(rc);
-    if (hipStreamCreate(&c->aux) != hipSuccess) {
-        c->msg = "hipStreamCreate failed";
-        return bail(SDFIBM_ERR_HIP);
-    }
-    for (auto& e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            c->msg = "hipEventCreate failed";
-            return bail(SDFIBM_ERR_HIP);
-        }
-    if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->joinc, hipEventDisableTiming) != hipSuccess) {
-        c->msg = "hipEventCreate failed";
-        return bail(SDFIBM_ERR_HIP);
-    }
-    if (hipHostMalloc((void**)&c->h_flags, 64, hipHostMallocMapped) != hipSuccess ||
-        hipHostGetDevicePointer((void**)&c->d_hflags, c->h_flags, 0) != hipSuccess) {
-        c->msg = "pinned host buffer for the error flags failed";
-        return bail(SDFIBM_ERR_HIP);
-    }
-    const size_t rs = rsize(c);
-    for (int a = 0; a < 3; ++a) {
-        if ((rc = dalloc(c, &c->U[a], c->N * rs))) return bail(rc);
-        if ((rc = dalloc(c, &c->Fs[a], c->N * rs))) return bail(rc);
-        if (hipMemset(c->U[a], 0, c->N * rs) != hipSuccess) return bail(fail(c, SDFIBM_ERR_HIP, "memset U"));
-    }
-    if ((rc = dalloc(c, &c->As, c->N * rs))) return bail(rc);
-    if ((rc = dalloc(c, &c->Tsf, c->N * rs))) return bail(rc);
-    if ((rc = dalloc(c, (void**)&c->Ct, c->N * 4))) return bail(rc);
+    c->kind = d->kind;
+    c->nx = d->nx;
+    c->ny = d->ny;
+    c->nz = d->nz;
+    c->N = (long long)d->nx * d->ny * d->nz;
+    if ((rc = setup_mesh(c, d))) return bail(rc);
+    if ((rc = create_end(c))) return bail(rc);
     const bool flat = c->nz == 1;
     c->bx = flat ? kBX2 : kBX3;
     c->by = flat ? kBY2 : kBY3;
@@ -2507,14 +3077,28 @@ int sdfibm_create(const sdfibm_mesh_desc* d, const sdfibm_config* cfg, sdfibm_ct
         return bail(fail(c, SDFIBM_ERR_MESH, "mesh too large along y or z for the brick grid"));
     if ((rc = dalloc(c, (void**)&c->bcount, c->nbricks * 4))) return bail(rc);
     if ((rc = dalloc(c, (void**)&c->d_blist, c->nbricks * kMaxPerBrick * 4))) return bail(rc);
-    if ((rc = dalloc(c, (void**)&c->err, 64))) return bail(rc);
-    if ((rc = dalloc(c, (void**)&c->qmeta, 64))) return bail(rc);
+    *out = c;
+    return SDFIBM_OK;
+}
 
-    {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
-            c->n_cu = prop.multiProcessorCount;
-    }
+int sdfibm_create_polymesh(const sdfibm_polymesh_desc* d, const sdfibm_config* cfg, sdfibm_ctx** out) {
+    if (!out) return fail(nullptr, SDFIBM_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!d || !cfg) return fail(nullptr, SDFIBM_ERR_ARG, "mesh or config is NULL");
+    sdfibm_ctx* c = nullptr;
+    int rc = create_begin(cfg, &c);
+    if (rc) return rc;
+    auto bail = [&](int code) {
+        g_noctx_msg = c->msg;
+        sdfibm_destroy(c);
+        return code;
+    };
+    c->kind = SDFIBM_MESH_POLY;
+    c->nx = (int)std::min<long long>(d->n_cells, INT_MAX);
+    c->ny = c->nz = 1;
+    c->N = d->n_cells;
+    if ((rc = poly_setup(c, d))) return bail(rc);
+    if ((rc = create_end(c))) return bail(rc);
     *out = c;
     return SDFIBM_OK;
 }
@@ -2523,8 +3107,7 @@ int sdfibm_destroy(sdfibm_ctx* c) {
     if (!c) return SDFIBM_ERR_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
+    graph_reset(c);
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     for (auto& e : c->ev)
@@ -2559,6 +3142,7 @@ int sdfibm_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if (!c) return SDFIBM_ERR_ARG;
     if (n < 0 || (n > 0 && !solids)) return fail(c, SDFIBM_ERR_ARG, "bad solid array");
     if (n > (1LL << 30)) return fail(c, SDFIBM_ERR_ARG, "too many solids");
+    if (c->kind == SDFIBM_MESH_POLY) return poly_set_solids(c, solids, n);
     // Per-step updates of the same solids (poses and velocities only; the reference's solver
     // hands over its moved solids every step): no tile estimates, buffers, classes or captured
     // step to rebuild -- the interact grows its code buffer itself if moving tiles need it.
@@ -2694,10 +3278,9 @@ int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
                 for (int d = 0; d < 3; ++d)
                     if (std::isfinite(R)) e[d] = std::min<long long>(nn[d], (long long)std::ceil(2.0 * R / hmin[d]) + 2);
             }
-            if (e[0] <= kMaxTileX && (e[1] + 1) * (e[2] + 1) > kRowsSmall &&
-                (e[1] + 1) * (e[2] + 1) <= kRowsLarge)
-                ++n_lm;  // a medium or large tile in some pose
-            if (e[0] > kMaxTileX || (e[1] + 1) * (e[2] + 1) > kRowsLarge) {
+            if (e[0] > kMaxTileX || (e[1] + 1) * (e[2] + 1) > kRowsSmall)
+                ++n_lm;  // a medium (or big) tile in some pose
+            if (e[0] > kMaxTileX || (e[1] + 1) * (e[2] + 1) > kRowsMedium) {
                 big_on = true;
                 big_words += (e[0] + 2) * (e[1] + 2) * (e[2] + 2) / 64 + 2;
                 big_chunks += e[0] * e[1] * e[2] / kBigChunk + 2;
@@ -2878,6 +3461,7 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     if (!c) return SDFIBM_ERR_ARG;
     if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "interact before set_solids");
     if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
+    if (c->kind == SDFIBM_MESH_POLY) return poly_interact(c, dt, ft);
     int rc = dispatch_interact(c, dt);
     if (rc) return rc;
     // the flags come back through a pinned mapped word (no copy-engine round trip)
@@ -2964,7 +3548,17 @@ int sdfibm_update_velocity(sdfibm_ctx* c, double dt) {
 int sdfibm_fix_internal(sdfibm_ctx* c) {
     if (!c) return SDFIBM_ERR_ARG;
     if (!c->have_ct) return fail(c, SDFIBM_ERR_STATE, "fix_internal needs the Ct field of a successful interact");
-    launch_fix_internal(c);
+    if (c->kind == SDFIBM_MESH_POLY) {
+        const unsigned g = grid_for(c->N, 256);
+        if (c->prec == SDFIBM_FP64)
+            k_poly_fix<double><<<g, 256, 0, c->stream>>>(c->pm, c->Ct, c->d_solids, (int)c->ns, (double*)c->U[0],
+                                                         (double*)c->U[1], (double*)c->U[2]);
+        else
+            k_poly_fix<float><<<g, 256, 0, c->stream>>>(c->pm, c->Ct, c->d_solids, (int)c->ns, (float*)c->U[0],
+                                                        (float*)c->U[1], (float*)c->U[2]);
+    } else {
+        launch_fix_internal(c);
+    }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SDFIBM_OK;
@@ -3102,6 +3696,8 @@ int sdfibm_evolve(sdfibm_ctx* c, double time, double dt, const double* gravity) 
 
 int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, const double* gravity, double* ft) {
     if (!c) return SDFIBM_ERR_ARG;
+    if (c->kind == SDFIBM_MESH_POLY)
+        return fail(c, SDFIBM_ERR_STATE, "run_steps needs a structured mesh (a polyMesh step is not captured)");
     if (int rc = check_dynamics_ready(c, "run_steps")) return rc;
     if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
     if (n_steps < 0) return fail(c, SDFIBM_ERR_ARG, "n_steps must be >= 0");
@@ -3211,6 +3807,8 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
 
 int sdfibm_interact_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, double* ft) {
     if (!c) return SDFIBM_ERR_ARG;
+    if (c->kind == SDFIBM_MESH_POLY)
+        return fail(c, SDFIBM_ERR_STATE, "interact_steps needs a structured mesh (a polyMesh step is not captured)");
     if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "interact_steps before set_solids");
     if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
     if (n_steps < 0) return fail(c, SDFIBM_ERR_ARG, "n_steps must be >= 0");
@@ -3267,6 +3865,8 @@ int sdfibm_interact_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps
 
 int sdfibm_set_surrogate(sdfibm_ctx* c, const float* params, double band_lo, double band_hi, double max_angle_deg) {
     if (!c) return SDFIBM_ERR_ARG;
+    if (c->kind == SDFIBM_MESH_POLY && params)
+        return fail(c, SDFIBM_ERR_STATE, "the surrogate needs a rectilinear mesh");
     graph_reset(c);
     if (!params) {
         dfree(c, c->d_sg);
@@ -3389,6 +3989,7 @@ int sdfibm_candidates(sdfibm_ctx* c, int64_t solid, int64_t* counts, int64_t* ce
     if (!c) return SDFIBM_ERR_ARG;
     if (!c->have_step) return fail(c, SDFIBM_ERR_STATE, "candidates before a successful interact");
     if (solid < 0 || solid >= c->ns || !counts) return fail(c, SDFIBM_ERR_ARG, "bad solid index or counts");
+    if (c->kind == SDFIBM_MESH_POLY) return poly_candidates(c, solid, counts, cells, alpha, capacity);
     const size_t rs = rsize(c);
     TileInfo T;
     HIPCHK(c, hipMemcpy(&T, c->d_T + solid, sizeof(TileInfo), hipMemcpyDeviceToHost));
@@ -3595,6 +4196,7 @@ extern "C" {
 
 int sdfibm_autotune(sdfibm_ctx* c, double dt, const char* cache_path, char* chosen, int64_t chosen_cap) {
     if (!c) return SDFIBM_ERR_ARG;
+    if (c->kind == SDFIBM_MESH_POLY) return fail(c, SDFIBM_ERR_STATE, "autotune: a polyMesh step has no variants");
     if (!c->have_solids) return fail(c, SDFIBM_ERR_STATE, "autotune before set_solids");
     if (!(dt > 0) || !std::isfinite(dt)) return fail(c, SDFIBM_ERR_ARG, "dt must be finite and > 0");
     Tuning best{c->cells_wave, c->big_chunk, c->big_grid, 0.0};

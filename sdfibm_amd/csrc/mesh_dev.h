@@ -235,17 +235,19 @@ __device__ inline R cell_fraction(const MeshDev& m, const SolidK& K, int i, int 
 // per-solid tiles
 // ---------------------------------------------------------------------------
 // TC_BIG: a tile too big for the on-chip tile pass (> kMaxTileX cells along x or
-// > kRowsLarge vertex rows) but with fewer than kBigVerts vertices: the big-tile
-// pass (global-memory labelling, chunked cut cells) with the same outputs.
+// > kRowsMedium vertex rows, where one workgroup per solid would be the step's
+// long pole) but with fewer than kBigVerts vertices: the big-tile pass (flat
+// index spaces over the whole chip, global-memory labelling, chunked cut cells)
+// with the same outputs.
 // TC_TOOBIG: a tile beyond that; set_solids sends such a solid the planes' way
 // (TC_HUGE: cells over the mesh, union-find component of the seed, its tile box
 // as the cull).
 enum TileClass {
-    TC_EMPTY = 0, TC_SMALL = 1, TC_LARGE = 2, TC_PLANE = 3, TC_TOOBIG = 4, TC_MEDIUM = 5, TC_HUGE = 6, TC_BIG = 7
+    TC_EMPTY = 0, TC_SMALL = 1, TC_PLANE = 3, TC_TOOBIG = 4, TC_MEDIUM = 5, TC_HUGE = 6, TC_BIG = 7
 };
 
 __host__ __device__ inline bool tile_is_finite(int cls) {
-    return cls == TC_SMALL || cls == TC_MEDIUM || cls == TC_LARGE || cls == TC_BIG;
+    return cls == TC_SMALL || cls == TC_MEDIUM || cls == TC_BIG;
 }
 
 struct TileInfo {
@@ -259,8 +261,7 @@ struct TileInfo {
 constexpr int kMaxTileX = 63;      // cells along x per tile (vertex row fits one 64-bit word)
 constexpr int kRowsTiny = 64;      // vertex rows of a tiny tile (classified by one wave)
 constexpr int kRowsSmall = 400;    // vertex rows (ny+1)(nz+1) of a small tile
-constexpr int kRowsMedium = 1024;  // ... of a medium tile
-constexpr int kRowsLarge = 4096;   // ... of a large tile
+constexpr int kRowsMedium = 1024;  // ... of a medium tile (beyond: the big-tile pass)
 constexpr long long kBigVerts = 1LL << 22;  // vertices of a big tile (the big pass's 22-bit index arithmetic)
 
 // Searches in an increasing array a[0..n): start from a linear-interpolation
@@ -433,10 +434,10 @@ __host__ __device__ inline TileInfo make_tile(const MeshDev& m, const SolidK& K)
         T.n[d] = hi[d] - lo[d] + 1;
     }
     const long long vrows = (long long)(T.n[1] + 1) * (T.n[2] + 1);
-    if (T.n[0] > kMaxTileX || vrows > kRowsLarge)
+    if (T.n[0] > kMaxTileX || vrows > kRowsMedium)
         T.cls = (vrows * (T.n[0] + 1) < kBigVerts && T.n[0] < 4095 && T.n[1] < 4095) ? TC_BIG : TC_TOOBIG;
     else
-        T.cls = vrows > kRowsMedium ? TC_LARGE : (vrows > kRowsSmall ? TC_MEDIUM : TC_SMALL);
+        T.cls = vrows > kRowsSmall ? TC_MEDIUM : TC_SMALL;
     return T;
 }
 

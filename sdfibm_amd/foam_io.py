@@ -197,7 +197,10 @@ class PolyMesh:
 
     @property
     def n_cells(self) -> int:
-        return int(self.owner.max()) + 1
+        top = int(self.owner.max())
+        if self.neighbour.size:
+            top = max(top, int(self.neighbour.max()))
+        return top + 1
 
     @property
     def n_faces(self) -> int:

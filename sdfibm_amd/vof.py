@@ -4,8 +4,9 @@ tool_vof/solidcloud.cpp:141-173: clip(sum of fractions) per cell, no forcing).
 
     python -m sdfibm_amd.vof CASE_DIR [--dict solidDict] [--field alpha.water] [--ascii]
 
-The mesh must be one blockMesh block (points and cells i-fastest), read from
-CASE_DIR/constant/polyMesh; the fractions run on the GPU through the coupling library in FP64.
+The mesh is read from CASE_DIR/constant/polyMesh: one blockMesh block (points and cells
+i-fastest) runs on the structured path, any other mesh on the general polyMesh path; the
+fractions run on the GPU through the coupling library in FP64.
 """
 from __future__ import annotations
 
@@ -41,20 +42,20 @@ def block_dims(pm) -> list:
     return [(n, 1, 1)]
 
 
-def mesh_from_case(case_dir: str) -> M.StructuredHexMesh:
+def mesh_from_case(case_dir: str):
+    """The case's mesh: a StructuredHexMesh when it is one blockMesh block, else the PolyMesh."""
     pm = foam_io.read_polymesh(case_dir)
-    err = None
     for nx, ny, nz in block_dims(pm):
         if nx * ny * nz != pm.n_cells:
             continue
         try:
             return M.structured_from_polymesh(pm, nx, ny, nz)
-        except ValueError as e:  # numbering of another block shape
-            err = e
-    raise ValueError(f"{case_dir}: not a single-block blockMesh mesh ({err})")
+        except ValueError:  # numbering of another block shape
+            pass
+    return pm
 
 
-def vof_field(mesh: M.StructuredHexMesh, solids: np.ndarray, two_d: bool, precision: str = "fp64") -> np.ndarray:
+def vof_field(mesh, solids: np.ndarray, two_d: bool, precision: str = "fp64") -> np.ndarray:
     """clip(sum of the solids' cell fractions, 1) per cell."""
     with SolidCloud(mesh, two_d=two_d, rho_f=0.0, precision=precision) as c:
         c.set_solids(solids)
@@ -73,11 +74,10 @@ def main(argv=None) -> int:
     setup = load_solid_dict(os.path.join(a.case, a.dict))
     mesh = mesh_from_case(a.case)
     alpha = vof_field(mesh, setup.solids, setup.two_d)
-    g = mesh.geometry()
     os.makedirs(os.path.join(a.case, "0"), exist_ok=True)
     out = os.path.join(a.case, "0", a.field)
     foam_io.write_scalar_field(out, alpha, a.field, binary=not a.ascii)
-    print(f"wrote {out}: {mesh.n_cells} cells, total solid volume {float(np.sum(alpha * g['V'])):.9g}")
+    print(f"wrote {out}: {mesh.n_cells} cells")
     return 0
 
 

@@ -54,6 +54,31 @@ def cylinder_case():
     return m, sol, g
 
 
+def cylinder_re200():
+    """The nine-block mesh of examples/flow_past_cylinder/re200 regenerated from its
+    blockMeshDict numbers (fixture), the circle r = 1 at the origin, and the file's 0/As."""
+    from sdfibm_amd import blockmesh
+    g = np.load(os.path.join(GOLDEN, "cylinder_re200.npz"))
+    blocks = [(list(map(int, lab)), tuple(map(int, n)), tuple(map(float, gr)))
+              for lab, n, gr in zip(g["block_labels"], g["block_cells"], g["block_grading"])]
+    faces = g["patch_faces"].tolist()
+    patches, k = [], 0
+    for name, ptype, nf in zip(g["patch_names"], g["patch_types"], g["patch_nfaces"]):
+        patches.append((str(name), str(ptype), faces[k:k + int(nf)]))
+        k += int(nf)
+    pm, info = blockmesh.block_mesh(blockmesh.BlockMesh(g["vertices"], blocks, {}, patches))
+    sol = S.make_solids(1)
+    sid, prm = S.make_shape("Circle", radius=1.0)
+    sol["shape"] = sid
+    sol["prm"] = prm
+    return pm, info, sol, np.asarray(g["As"])
+
+
+# the one cell of the file no variant of the algorithm reproduces: (i=57, j=50) of the central
+# 120 x 120 block, whose cells start at 20,800 in the nine-block numbering
+CYLINDER_ODD_CELL = 20800 + 57 + 120 * 50
+
+
 def oracle_run(mesh, solids, U, two_d, rho_f=1.0, dt=1.0, mode=0, capture=True, om=None):
     om = om or oracle.OracleMesh(mesh.to_polymesh())
     return om, om.interact(solids, U, two_d=two_d, rho_f=rho_f, dt=dt, mode=mode, capture=capture)

@@ -16,6 +16,11 @@ cylinder_block5.npz
                  examples/flow_past_cylinder/re200/0/As restricted to blockMeshDict block 5
                  (the uniform 120 x 120 block on [-6,6]^2 holding the circle r = 1): the
                  14,400 values and the count of non-zero values outside the block (0).
+cylinder_re200.npz
+                 the same case whole: all 78,400 values of 0/As, and the numbers of its
+                 system/blockMeshDict (32 vertices; 9 hex blocks: vertex labels, cell counts,
+                 simpleGrading ratios; the boundary patches' names, types and block faces),
+                 from which the tests regenerate the nine-block mesh.
 """
 from __future__ import annotations
 
@@ -69,7 +74,18 @@ def main():
     np.savez_compressed(os.path.join(HERE, "cylinder_block5.npz"), As=blk, outside_nonzero=outside,
                         n_total=As.size, offset=20800, dims=np.array([120, 120, 1]),
                         lo=np.array([-6.0, -6.0, -0.5]), hi=np.array([6.0, 6.0, 0.5]))
-    print("wrote toolvof.npz, cylinder_block5.npz")
+    from sdfibm_amd import blockmesh  # noqa: E402
+    bm = blockmesh.read_block_mesh_dict(open(os.path.join(REF, "examples", "flow_past_cylinder", "re200", "system",
+                                                          "blockMeshDict")).read())
+    np.savez_compressed(os.path.join(HERE, "cylinder_re200.npz"), As=As, vertices=bm.vertices,
+                        block_labels=np.array([b[0] for b in bm.blocks], dtype=np.int32),
+                        block_cells=np.array([b[1] for b in bm.blocks], dtype=np.int32),
+                        block_grading=np.array([b[2] for b in bm.blocks], dtype=np.float64),
+                        patch_names=np.array([p[0] for p in bm.patches]),
+                        patch_types=np.array([p[1] for p in bm.patches]),
+                        patch_nfaces=np.array([len(p[2]) for p in bm.patches], dtype=np.int32),
+                        patch_faces=np.array([f for p in bm.patches for f in p[2]], dtype=np.int32))
+    print("wrote toolvof.npz, cylinder_block5.npz, cylinder_re200.npz")
 
 
 if __name__ == "__main__":

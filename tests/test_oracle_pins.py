@@ -11,7 +11,8 @@ import pytest
 from scipy import ndimage
 
 import oracle
-from _helpers import cylinder_case, oracle_run, split_candidates, toolvof_mesh, toolvof_solids
+from _helpers import (CYLINDER_ODD_CELL, cylinder_case, cylinder_re200, oracle_run, split_candidates,
+                      toolvof_mesh, toolvof_solids)
 from sdfibm_amd import mesh as M
 from sdfibm_amd import solids as S
 
@@ -69,6 +70,21 @@ def test_cylinder_fixture():
     assert abs(r["As"][62 + 120 * 50] - 0.667591) < 1e-6
     area = float((r["As"] * 0.01).sum())
     assert abs(area - (float((g["As"] * 0.01).sum()) - (1.0 - r["As"][bad[0]]) * 0.01)) < 1e-10
+
+
+def test_cylinder_nine_blocks():
+    """The same file, whole: the nine graded blocks regenerated from the blockMeshDict numbers
+    (blockmesh.py: merged points, blockMesh's face order) and run as a general polyMesh.  The
+    oracle reproduces all 78,400 values to 1e-9 except the one documented cell; the mesh has
+    blockMesh's sizes (157,922 points, 156,240 internal faces, the six patches)."""
+    pm, info, sol, As = cylinder_re200()
+    assert pm.points.shape[0] == 157922 and pm.n_cells == 78400 and pm.neighbour.size == 156240
+    assert info["patches"]["front"] == (156240, 78400) and info["patches"]["inlet"] == (313880, 280)
+    om = oracle.OracleMesh(pm)
+    r = om.interact(sol, np.zeros((pm.n_cells, 3)), two_d=True, rho_f=1.0, dt=1.0)
+    bad = np.nonzero(np.abs(r["As"] - As) > 1e-9)[0]
+    assert list(bad) == [CYLINDER_ODD_CELL]
+    assert np.count_nonzero(As) == 352 and np.abs(r["As"]).sum() > 300
 
 
 # ---------------------------------------------------------------------------------------------
