@@ -1948,8 +1948,8 @@ void launch_big(sdfibm_ctx* c, double dt, hipStream_t st) {
                                      c->err);
     k_big_vertices<MK><<<G, 256, 0, st>>>(c->m, c->d_K, c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits);
     k_big_cells<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, c->code_cap);
-    k_big_hook<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand, c->code_cap);
-    k_big_compress<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand, c->code_cap);
+    k_big_hook<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, c->code_cap);
+    k_big_compress<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, c->code_cap);
     k_big_member<R><<<G, 256, 0, st>>>(c->m, c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, codes,
                                        c->d_big_bcnt, c->code_cap, c->err);
     k_big_scan<<<(unsigned)c->n_cu, 1024, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_bcnt, c->d_big_boff,
@@ -2900,6 +2900,9 @@ int poly_step(sdfibm_ctx* c, double dt) {
 }
 
 int poly_interact(sdfibm_ctx* c, double dt, double* ft) {
+    // until this step succeeds, nothing reads the pair buffers (a failed growth frees them)
+    c->have_step = false;
+    c->have_ct = false;
     const int rc = c->prec == SDFIBM_FP64 ? poly_step<double>(c, dt) : poly_step<float>(c, dt);
     if (rc) return rc;
     if (ft && c->ns > 0) HIPCHK(c, hipMemcpyAsync(ft, c->d_ft, c->ns * 48, hipMemcpyDeviceToHost, c->stream));
@@ -3221,8 +3224,8 @@ int grow_big(sdfibm_ctx* c, long long words, long long chunks) {
         c->big_word_cap = 0;
         const long long want = words + words / 4 + 1024;
         if ((rc = dalloc(c, (void**)&c->d_big_vbits, want * 8))) return rc;
-        // cell blocks: at most want / 16 + kBigMaxSolids (k_big_layout checks)
-        const long long blocks = want / 16 + kBigMaxSolids;
+        // unit blocks: at most 4 want + kBigMaxSolids (k_big_layout checks)
+        const long long blocks = 4 * want + kBigMaxSolids;
         if ((rc = dalloc(c, (void**)&c->d_big_bcnt, blocks * sizeof(int2)))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_big_boff, blocks * sizeof(int2)))) return rc;
         c->big_word_cap = want;
@@ -3269,14 +3272,29 @@ int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
             ++hist[t.cls == TC_BIG ? 33 : std::min(31, 63 - __builtin_clzll((unsigned long long)nt))];
         }
         // can the solid's tile, in any pose, outgrow the on-chip tile pass?  (its bounding
-        // sphere in cells of the finest spacing)
+        // sphere in cells of the finest spacing: about the box of its local bounds, or its
+        // largest semi-axis for spheres and ellipsoids; the z-extruded 2-D shapes reach that far
+        // in the plane normal to their local z, whose current direction they span whole)
         {
             d3 lc, lh;
             long long e[3] = {nn[0], nn[1], nn[2]};
             if (local_bounds(K, lc, lh)) {
-                const double R = std::sqrt(dot(lc, lc)) + std::sqrt(dot(lh, lh));
-                for (int d = 0; d < 3; ++d)
-                    if (std::isfinite(R)) e[d] = std::min<long long>(nn[d], (long long)std::ceil(2.0 * R / hmin[d]) + 2);
+                const bool quad = K.shape == SDFIBM_SHAPE_SPHERE || K.shape == SDFIBM_SHAPE_ELLIPSOID;
+                const bool extruded = !(lh.z < 1e300);
+                double R;
+                if (quad)
+                    R = std::sqrt(dot(lc, lc)) + std::max(lh.x, std::max(lh.y, lh.z));
+                else if (extruded)
+                    R = std::sqrt(lc.x * lc.x + lc.y * lc.y) + std::sqrt(lh.x * lh.x + lh.y * lh.y);
+                else
+                    R = std::sqrt(dot(lc, lc)) + std::sqrt(dot(lh, lh));
+                double Rm[3][3];
+                rot_of(K.q, Rm);
+                for (int d = 0; d < 3; ++d) {
+                    const bool along = extruded && std::fabs(Rm[d][2]) > 1e-12;  // spans the mesh
+                    if (std::isfinite(R) && !along)
+                        e[d] = std::min<long long>(nn[d], (long long)std::ceil(2.0 * R / hmin[d]) + 2);
+                }
             }
             if (e[0] > kMaxTileX || (e[1] + 1) * (e[2] + 1) > kRowsSmall)
                 ++n_lm;  // a medium (or big) tile in some pose
