@@ -363,6 +363,7 @@ def bench_ellipsoids_loop(args):
     ph /= 5
     f = cloud.fields()
     nc, nv = touched_stats(sc.mesh, f["Ct"])
+    pairs_init = sum(cloud.pair_counts())
     cloud.run_steps(0.0, dt, max(args.warmup, 1), g)
     batches = 5
     per = max(1, steps // batches)
@@ -390,6 +391,30 @@ def bench_ellipsoids_loop(args):
     eager_ms = 1e3 * (time.perf_counter() - t) / k_eager
     fin = cloud.solids()
     cloud.close()
+    # the same initial interact through the general polyMesh path (the mesh handed over as
+    # points/faces/owner/neighbour; addressing and geometry derived by the library)
+    poly = None
+    if not args.no_poly:
+        t = time.perf_counter()
+        pm = sc.mesh.to_polymesh()
+        pc = SolidCloud(pm, rho_f=sc.rho_f, precision=args.precision)
+        setup_s = time.perf_counter() - t
+        pc.set_solids(sc.solids)
+        pc.set_velocity(sc.U)
+        pc.interact(0.0, dt)
+        pairs_poly = sum(pc.pair_counts())
+        pt = []
+        for _ in range(5):
+            t = time.perf_counter()
+            pc.interact_device(0.0, dt)
+            pt.append(1e3 * (time.perf_counter() - t))
+        tim = pc.timing()
+        pc.close()
+        poly = {"interact_ms": float(np.median(pt)), "device_ms": float(tim[3]),
+                "phases_ms": {"pairs_labels_force": float(tim[0]), "cell_lists": float(tim[1]), "fields": float(tim[2])},
+                "candidate_pairs": int(pairs_poly), "pairs_equal_structured": int(pairs_poly) == int(pairs_init),
+                "structured_interact_ms": float(ph[3]), "setup_s": setup_s,
+                "note": "interact of the initial state, eager (two counts read back per step)"}
     rb = 4 if args.precision == "fp32" else 8
     N = sc.mesh.n_cells
     kernels = {"k_fields": {"ms": ph[2], "bytes": (5 * rb + 4) * N + 3 * rb * nc},
@@ -416,6 +441,7 @@ def bench_ellipsoids_loop(args):
                      "traffic": measured_traffic(dom, f"ellipsoids3d_jittered_{n}_{nsol}_{args.precision}"),
                      "algorithmic_bytes": kernels[dom]["bytes"]},
         "surrogate": "not applicable: the learned fraction serves cubic cells of rectilinear meshes only",
+        "polymesh_path": poly,
         "mean_displacement": float(np.mean(np.linalg.norm(fin["center"][:nsol] - sc.solids["center"][:nsol], axis=1))),
     }
     if not args.no_cpu:
@@ -656,6 +682,8 @@ def main():
                     help="cloud3d: spheres of uniform diameter 2-4 cells instead of the lognormal mix")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="cloud3d fp32: skip the learned-fraction comparison run")
+    ap.add_argument("--no-poly", action="store_true",
+                    help="ellipsoids3d_jittered: skip the same interact through the general polyMesh path")
     ap.add_argument("--copy-test", type=float, default=0.0,
                     help="only run a plain device copy of this many GB (HBM counter calibration)")
     args = ap.parse_args()

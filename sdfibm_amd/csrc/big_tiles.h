@@ -18,12 +18,11 @@
 // small tiles' packed (i, j, k) does not fit 80-cell tiles).
 //
 // All big solids of a step are processed together, over flat index spaces
-// (their tile vertices; their tile cells as units of up to 64 cells of one row,
-// a wave each, in blocks of kBigUPB units, one workgroup iteration each, so a
-// workgroup finds its solid once; their candidate chunks) laid out by one scan
-// kernel; no host round trip, so the step stays capturable.  A unit's activity
-// comes from its vertex bits by word operations at wave-uniform addresses.
-//   k_big_layout    offsets of each big solid's vertex bits, cells, unit blocks, totals
+// (their tile vertices; their tile cells in blocks of kBigCPB, one workgroup
+// iteration each, so a workgroup finds its solid once; for the activity pass,
+// their rows in units of up to 64 cells, a wave each; their candidate chunks)
+// laid out by one scan kernel; no host round trip, so the step stays capturable.
+//   k_big_layout    offsets of each big solid's vertex bits, cells, cell and unit blocks, totals
 //   k_big_vertices  strict inside test per tile vertex (FP64) -> bit words
 //   k_big_cells     cells with >= 1 inside vertex -> union-find labels (in the
 //                   candidate-list buffer): the first cell of their run along x;
@@ -45,8 +44,9 @@
 constexpr int kBigPer = 16;                  // big solids per thread of the one-workgroup scans
 constexpr int kBigMaxSolids = 1024 * kBigPer;  // big tiles per step (SDFIBM_ERR_CAPACITY beyond)
 constexpr int kBigChunk = 1024;      // default candidate entries per chunk of k_big_force (the autotuner's choice)
-constexpr int kBigUPB = 16;          // units (64-cell row pieces) per block of the flat-cell kernels
+constexpr int kBigCPB = 1024;        // tile cells per block of the flat-cell kernels (4 per thread of 256)
 constexpr int kBigWPB = 32;          // vertex bit words per block of k_big_vertices (8 per wave of 4)
+constexpr int kBigUPB = 16;          // units (up to 64 cells of one row) per block of k_big_cells
 
 struct BigMeta {
     int solid;
@@ -56,16 +56,16 @@ struct BigMeta {
     long long cbase; // first flat cell index
     int chunk0;      // first chunk of its candidate entries
     int nchunk;
-    int blk0;        // first unit block (kBigUPB units of up to 64 cells of one row each)
+    int blk0;        // first cell block (kBigCPB tile cells each)
     int nblk;
     int vblk0;       // first vertex block (kBigWPB bit words each)
-    int pad;
+    int ublk0;       // first unit block (kBigUPB units each)
 };
 
 // Totals of the step's big tiles (BigTot fields as long long):
 // [0] big solids, [1] flat vertices (padded), [2] flat cells, [3] bit words, [4] chunks,
-// [5] bit words needed (for the host's buffer growth), [6] chunks needed, [7] unit blocks,
-// [8] vertex blocks
+// [5] bit words needed (for the host's buffer growth), [6] chunks needed, [7] cell blocks,
+// [8] vertex blocks, [9] unit blocks
 constexpr int kBigTotN = 10;
 
 // b with base[b] <= t < base[b + 1] (base ascending, n entries, base[0] = 0)
@@ -88,43 +88,46 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
                                                      const int* __restrict__ qmeta, const int* __restrict__ qlist,
                                                      BigMeta* __restrict__ meta, long long* __restrict__ tot,
                                                      long long word_cap, int* __restrict__ err) {
-    __shared__ long long part[4][1024];
+    __shared__ long long part[5][1024];
     const int tid = threadIdx.x;
     int nbig = qmeta[4];
     if (nbig > kBigMaxSolids) {
         if (tid == 0) atomicOr(err, EF_CODECAP);
         nbig = 0;
     }
-    long long w[kBigPer], c[kBigPer], k[kBigPer];
-    long long sw = 0, sc = 0, sk = 0, sv = 0;
+    long long w[kBigPer], c[kBigPer], un[kBigPer];
+    long long sw = 0, sc = 0, sk = 0, sv = 0, su = 0;
     for (int u = 0; u < kBigPer; ++u) {
         const int b = tid * kBigPer + u;
-        w[u] = c[u] = k[u] = 0;
+        w[u] = c[u] = un[u] = 0;
         if (b < nbig) {
             const TileInfo T = Ts[qlist[4LL * ns + b]];
             const long long V = (long long)(T.n[0] + 1) * (T.n[1] + 1) * (T.n[2] + 1);
             w[u] = (V + 63) / 64;
             c[u] = (long long)T.n[0] * T.n[1] * T.n[2];
-            k[u] = (long long)T.n[1] * T.n[2] * ((T.n[0] + 63) >> 6);  // units: 64-cell row pieces
+            un[u] = (long long)T.n[1] * T.n[2] * ((T.n[0] + 63) >> 6);
         }
         sw += w[u];
         sc += c[u];
-        sk += (k[u] + kBigUPB - 1) / kBigUPB;
+        sk += (c[u] + kBigCPB - 1) / kBigCPB;
         sv += (w[u] + kBigWPB - 1) / kBigWPB;
+        su += (un[u] + kBigUPB - 1) / kBigUPB;
     }
     part[0][tid] = sw;
     part[1][tid] = sc;
     part[2][tid] = sk;
     part[3][tid] = sv;
+    part[4][tid] = su;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan of the thread sums
-        long long a[4];
-        for (int r = 0; r < 4; ++r) a[r] = tid >= off ? part[r][tid - off] : 0;
+        long long a[5];
+        for (int r = 0; r < 5; ++r) a[r] = tid >= off ? part[r][tid - off] : 0;
         __syncthreads();
-        for (int r = 0; r < 4; ++r) part[r][tid] += a[r];
+        for (int r = 0; r < 5; ++r) part[r][tid] += a[r];
         __syncthreads();
     }
-    long long ow = part[0][tid] - sw, oc = part[1][tid] - sc, ok = part[2][tid] - sk, ov = part[3][tid] - sv;
+    long long ow = part[0][tid] - sw, oc = part[1][tid] - sc, ok = part[2][tid] - sk, ov = part[3][tid] - sv,
+              ou = part[4][tid] - su;
     for (int u = 0; u < kBigPer; ++u) {
         const int b = tid * kBigPer + u;
         if (b < nbig) {
@@ -137,21 +140,23 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
             M.chunk0 = 0;
             M.nchunk = 0;
             M.blk0 = (int)ok;
-            M.nblk = (int)((k[u] + kBigUPB - 1) / kBigUPB);
+            M.nblk = (int)((c[u] + kBigCPB - 1) / kBigCPB);
             M.vblk0 = (int)ov;
-            M.pad = 0;
+            M.ublk0 = (int)ou;
             meta[b] = M;
         }
         ow += w[u];
         oc += c[u];
-        ok += (k[u] + kBigUPB - 1) / kBigUPB;
+        ok += (c[u] + kBigCPB - 1) / kBigCPB;
         ov += (w[u] + kBigWPB - 1) / kBigWPB;
+        ou += (un[u] + kBigUPB - 1) / kBigUPB;
     }
     if (tid == 1023) {
-        const long long words = part[0][1023], cells = part[1][1023], blocks = part[2][1023], vblocks = part[3][1023];
-        // the block counts live in a buffer of 4 words + kBigMaxSolids entries (a tile has at least
-        // as many vertices as units, so blocks <= words * 64 / kBigUPB + big solids)
-        const bool fits = words + 1 <= word_cap && blocks <= 4 * word_cap + kBigMaxSolids;  // + the guard word
+        const long long words = part[0][1023], cells = part[1][1023], blocks = part[2][1023], vblocks = part[3][1023],
+                        ublocks = part[4][1023];
+        // the block counts live in a buffer of words / 16 + kBigMaxSolids entries (a tile has at
+        // least as many vertices as cells, so blocks <= words * 64 / kBigCPB + big solids)
+        const bool fits = words + 1 <= word_cap && blocks <= word_cap / 16 + kBigMaxSolids;  // + the guard word
         if (!fits) atomicOr(err, EF_CODECAP);
         tot[0] = fits ? nbig : 0;
         tot[1] = fits ? words * 64 : 0;
@@ -160,6 +165,7 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
         tot[5] = words;
         tot[7] = fits ? blocks : 0;
         tot[8] = fits ? vblocks : 0;
+        tot[9] = fits ? ublocks : 0;
     }
 }
 
@@ -201,6 +207,49 @@ __global__ void __launch_bounds__(256) k_big_vertices(MeshDev m, const SolidK* _
 
 __device__ inline bool big_vbit(const unsigned long long* w, long long v) { return (w[v >> 6] >> (v & 63)) & 1ull; }
 
+// Inside-vertex count of tile cell (i, j, k) from the vertex bits.
+__device__ inline int big_nin(const unsigned long long* w, int i, int j, int k, int VX, int VY) {
+    const long long v0 = i + (long long)VX * (j + (long long)VY * k);
+    const long long dy = VX, dz = (long long)VX * VY;
+    return (int)big_vbit(w, v0) + (int)big_vbit(w, v0 + 1) + (int)big_vbit(w, v0 + dy) + (int)big_vbit(w, v0 + dy + 1) +
+           (int)big_vbit(w, v0 + dz) + (int)big_vbit(w, v0 + dz + 1) + (int)big_vbit(w, v0 + dz + dy) +
+           (int)big_vbit(w, v0 + dz + dy + 1);
+}
+
+// Iterates the cell blocks of the step's big tiles, one workgroup iteration per
+// block (the block's solid found once, uniform): f(b, meta, tile, block, q0, q1)
+// for the tile-local cells [q0, q1).
+template <class F>
+__device__ inline void big_blocks_loop(const BigMeta* meta, const TileInfo* Ts, const long long* tot, long long code_cap,
+                                       F f) {
+    const int nbig = (int)tot[0];
+    const long long nblk = tot[7];
+    for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int b = big_find(nbig, blk, [&](int x) { return (long long)meta[x].blk0; });
+        const BigMeta M = meta[b];
+        const TileInfo T = Ts[M.solid];
+        const int C = T.n[0] * T.n[1] * T.n[2];
+        if (T.off + (long long)C > code_cap) continue;  // flagged by the tile offsets
+        const int q0 = (int)(blk - M.blk0) * kBigCPB;
+        f(b, M, T, blk, q0, min(q0 + kBigCPB, C));
+    }
+}
+
+// Iterates the flat cells of the step's big tiles: f(b, meta, tile, q, i, j, k).
+template <class F>
+__device__ inline void big_cells_loop(const BigMeta* meta, const TileInfo* Ts, const long long* tot, long long code_cap,
+                                      F f) {
+    big_blocks_loop(meta, Ts, tot, code_cap, [&](int b, const BigMeta& M, const TileInfo& T, long long, int q0, int q1) {
+        const float r0 = 1.0f / (float)T.n[0], r1 = 1.0f / (float)T.n[1];
+        for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+            int r, i, j, k;
+            divmod_small(q, T.n[0], r0, r, i);
+            divmod_small(r, T.n[1], r1, k, j);
+            f(b, M, T, q, i, j, k);
+        }
+    });
+}
+
 // 64 bits of a bit array starting at bit pos (the word after pos's word is read:
 // the vertex-bit buffer keeps one guard word after the last solid's words).
 __device__ inline unsigned long long big_bits64(const unsigned long long* w, long long pos) {
@@ -208,6 +257,27 @@ __device__ inline unsigned long long big_bits64(const unsigned long long* w, lon
     const int sh = (int)(pos & 63);
     const unsigned long long lo = w[wi] >> sh;
     return sh == 0 ? lo : (lo | (w[wi + 1] << (64 - sh)));
+}
+
+// First cell of the run of active cells along x that holds active cell i of row
+// (j, k): one past the highest inactive cell below i (0 if none).  Cell p is
+// inactive when none of its 8 vertices is inside: with R(p) the OR of the row's
+// four vertex rows at vertex p, when R(p) = R(p + 1) = 0.  Windows of 63 cells.
+__device__ inline int big_run_start(const unsigned long long* w, int i, int j, int k, int VX, int VY) {
+    const long long r00 = (long long)VX * (j + (long long)VY * k), r10 = r00 + VX;
+    const long long r01 = r00 + (long long)VX * VY, r11 = r01 + VX;
+    int pos = i - 1;  // highest candidate cell
+    while (pos >= 0) {
+        const int lo = pos >= 62 ? pos - 62 : 0;
+        const unsigned long long R = big_bits64(w, r00 + lo) | big_bits64(w, r10 + lo) | big_bits64(w, r01 + lo) |
+                                     big_bits64(w, r11 + lo);
+        // cells lo .. pos inactive where R(p) = R(p + 1) = 0
+        const unsigned long long Z = ~R & ~(R >> 1) & ((2ull << (pos - lo)) - 1ull);
+        if (Z) return lo + (63 - __builtin_clzll(Z)) + 1;
+        if (lo == 0) return 0;
+        pos = lo - 1;
+    }
+    return 0;
 }
 
 // A unit: up to 64 consecutive cells i0 .. i0 + len - 1 of tile row (j, k) --
@@ -232,27 +302,6 @@ __device__ inline void big_unit_masks(const unsigned long long* w, int VX, int V
     const unsigned long long valid = len == 64 ? ~0ull : ((1ull << len) - 1ull);
     act = (O | (O >> 1) | (hiO << 63)) & valid;
     all8 = (A & ((A >> 1) | (hiA << 63))) & valid;
-}
-
-// First cell of the run of active cells along x that holds active cell i of row
-// (j, k): one past the highest inactive cell below i (0 if none).  Cell p is
-// inactive when none of its 8 vertices is inside: with R(p) the OR of the row's
-// four vertex rows at vertex p, when R(p) = R(p + 1) = 0.  Windows of 63 cells.
-__device__ inline int big_run_start(const unsigned long long* w, int i, int j, int k, int VX, int VY) {
-    const long long r00 = (long long)VX * (j + (long long)VY * k), r10 = r00 + VX;
-    const long long r01 = r00 + (long long)VX * VY, r11 = r01 + VX;
-    int pos = i - 1;  // highest candidate cell
-    while (pos >= 0) {
-        const int lo = pos >= 62 ? pos - 62 : 0;
-        const unsigned long long R = big_bits64(w, r00 + lo) | big_bits64(w, r10 + lo) | big_bits64(w, r01 + lo) |
-                                     big_bits64(w, r11 + lo);
-        // cells lo .. pos inactive where R(p) = R(p + 1) = 0
-        const unsigned long long Z = ~R & ~(R >> 1) & ((2ull << (pos - lo)) - 1ull);
-        if (Z) return lo + (63 - __builtin_clzll(Z)) + 1;
-        if (lo == 0) return 0;
-        pos = lo - 1;
-    }
-    return 0;
 }
 
 // Unit u of a tile: its row (j, k), first cell i0, length and first tile index.
@@ -283,15 +332,15 @@ template <class F, class Pre, class Post>
 __device__ inline void big_units_loop(const BigMeta* meta, const TileInfo* Ts, const long long* tot, long long code_cap,
                                       F f, Pre pre, Post post) {
     const int nbig = (int)tot[0];
-    const long long nblk = tot[7];
+    const long long nblk = tot[9];
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-        const int b = big_find(nbig, blk, [&](int x) { return (long long)meta[x].blk0; });
+        const int b = big_find(nbig, blk, [&](int x) { return (long long)meta[x].ublk0; });
         const BigMeta M = meta[b];
         const TileInfo T = Ts[M.solid];
         if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) continue;  // flagged by the tile offsets
         const int units = big_units(T);
-        const int ub = (int)(blk - M.blk0) * kBigUPB;
+        const int ub = (int)(blk - M.ublk0) * kBigUPB;
         pre(b, M, T, blk);
         for (int t = 0; t < kBigUPB / 4; ++t) {
             const int ui = wv * (kBigUPB / 4) + t;
@@ -346,60 +395,39 @@ __global__ void __launch_bounds__(256) k_big_cells(const TileInfo* __restrict__ 
 
 // Union of the runs over the +y and +z neighbours (hook the larger root under
 // the smaller: the same concurrent union as the planes' labelling), once per
-// overlap of two runs within a unit -- at the overlap's first cell (a unit's
-// first cell always, which repeats a union at most once per unit boundary).
+// overlap of two runs -- at its first cell, where the cells before the pair are
+// not both active.
 __global__ void __launch_bounds__(256) k_big_hook(const TileInfo* __restrict__ Ts, const BigMeta* __restrict__ meta,
-                                                  const long long* __restrict__ tot,
-                                                  const unsigned long long* __restrict__ vbits, int* __restrict__ cand,
+                                                  const long long* __restrict__ tot, int* __restrict__ cand,
                                                   long long code_cap) {
-    const int lane = threadIdx.x & 63;
-    big_units_loop(
-        meta, Ts, tot, code_cap,
-        [&](int, const BigMeta& M, const TileInfo& T, long long, int, const BigUnit U) {
-            const unsigned long long* w = vbits + M.woff;
-            const int VX = T.n[0] + 1, VY = T.n[1] + 1;
-            unsigned long long act, all8, ay = 0, az = 0;
-            big_unit_masks(w, VX, VY, U.j, U.k, U.i0, U.len, act, all8);
-            if (!act) return;
-            if (U.j + 1 < T.n[1]) big_unit_masks(w, VX, VY, U.j + 1, U.k, U.i0, U.len, ay, all8);
-            if (U.k + 1 < T.n[2]) big_unit_masks(w, VX, VY, U.j, U.k + 1, U.i0, U.len, az, all8);
-            const unsigned long long oy = act & ay, oz = act & az;
-            const unsigned long long sy = oy & ~(oy << 1), sz = oz & ~(oz << 1);
-            int* L = cand + T.off;
-            const int q = U.q0 + lane;
-            const int nx = T.n[0], nxy = T.n[0] * T.n[1];
-            for (int d = 0; d < 2; ++d) {
-                if (!(((d == 0 ? sy : sz) >> lane) & 1ull)) continue;
-                int a = uf_find(L, q), b2 = uf_find(L, q + (d == 0 ? nx : nxy));
-                while (a != b2) {
-                    const int hi = a > b2 ? a : b2, lo = a > b2 ? b2 : a;
-                    const int old = atomicMin(&L[hi], lo);
-                    if (old == hi) break;  // hi was a root and now points at lo
-                    a = uf_find(L, old);
-                    b2 = uf_find(L, lo);
-                }
+    big_cells_loop(meta, Ts, tot, code_cap, [&](int, const BigMeta&, const TileInfo& T, int q, int i, int j, int k) {
+        int* L = cand + T.off;
+        if (L[q] < 0) return;
+        const int nx = T.n[0], nxy = T.n[0] * T.n[1];
+        const bool left = i > 0 && L[q - 1] >= 0;
+        int nb[2], nn = 0;
+        if (j + 1 < T.n[1] && L[q + nx] >= 0 && !(left && L[q - 1 + nx] >= 0)) nb[nn++] = q + nx;
+        if (k + 1 < T.n[2] && L[q + nxy] >= 0 && !(left && L[q - 1 + nxy] >= 0)) nb[nn++] = q + nxy;
+        for (int u = 0; u < nn; ++u) {
+            int a = uf_find(L, q), b2 = uf_find(L, nb[u]);
+            while (a != b2) {
+                const int hi = a > b2 ? a : b2, lo = a > b2 ? b2 : a;
+                const int old = atomicMin(&L[hi], lo);
+                if (old == hi) break;  // hi was a root and now points at lo
+                a = uf_find(L, old);
+                b2 = uf_find(L, lo);
             }
-        },
-        BigNone{}, BigNone{});
+        }
+    });
 }
 
 __global__ void __launch_bounds__(256) k_big_compress(const TileInfo* __restrict__ Ts, const BigMeta* __restrict__ meta,
-                                                      const long long* __restrict__ tot,
-                                                      const unsigned long long* __restrict__ vbits,
-                                                      int* __restrict__ cand, long long code_cap) {
-    const int lane = threadIdx.x & 63;
-    big_units_loop(
-        meta, Ts, tot, code_cap,
-        [&](int, const BigMeta& M, const TileInfo& T, long long, int, const BigUnit U) {
-            unsigned long long act, all8;
-            big_unit_masks(vbits + M.woff, T.n[0] + 1, T.n[1] + 1, U.j, U.k, U.i0, U.len, act, all8);
-            if ((act >> lane) & 1ull) {
-                int* L = cand + T.off;
-                const int q = U.q0 + lane;
-                L[q] = uf_find(L, q);
-            }
-        },
-        BigNone{}, BigNone{});
+                                                      const long long* __restrict__ tot, int* __restrict__ cand,
+                                                      long long code_cap) {
+    big_cells_loop(meta, Ts, tot, code_cap, [&](int, const BigMeta&, const TileInfo& T, int q, int, int, int) {
+        int* L = cand + T.off;
+        if (L[q] >= 0) L[q] = uf_find(L, q);
+    });
 }
 
 // Codes of the seed's component; every active cell's label must equal its active
@@ -411,75 +439,52 @@ __global__ void __launch_bounds__(256) k_big_member(const MeshDev m, const TileI
                                                     const unsigned long long* __restrict__ vbits,
                                                     const int* __restrict__ cand, R* __restrict__ codes,
                                                     int2* __restrict__ bcnt, long long code_cap, int* __restrict__ err) {
-    __shared__ int wcnt[2][4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int nall = 0, ncut = 0;  // the wave's counts in the current block (uniform)
-    big_units_loop(
-        meta, Ts, tot, code_cap,
-        [&](int, const BigMeta& M, const TileInfo& T, long long, int, const BigUnit U) {
-            const int* L = cand + T.off;
-            R* tc = codes + T.off;
-            const unsigned long long* w = vbits + M.woff;
-            const int VX = T.n[0] + 1, VY = T.n[1] + 1;
-            // the seed: the nearest cell when it has an inside vertex, else the lowest such cell
-            int seed = -1;
-            if (T.seed >= 0) {
-                const int si = T.seed % m.nx, sj = (T.seed / m.nx) % m.ny, sk = T.seed / (m.nx * m.ny);
-                const int li = si - T.lo[0], lj = sj - T.lo[1], lk = sk - T.lo[2];
-                if (li >= 0 && li < T.n[0] && lj >= 0 && lj < T.n[1] && lk >= 0 && lk < T.n[2]) {
-                    const int sq = li + T.n[0] * (lj + T.n[1] * lk);
-                    if (L[sq] >= 0) seed = sq;
-                }
+    __shared__ int scan_lds[32];
+    big_blocks_loop(meta, Ts, tot, code_cap, [&](int, const BigMeta& M, const TileInfo& T, long long blk, int q0, int q1) {
+        const int* L = cand + T.off;
+        R* tc = codes + T.off;
+        // the seed: the nearest cell when it has an inside vertex, else the lowest such cell
+        int seed = -1;
+        if (T.seed >= 0) {
+            const int si = T.seed % m.nx, sj = (T.seed / m.nx) % m.ny, sk = T.seed / (m.nx * m.ny);
+            const int li = si - T.lo[0], lj = sj - T.lo[1], lk = sk - T.lo[2];
+            if (li >= 0 && li < T.n[0] && lj >= 0 && lj < T.n[1] && lk >= 0 && lk < T.n[2]) {
+                const int sq = li + T.n[0] * (lj + T.n[1] * lk);
+                if (L[sq] >= 0) seed = sq;
             }
-            if (seed < 0) seed = M.first;
-            const int lseed = seed >= 0 && seed != 0x7fffffff ? L[seed] : -2;
-            unsigned long long act, all8, ay = 0, az = 0, dummy;
-            big_unit_masks(w, VX, VY, U.j, U.k, U.i0, U.len, act, all8);
-            const int q = U.q0 + lane;
-            const bool a = (act >> lane) & 1ull;
-            const int lq = a ? L[q] : -1;
-            bool member = false;
-            if (act) {
-                if (U.j + 1 < T.n[1]) big_unit_masks(w, VX, VY, U.j + 1, U.k, U.i0, U.len, ay, dummy);
-                if (U.k + 1 < T.n[2]) big_unit_masks(w, VX, VY, U.j, U.k + 1, U.i0, U.len, az, dummy);
-                const int lx = __shfl_down(lq, 1, 64);  // the next cell along x in the unit
-                const int nx = T.n[0], nxy = T.n[0] * T.n[1];
-                bool bad = false;
-                if (a) {
-                    if (lane + 1 < U.len) {
-                        bad = bad || ((act >> (lane + 1)) & 1ull && lx != lq);
-                    } else if (U.i0 + lane + 1 < T.n[0]) {
-                        const int nb = L[q + 1];
-                        bad = bad || (nb >= 0 && nb != lq);
-                    }
-                    bad = bad || (((ay >> lane) & 1ull) && L[q + nx] != lq) || (((az >> lane) & 1ull) && L[q + nxy] != lq);
-                }
-                if (bad) atomicOr(err, EF_LABEL);
-                member = a && lq == lseed;
+        }
+        if (seed < 0) seed = M.first;
+        const int lseed = seed >= 0 && seed != 0x7fffffff ? L[seed] : -2;
+        const int nx = T.n[0], nxy = T.n[0] * T.n[1];
+        const float r0 = 1.0f / (float)T.n[0], r1 = 1.0f / (float)T.n[1];
+        int nall = 0, ncut = 0;
+        for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+            const int lq = L[q];
+            if (lq < 0) {
+                tc[q] = code_none<R>();
+                continue;
             }
-            if (lane < U.len) {
-                const bool in8 = (all8 >> lane) & 1ull;
-                tc[q] = member ? (in8 ? R(__builtin_huge_val()) : R(0)) : code_none<R>();
+            int r, i, j, k;
+            divmod_small(q, T.n[0], r0, r, i);
+            divmod_small(r, T.n[1], r1, k, j);
+            const bool bad = (i + 1 < T.n[0] && L[q + 1] >= 0 && L[q + 1] != lq) ||
+                             (j + 1 < T.n[1] && L[q + nx] >= 0 && L[q + nx] != lq) ||
+                             (k + 1 < T.n[2] && L[q + nxy] >= 0 && L[q + nxy] != lq);
+            if (bad) atomicOr(err, EF_LABEL);
+            if (lq != lseed) {
+                tc[q] = code_none<R>();
+            } else {
+                const bool all8 = big_nin(vbits + M.woff, i, j, k, T.n[0] + 1, T.n[1] + 1) == 8;
+                tc[q] = all8 ? R(__builtin_huge_val()) : R(0);
+                nall += all8 ? 1 : 0;
+                ncut += all8 ? 0 : 1;
             }
-            const unsigned long long mm = __ballot(member);
-            nall += __popcll(mm & all8);
-            ncut += __popcll(mm & ~all8);
-        },
-        [&](int, const BigMeta&, const TileInfo&, long long) {
-            nall = 0;
-            ncut = 0;
-        },
-        [&](int, const BigMeta&, const TileInfo&, long long blk) {
-            if (lane == 0) {
-                wcnt[0][wv] = nall;
-                wcnt[1][wv] = ncut;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0)
-                bcnt[blk] = make_int2(wcnt[0][0] + wcnt[0][1] + wcnt[0][2] + wcnt[0][3],
-                                      wcnt[1][0] + wcnt[1][1] + wcnt[1][2] + wcnt[1][3]);
-            __syncthreads();
-        });
+        }
+        int pa, pc, ta, tcut;
+        block_exscan2<256>(nall, ncut, pa, pc, ta, tcut, scan_lds);
+        if (threadIdx.x == 0) bcnt[blk] = make_int2(ta, tcut);
+        __syncthreads();  // scan_lds is reused by the next block
+    });
 }
 
 // Per big solid (one workgroup each): its blocks' counts scanned in block order
@@ -515,56 +520,39 @@ __global__ void __launch_bounds__(1024) k_big_scan(const TileInfo* __restrict__ 
 
 // Candidate lists of the big tiles: ALL_INSIDE cells first, then cut cells,
 // each ascending (tile-local linear indices); per block a stable compaction of
-// its units (in order, each unit's cells by lane) at the offsets k_big_scan gave it.
+// its cells (4 consecutive per thread) at the offsets k_big_scan gave it.
 template <class R>
 __global__ void __launch_bounds__(256) k_big_write(const TileInfo* __restrict__ Ts, const BigMeta* __restrict__ meta,
                                                    const long long* __restrict__ tot, const R* __restrict__ codes,
                                                    const int2* __restrict__ boff, const int2* __restrict__ ncand,
                                                    int* __restrict__ cand, long long code_cap) {
-    __shared__ int ucnt[2][kBigUPB];
-    __shared__ unsigned long long umask[2][kBigUPB];
-    const int lane = threadIdx.x & 63;
-    big_units_loop(
-        meta, Ts, tot, code_cap,
-        // pass 1: each unit's masks and counts
-        [&](int, const BigMeta&, const TileInfo& T, long long, int ui, const BigUnit U) {
-            const R v = lane < U.len ? codes[T.off + U.q0 + lane] : code_none<R>();
-            const unsigned long long mi = __ballot(v == R(__builtin_huge_val())), mc = __ballot(v == R(0));
-            if (lane == 0) {
-                umask[0][ui] = mi;
-                umask[1][ui] = mc;
-                ucnt[0][ui] = __popcll(mi);
-                ucnt[1][ui] = __popcll(mc);
-            }
-        },
-        [&](int, const BigMeta&, const TileInfo&, long long) {
-            if (threadIdx.x < kBigUPB) ucnt[0][threadIdx.x] = ucnt[1][threadIdx.x] = 0;
-            __syncthreads();
-        },
-        // pass 2: the entries at the block's offsets plus the units before
-        [&](int, const BigMeta& M, const TileInfo& T, long long blk) {
-            __syncthreads();
-            const int units = big_units(T), ub = (int)(blk - M.blk0) * kBigUPB;
-            const int2 o = boff[blk];
-            const int tin = ncand[M.solid].x;
-            int* tl = cand + T.off;
-            const int wv = threadIdx.x >> 6;
-            for (int t = 0; t < kBigUPB / 4; ++t) {
-                const int ui = wv * (kBigUPB / 4) + t;
-                if (ub + ui >= units) break;
-                int pi = 0, pc = 0;
-                for (int x = 0; x < ui; ++x) {
-                    pi += ucnt[0][x];
-                    pc += ucnt[1][x];
-                }
-                const BigUnit U = big_unit(T, ub + ui);
-                const unsigned long long below = (1ull << lane) - 1ull;
-                const unsigned long long mi = umask[0][ui], mc = umask[1][ui];
-                if ((mi >> lane) & 1ull) tl[o.x + pi + __popcll(mi & below)] = U.q0 + lane;
-                if ((mc >> lane) & 1ull) tl[tin + o.y + pc + __popcll(mc & below)] = U.q0 + lane;
-            }
-            __syncthreads();
-        });
+    static_assert(kBigCPB == 4 * 256, "4 cells per thread");
+    __shared__ int scan_lds[32];
+    big_blocks_loop(meta, Ts, tot, code_cap, [&](int, const BigMeta& M, const TileInfo& T, long long blk, int q0, int q1) {
+        const R* tc = codes + T.off;
+        const int qa = q0 + 4 * (int)threadIdx.x;
+        int cls[4];
+        int na = 0, nc = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = qa + u;
+            const R v = q < q1 ? tc[q] : code_none<R>();
+            cls[u] = v == R(__builtin_huge_val()) ? 1 : (v == R(0) ? 2 : 0);
+            na += cls[u] == 1 ? 1 : 0;
+            nc += cls[u] == 2 ? 1 : 0;
+        }
+        int pa, pc, ta, tcut;
+        block_exscan2<256>(na, nc, pa, pc, ta, tcut, scan_lds);
+        const int2 o = boff[blk];
+        int* tl = cand + T.off;
+        int wa = o.x + pa, wc = ncand[M.solid].x + o.y + pc;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (cls[u] == 1) tl[wa++] = qa + u;
+            if (cls[u] == 2) tl[wc++] = qa + u;
+        }
+        __syncthreads();
+    });
 }
 
 // Chunk offsets (one workgroup, exclusive scan over the big solids).

@@ -1948,8 +1948,8 @@ void launch_big(sdfibm_ctx* c, double dt, hipStream_t st) {
                                      c->err);
     k_big_vertices<MK><<<G, 256, 0, st>>>(c->m, c->d_K, c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits);
     k_big_cells<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, c->code_cap);
-    k_big_hook<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, c->code_cap);
-    k_big_compress<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, c->code_cap);
+    k_big_hook<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand, c->code_cap);
+    k_big_compress<<<G, 256, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_cand, c->code_cap);
     k_big_member<R><<<G, 256, 0, st>>>(c->m, c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_vbits, c->d_cand, codes,
                                        c->d_big_bcnt, c->code_cap, c->err);
     k_big_scan<<<(unsigned)c->n_cu, 1024, 0, st>>>(c->d_T, c->d_big_meta, c->d_big_tot, c->d_big_bcnt, c->d_big_boff,
@@ -2828,8 +2828,8 @@ int poly_step(sdfibm_ctx* c, double dt) {
             c->g_vcap = want;
         }
         if ((rc = poly_grow(c, c->g_tmp, c->g_tmpcap, nv / kScanTile + 2, 8))) return rc;
-        k_poly_visit<<<grid_for(ns, 4), 256, 0, st>>>(ns, c->pm, c->g_box, c->g_voff, c->g_vcell, c->g_vsolid,
-                                                      c->poly_reach);
+        k_poly_visit<<<grid_for(ns, 4), 256, 0, st>>>(ns, c->pm, c->d_K, c->g_box, c->g_voff, c->g_vcell,
+                                                      c->g_vsolid, c->poly_reach);
         if (nv > 0)
             k_poly_test<<<G, 256, 0, st>>>(c->pm, c->d_K, nv, c->g_vcell, c->g_vsolid, c->g_act, c->g_all, c->g_first);
         scan<unsigned char>(c->g_act, nv, c->g_vpos, c->g_tmp, st);
@@ -3224,8 +3224,8 @@ int grow_big(sdfibm_ctx* c, long long words, long long chunks) {
         c->big_word_cap = 0;
         const long long want = words + words / 4 + 1024;
         if ((rc = dalloc(c, (void**)&c->d_big_vbits, want * 8))) return rc;
-        // unit blocks: at most 4 want + kBigMaxSolids (k_big_layout checks)
-        const long long blocks = 4 * want + kBigMaxSolids;
+        // cell blocks: at most want / 16 + kBigMaxSolids (k_big_layout checks)
+        const long long blocks = want / 16 + kBigMaxSolids;
         if ((rc = dalloc(c, (void**)&c->d_big_bcnt, blocks * sizeof(int2)))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_big_boff, blocks * sizeof(int2)))) return rc;
         c->big_word_cap = want;

@@ -101,6 +101,20 @@ __device__ inline bool poly_box_hit(const double* bb, const double* lo, const do
     return bb[0] <= hi[0] && bb[3] >= lo[0] && bb[1] <= hi[1] && bb[4] >= lo[1] && bb[2] <= hi[2] && bb[5] >= lo[2];
 }
 
+// A bin's box can hold a point of the solid: it meets the solid's world box and, for a plane
+// (inside: local y < 0), its lowest local y is below 0.
+__device__ inline bool poly_bin_hit(const SolidK& K, const double* bb, const double* lo, const double* hi) {
+    if (!poly_box_hit(bb, lo, hi)) return false;
+    if (K.shape != SDFIBM_SHAPE_PLANE) return true;
+    double y = 0.0, r = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double n = K.rt[3 + d], c = 0.5 * (bb[d] + bb[3 + d]) - K.c[d], h = 0.5 * (bb[3 + d] - bb[d]);
+        y += n * c;
+        r += fabs(n) * h;
+    }
+    return y - r < 0.0;
+}
+
 // Per solid (one wave; 4 per workgroup): constants, world box (a cell can hold an
 // inside point only if its point box meets it), the nearest cell, and the number
 // of cells in the bins whose cell boxes meet the solid's box.  A bin's cells can
@@ -144,7 +158,7 @@ __global__ void __launch_bounds__(256) k_poly_setup(const sdfibm_solid* __restri
     for (long long t = lane; t < nb; t += 64) {
         const int bi = b0[0] + (int)(t % ex), bj = b0[1] + (int)((t / ex) % ey), bk = b0[2] + (int)(t / ((long long)ex * ey));
         const long long b = bi + (long long)P.bn[0] * (bj + (long long)P.bn[1] * bk);
-        if (poly_box_hit(P.b_box + 6 * b, lo, hi)) cnt += P.b_off[b + 1] - P.b_off[b];
+        if (poly_bin_hit(K, P.b_box + 6 * b, lo, hi)) cnt += P.b_off[b + 1] - P.b_off[b];
     }
     for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
     if (lane == 0) vcnt[s] = cnt;
@@ -225,12 +239,14 @@ __global__ void __launch_bounds__(256) k_poly_setup(const sdfibm_solid* __restri
 
 // The visited cells of each solid (one wave per solid), bin by bin in bin index
 // order, each bin's cells ascending.
-__global__ void __launch_bounds__(256) k_poly_visit(int ns, PolyDev P, const double* __restrict__ box,
+__global__ void __launch_bounds__(256) k_poly_visit(int ns, PolyDev P, const SolidK* __restrict__ Ks,
+                                                    const double* __restrict__ box,
                                                     const long long* __restrict__ voff, int* __restrict__ vcell,
                                                     int* __restrict__ vsolid, int reach) {
     const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (s >= ns) return;
+    const SolidK K = Ks[s];
     const double* lo = box + 6 * s;
     const double* hi = lo + 3;
     int b0[3], b1[3];
@@ -250,7 +266,7 @@ __global__ void __launch_bounds__(256) k_poly_visit(int ns, PolyDev P, const dou
         if (t < nb) {
             const int bi = b0[0] + (int)(t % ex), bj = b0[1] + (int)((t / ex) % ey), bk = b0[2] + (int)(t / ((long long)ex * ey));
             const long long b = bi + (long long)P.bn[0] * (bj + (long long)P.bn[1] * bk);
-            if (poly_box_hit(P.b_box + 6 * b, lo, hi)) {
+            if (poly_bin_hit(K, P.b_box + 6 * b, lo, hi)) {
                 q0 = P.b_off[b];
                 n = P.b_off[b + 1] - q0;
             }
