@@ -2673,6 +2673,10 @@ int poly_setup(sdfibm_ctx* c, const sdfibm_polymesh_desc* d) {
             c->box_lo[e] = std::min(c->box_lo[e], d->points[3 * q + e]);
             c->box_hi[e] = std::max(c->box_hi[e], d->points[3 * q + e]);
         }
+    for (int e = 0; e < 3; ++e) {
+        P.mlo[e] = c->box_lo[e];
+        P.mhi[e] = c->box_hi[e];
+    }
     // uploads
     int rc;
     auto up = [&](const void* src, size_t bytes, void** dst) -> int {

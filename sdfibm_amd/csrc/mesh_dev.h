@@ -139,9 +139,14 @@ __device__ inline void hex_faces(const MeshDev& m, int i, int j, int k, d3 v0, H
 }
 
 // Signed distance at world point p for the fraction: local = R^T (p - c) with
-// the solid's rotation matrix (the inside tests keep the quaternion form).
+// the solid's rotation matrix (the inside tests keep the quaternion form).  A
+// plane keeps the quaternion form: its distance is the local y itself, and a
+// wall lying in a mesh face gives its points distances of a few 1e-17 (the
+// rotation's rounding) whose ratios the fraction's edge crossings take, so the
+// bits of those distances are the reference's only in its own transform.
 template <class R>
 __device__ inline R sdf_fast(const SolidK& K, d3 p) {
+    if (K.shape == SDFIBM_SHAPE_PLANE) return R(world_to_local(K, p).y);
     const double dx = p.x - K.c[0], dy = p.y - K.c[1], dz = p.z - K.c[2];
     const d3 l{K.rt[0] * dx + K.rt[1] * dy + K.rt[2] * dz, K.rt[3] * dx + K.rt[4] * dy + K.rt[5] * dz,
                K.rt[6] * dx + K.rt[7] * dy + K.rt[8] * dz};

@@ -52,6 +52,7 @@ struct PolyDev {
     const int* c_nbs;
     const double* cc;    // cell centres [n_cells*3]
     const double* vol;   // cell volumes
+    double mlo[3], mhi[3];  // the box of the mesh points
     // bins of cells by centre: a uniform grid over the cell centres' box
     double blo[3], binv[3], bw[3];
     int bn[3];
@@ -82,6 +83,12 @@ __device__ inline int poly_lookup(const unsigned long long* hk, const int* hv, i
 
 // The bins a world box [lo, hi] overlaps (index box, clamped), false if none.
 __device__ inline bool poly_bin_box(const PolyDev& P, const double* lo, const double* hi, int* b0, int* b1) {
+    for (int d = 0; d < 3; ++d)
+        if (lo[d] > hi[d]) {  // an empty box: no bins
+            b0[0] = b0[1] = b0[2] = 0;
+            b1[0] = b1[1] = b1[2] = -1;
+            return false;
+        }
     for (int d = 0; d < 3; ++d) {
         double a = (lo[d] - P.blo[d]) * P.binv[d], b = (hi[d] - P.blo[d]) * P.binv[d];
         if (!(a > -1e9)) a = -1e9;  // unbounded axes
@@ -102,17 +109,69 @@ __device__ inline bool poly_box_hit(const double* bb, const double* lo, const do
 }
 
 // A bin's box can hold a point of the solid: it meets the solid's world box and, for a plane
-// (inside: local y < 0), its lowest local y is below 0.
+// (inside: local y < 0), its lowest local y is below 0.  The plane test keeps a margin of
+// 1e-12 of the terms' size: a point on a wall lying in a mesh face has a local y of a few
+// 1e-17 (the rotation's rounding), which decides it and which this sum's rounding can lose.
 __device__ inline bool poly_bin_hit(const SolidK& K, const double* bb, const double* lo, const double* hi) {
     if (!poly_box_hit(bb, lo, hi)) return false;
     if (K.shape != SDFIBM_SHAPE_PLANE) return true;
-    double y = 0.0, r = 0.0;
+    double y = 0.0, r = 0.0, m = 0.0;
     for (int d = 0; d < 3; ++d) {
         const double n = K.rt[3 + d], c = 0.5 * (bb[d] + bb[3 + d]) - K.c[d], h = 0.5 * (bb[3 + d] - bb[d]);
         y += n * c;
         r += fabs(n) * h;
+        m += fabs(n) * (fabs(bb[d]) + fabs(bb[3 + d]) + fabs(K.c[d]));
     }
-    return y - r < 0.0;
+    return y - r < 1e-12 * m + 1e-300;
+}
+
+// The world box of the mesh points on the inside of a plane (local y < 0): the
+// box of the mesh box's corners on the inside of a relaxed plane, f(p) < delta, and
+// of its edges' crossings of f = delta (that part of the mesh box is the convex hull
+// of those points).  delta (1e-12 of the terms' size) bounds the rounding of the
+// inside test's local y, which decides points lying in the plane (a wall on a mesh
+// face): the relaxed half-space holds every point that test can call inside.
+__device__ inline void poly_plane_box(const PolyDev& P, const SolidK& K, double* lo, double* hi) {
+    const double n[3] = {K.rt[3], K.rt[4], K.rt[5]};
+    double delta = 1e-300;
+    for (int d = 0; d < 3; ++d)
+        delta += 1e-12 * fabs(n[d]) * (fmax(fabs(P.mlo[d]), fabs(P.mhi[d])) + fabs(K.c[d]));
+    auto g = [&](const double* p) {
+        return n[0] * (p[0] - K.c[0]) + n[1] * (p[1] - K.c[1]) + n[2] * (p[2] - K.c[2]) - delta;
+    };
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = __builtin_huge_val();
+        hi[d] = -__builtin_huge_val();
+    }
+    auto add = [&](const double* p) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], p[d]);
+            hi[d] = fmax(hi[d], p[d]);
+        }
+    };
+    for (int v = 0; v < 8; ++v) {
+        double a[3];
+        for (int d = 0; d < 3; ++d) a[d] = (v >> d) & 1 ? P.mhi[d] : P.mlo[d];
+        const double ga = g(a);
+        if (ga < 0) add(a);
+        for (int d = 0; d < 3; ++d) {  // the edge to the corner across axis d (each edge once)
+            if ((v >> d) & 1) continue;
+            double b[3] = {a[0], a[1], a[2]};
+            b[d] = P.mhi[d];
+            const double gb = g(b);
+            if ((ga < 0) != (gb < 0)) {
+                double x[3] = {a[0], a[1], a[2]};
+                x[d] = fmin(fmax(a[d] + ga / (ga - gb) * (b[d] - a[d]), a[d]), b[d]);
+                add(x);
+            }
+        }
+    }
+    // the crossings' own rounding: widen by a hair of the mesh size
+    for (int d = 0; d < 3; ++d) {
+        const double e = 1e-9 * (P.mhi[d] - P.mlo[d]) + 1e-12 * fmax(fabs(P.mlo[d]), fabs(P.mhi[d])) + 1e-300;
+        lo[d] -= e;
+        hi[d] += e;
+    }
 }
 
 // Per solid (one wave; 4 per workgroup): constants, world box (a cell can hold an
@@ -136,6 +195,7 @@ __global__ void __launch_bounds__(256) k_poly_setup(const sdfibm_solid* __restri
         lo[d] = bounded && wh[d] < 1e300 ? wc[d] - wh[d] : -__builtin_huge_val();
         hi[d] = bounded && wh[d] < 1e300 ? wc[d] + wh[d] : __builtin_huge_val();
     }
+    if (K.shape == SDFIBM_SHAPE_PLANE) poly_plane_box(P, K, lo, hi);
     if (lane == 0) {
         Ks[s] = K;
         for (int d = 0; d < 3; ++d) {
@@ -147,13 +207,13 @@ __global__ void __launch_bounds__(256) k_poly_setup(const sdfibm_solid* __restri
     }
     // cells in the bins whose boxes meet the solid's box
     int b0[3], b1[3];
-    poly_bin_box(P, lo, hi, b0, b1);
+    const bool any = poly_bin_box(P, lo, hi, b0, b1);
     for (int d = 0; d < 3; ++d) {
         b0[d] = max(0, b0[d] - reach);
         b1[d] = min(P.bn[d] - 1, b1[d] + reach);
     }
     const int ex = b1[0] - b0[0] + 1, ey = b1[1] - b0[1] + 1;
-    const long long nb = (long long)ex * ey * (b1[2] - b0[2] + 1);
+    const long long nb = any ? (long long)ex * ey * (b1[2] - b0[2] + 1) : 0;
     long long cnt = 0;
     for (long long t = lane; t < nb; t += 64) {
         const int bi = b0[0] + (int)(t % ex), bj = b0[1] + (int)((t / ex) % ey), bk = b0[2] + (int)(t / ((long long)ex * ey));
@@ -250,13 +310,13 @@ __global__ void __launch_bounds__(256) k_poly_visit(int ns, PolyDev P, const Sol
     const double* lo = box + 6 * s;
     const double* hi = lo + 3;
     int b0[3], b1[3];
-    poly_bin_box(P, lo, hi, b0, b1);
+    const bool any = poly_bin_box(P, lo, hi, b0, b1);
     for (int d = 0; d < 3; ++d) {
         b0[d] = max(0, b0[d] - reach);
         b1[d] = min(P.bn[d] - 1, b1[d] + reach);
     }
     const int ex = b1[0] - b0[0] + 1, ey = b1[1] - b0[1] + 1;
-    const long long nb = (long long)ex * ey * (b1[2] - b0[2] + 1);
+    const long long nb = any ? (long long)ex * ey * (b1[2] - b0[2] + 1) : 0;
     long long o = voff[s];
     // 64 bins at a time, one per lane: their cell counts scanned over the wave, then each lane
     // copies its bin's cells (bin order and ascending cells within a bin: a fixed order)

@@ -7,7 +7,8 @@
   polyMesh path reproduces alpha.water to 1e-11;
 * a 3-D O-grid (five blocks with arc edges, graded, 8 layers) with spheres, ellipsoids, boxes, a
   CSG solid and a plane, both precisions, against the oracle on the same polyMesh;
-* one jittered block through both paths (structured and polyMesh): the same candidates and fields;
+* one jittered block through both paths (structured and polyMesh): the same candidates and fields,
+  with and without plane walls (the polyMesh path against the oracle too);
 * three coupled steps (interact, U -= Fs dt, evolve, fixInternal) against the oracle;
 * the calls a polyMesh context refuses.
 Tolerances of test_gpu_parity.py."""
@@ -148,17 +149,29 @@ def test_ogrid_mixed_solids(prec):
         _compare(c, ref2, sol.size, TOL[prec], ft2)
 
 
+@pytest.mark.parametrize("walls", [False, True], ids=["bodies", "walls"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_jittered_block_both_paths(prec):
+def test_jittered_block_both_paths(prec, walls):
     sc = scenes.small_mixed_3d(20, seed=8, jitter=0.2)
+    sol = sc.solids
+    if walls:  # six walls of the cube (floor and ceiling inset) and one tilted plane
+        tilt = S.make_solids(1)
+        tilt["shape"] = S.PLANE
+        tilt["center"] = (0.45, 0.9, 0.55)
+        tilt["q"] = S.quaternion_xyz((0.3, 0.2, 2.8))
+        sol = np.concatenate([sol, scenes.walls_3d(0.13, 0.0), tilt])
     pm = sc.mesh.to_polymesh()
     with SolidCloud(sc.mesh, rho_f=sc.rho_f, precision=prec) as a, \
             SolidCloud(pm, rho_f=sc.rho_f, precision=prec) as b:
         for c in (a, b):
-            c.set_solids(sc.solids)
+            c.set_solids(sol)
             c.set_velocity(sc.U)
         fa, fb = a.interact(0.0, sc.dt), b.interact(0.0, sc.dt)
-        for s in range(sc.solids.size):
+        if walls and prec == "fp64":
+            ref = oracle.OracleMesh(pm).interact(sol, sc.U, two_d=False, rho_f=sc.rho_f, dt=sc.dt, capture=True,
+                                                 capacity=8 * pm.n_cells)
+            _compare(b, ref, sol.size, TOL[prec], ft=fb)
+        for s in range(sol.size):
             ca, cb = a.candidates(s), b.candidates(s)
             assert np.array_equal(ca[0], cb[0]) and np.array_equal(ca[1], cb[1]), s
             assert np.max(np.abs(ca[2] - cb[2]), initial=0.0) <= TOL[prec]["a"], s
