@@ -280,9 +280,11 @@ def fp32_error_statement(scene, om, n_sample=16):
 
 def surrogate_comparison(cloud, sc, ft0, As0, plain_ms, plain_cells_ms, steps):
     """The same step with the learned fraction (MLP 11-32-32-1, BF16 MFMA) serving the cut cells
-    of spheres and ellipsoids, at the prescribed fallback rule (prediction in [0.02, 0.98] and a
-    gradient angle above 30 degrees -> exact fraction).  Reports the step-time ratio and the
-    deviation from the plain method's As and force/torque."""
+    of spheres and ellipsoids, at three fallback rules: the prescribed one (prediction in
+    [0.02, 0.98] and a gradient angle above 30 degrees -> exact fraction), every in-band prediction
+    falling back, and every cell falling back (the network still evaluated).  Reports each rule's
+    step-time ratio and deviation from the plain method's As and force/torque, and the speedup of
+    the fastest rule within max|dAs| <= 1e-3."""
     import torch
 
     from sdfibm_amd import surrogate as SG
@@ -290,30 +292,39 @@ def surrogate_comparison(cloud, sc, ft0, As0, plain_ms, plain_cells_ms, steps):
     if not os.path.exists(SG.PARAMS_PATH):
         return {"skipped": "no trained parameter file"}
     p = SG.load_params()
-    band, angle = (0.02, 0.98), 30.0
-    cloud.set_surrogate(p, band=band, max_angle_deg=angle)
-    for _ in range(3):
-        cloud.interact_device(0.0, sc.dt)
-    torch.cuda.synchronize()
-    k = max(5, min(steps, 20))
-    ph = np.zeros(7)
-    t0 = time.perf_counter()
-    for _ in range(k):
-        cloud.interact_device(0.0, sc.dt)
-        ph += cloud.timing()
-    torch.cuda.synchronize()
-    sg_ms = 1e3 * (time.perf_counter() - t0) / k
-    ft1 = cloud.interact(0.0, sc.dt)
-    As1 = cloud.fields()["As"]
-    served = cloud.surrogate_count()
-    cloud.set_surrogate(None)
+    rules = {"prescribed": ((0.02, 0.98), 30.0), "in_band_exact": ((0.02, 0.98), -1.0),
+             "all_exact": ((-1e30, 1e30), -1.0)}
     scale = np.maximum(np.abs(ft0), 1e-3 * np.abs(ft0).max())
-    dAs = float(np.abs(As1.astype(np.float64) - As0).max())
-    out = {"band": list(band), "max_angle_deg": angle, "served_cut_cells": int(served),
-           "plain_ms_per_step": plain_ms, "surrogate_ms_per_step": sg_ms, "speedup": plain_ms / sg_ms,
-           "k_cells_ms": {"plain": plain_cells_ms, "surrogate": ph[5] / k},
-           "max_abs_dAs": dAs, "force_torque_max_rel_err": float(np.max(np.abs(ft1 - ft0) / scale)),
-           "meets_max_dAs_1e-3": dAs <= 1e-3}
+    res = {}
+    for name, (band, angle) in rules.items():
+        cloud.set_surrogate(p, band=band, max_angle_deg=angle)
+        for _ in range(3):
+            cloud.interact_device(0.0, sc.dt)
+        torch.cuda.synchronize()
+        k = max(5, min(steps, 20))
+        ph = np.zeros(7)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            cloud.interact_device(0.0, sc.dt)
+            ph += cloud.timing()
+        torch.cuda.synchronize()
+        sg_ms = 1e3 * (time.perf_counter() - t0) / k
+        ft1 = cloud.interact(0.0, sc.dt)
+        As1 = cloud.fields()["As"]
+        served = cloud.surrogate_count()
+        cloud.set_surrogate(None)
+        dAs = float(np.abs(As1.astype(np.float64) - As0).max())
+        res[name] = {"band": list(band) if band[0] > -1e29 else "all", "max_angle_deg": angle,
+                     "served_cut_cells": int(served), "surrogate_ms_per_step": sg_ms, "speedup": plain_ms / sg_ms,
+                     "k_cells_ms": ph[5] / k, "max_abs_dAs": dAs,
+                     "force_torque_max_rel_err": float(np.max(np.abs(ft1 - ft0) / scale)),
+                     "meets_max_dAs_1e-3": dAs <= 1e-3}
+    ok = [r for r in res.values() if r["meets_max_dAs_1e-3"]]
+    best = max(ok, key=lambda r: r["speedup"]) if ok else None
+    out = {"plain_ms_per_step": plain_ms, "plain_k_cells_ms": plain_cells_ms, "rules": res,
+           "speedup_at_max_dAs_1e-3": best["speedup"] if best else None,
+           "served_cut_cells_at_max_dAs_1e-3": best["served_cut_cells"] if best else None,
+           "amdahl_bound_no_k_cells": plain_ms / max(plain_ms - plain_cells_ms, 1e-9)}
     log(f"[bench] surrogate: {out}")
     return out
 

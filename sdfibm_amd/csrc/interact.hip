@@ -535,7 +535,17 @@ __global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK
 
     // 1. inside test at every tile vertex (FP64, reference operation order).  A
     //    wave covers g = 64 / VX vertex rows per pass; one ballot gives their bits.
-    {
+    //    Quadric solids on a staged rectilinear tile: a row per thread, the
+    //    interval between the roots of the row's quadratic (row_inside_bits).
+    if (staged && row_quadric(K.shape)) {
+        d3 col[3];
+        local_columns(K, col);
+        for (int r = tid; r < VR; r += NT) {
+            int vj, vk;
+            divmod_small(r, VY, rVY, vk, vj);
+            vin[r] = row_inside_bits(K, col, sx, VX, sy[vj], sz[vk]);
+        }
+    } else {
         const int g = 64 / VX;
         const int rr = lane / VX, vi = lane - rr * VX;
         const unsigned long long vmask = VX >= 64 ? ~0ull : ((1ull << VX) - 1ull);
@@ -872,7 +882,8 @@ __global__ void __launch_bounds__(NT) k_cells(MeshDev m, int ns, const SolidK* _
             if (ok && pred >= sg_lo && pred <= sg_hi && corner_alignment(phi8, g, ihx, ihy, ihz) < sg_cos) ok = false;
             if (cut) {
                 const double z0 = m.zs[gk], z1 = m.zs[gk + 1];
-                const float a = ok ? pred
+                // a prediction is clamped to [0, 1]: the code's sign carries the centre test
+                const float a = ok ? fminf(fmaxf(pred, 0.f), 1.f)
                                    : box_fraction<float>(h, gi == 0, gj == 0, gk == 0, m.two_d != 0,
                                                          (float)((0.0 - z0) / (z1 - z0)));
                 n_net += ok ? 1 : 0;

@@ -173,16 +173,20 @@ __host__ __device__ inline SolidK make_solid_k(const sdfibm_solid& s, const CsgK
     return K;
 }
 
-// world2local = conjugate(q).transform(p - t), with OpenFOAM's quaternion
-// product order: m = a.mulq0v(u), result = (m * conjugate(a)).v, a = conj(q).
-__host__ __device__ inline d3 world_to_local(const SolidK& K, d3 p) {
-    const d3 u = p - d3{K.c[0], K.c[1], K.c[2]};
+// The rotation part of world_to_local: conj(q) u q in the same operation order.
+__host__ __device__ inline d3 rotate_to_local(const SolidK& K, d3 u) {
     const double aw = K.q[0];
     const d3 av{-K.q[1], -K.q[2], -K.q[3]};
     const double mw = -dot(av, u);
     const d3 mv = (aw * u) + cross(av, u);
     const d3 cav{K.q[1], K.q[2], K.q[3]};
     return (mw * cav + aw * mv) + cross(mv, cav);
+}
+
+// world2local = conjugate(q).transform(p - t), with OpenFOAM's quaternion
+// product order: m = a.mulq0v(u), result = (m * conjugate(a)).v, a = conj(q).
+__host__ __device__ inline d3 world_to_local(const SolidK& K, d3 p) {
+    return rotate_to_local(K, p - d3{K.c[0], K.c[1], K.c[2]});
 }
 
 __host__ __device__ inline bool rect_bool(double x, double y, double ra, double rb) {
@@ -263,6 +267,106 @@ __host__ __device__ inline bool inside_local(const SolidK& K, d3 p) {
 }
 
 __host__ __device__ inline bool inside_world(const SolidK& K, d3 p) { return inside_local(K, world_to_local(K, p)); }
+
+// ---- inside bits of a row of vertices (quadric solids) ----------------------
+// Along a row of vertices x_0 < x_1 < ... at fixed (y, z), the local point is
+// affine in t = x - c.x (rotate_to_local is linear), so the strict predicate of
+// a sphere, circle, ellipsoid or ellipse reads f(t) = A t^2 + 2 B t + C < 0,
+// with A > 0: its inside set is the interval between the roots t1 < t2.  A
+// vertex farther than a margin from both roots takes the interval's answer;
+// one within it, and every vertex of a row near tangency (|B^2 - AC| below
+// 1e-6 of B^2 + |AC|), takes the exact test inside_world.
+//
+// The margin (1e-6 of the row's largest |t1|, |t2|, |y - c.y|, |z - c.z|)
+// keeps |f| at a decided vertex above 1e-9 A T^2, where the rounding of the
+// exact test's operation order moves f by about 1e-14 w T^2 (w the largest
+// weight below): the decided bits are the exact test's bits for solids whose
+// squared axis ratio stays below 1e4.  The roots' own rounding (some 1e-13 T
+// after the 1e-3 relative separation the tangency test leaves) is far inside
+// the margin.
+
+// Columns of rotate_to_local's linear map (applied to the unit vectors).
+__host__ __device__ inline void local_columns(const SolidK& K, d3* col) {
+    col[0] = rotate_to_local(K, d3{1.0, 0.0, 0.0});
+    col[1] = rotate_to_local(K, d3{0.0, 1.0, 0.0});
+    col[2] = rotate_to_local(K, d3{0.0, 0.0, 1.0});
+}
+
+__host__ __device__ inline bool row_quadric(int shape) {
+    return shape == SDFIBM_SHAPE_SPHERE || shape == SDFIBM_SHAPE_ELLIPSOID || shape == SDFIBM_SHAPE_CIRCLE ||
+           shape == SDFIBM_SHAPE_ELLIPSE;
+}
+
+// Inside bits (bit v for vertex v) of the row of n <= 64 vertices at x = xs[0..n), y, z.
+__host__ __device__ inline unsigned long long row_inside_bits(const SolidK& K, const d3* col, const double* xs,
+                                                              int n, double y, double z) {
+    const double uy = y - K.c[1], uz = z - K.c[2];
+    // local point l(t) = m t + b
+    const double m[3] = {col[0].x, col[0].y, col[0].z};
+    const double b[3] = {col[1].x * uy + col[2].x * uz, col[1].y * uy + col[2].y * uz, col[1].z * uy + col[2].z * uz};
+    // f(t) = sum_d w_d (a_d + m_d t)^2 + e, per shape (inside_local's predicates)
+    double w[3] = {1.0, 1.0, 1.0}, a[3] = {b[0], b[1], b[2]}, e = 0.0;
+    switch (K.shape) {
+    case SDFIBM_SHAPE_SPHERE:
+        for (int d = 0; d < 3; ++d) a[d] = K.com[d] + b[d];
+        e = -K.k[1];
+        break;
+    case SDFIBM_SHAPE_CIRCLE:  // q = com + (l.x, l.y, 0)
+        a[0] = K.com[0] + b[0];
+        a[1] = K.com[1] + b[1];
+        w[2] = 0.0;
+        e = K.com[2] * K.com[2] - K.k[1];
+        break;
+    case SDFIBM_SHAPE_ELLIPSOID:
+        w[0] = K.k[0];
+        w[1] = K.k[1];
+        w[2] = K.k[2];
+        e = -1.0;
+        break;
+    default:  // ellipse: q = com + l, the z term absent
+        a[0] = K.com[0] + b[0];
+        a[1] = K.com[1] + b[1];
+        w[0] = K.k[0];
+        w[1] = K.k[1];
+        w[2] = 0.0;
+        e = -1.0;
+        break;
+    }
+    double A = 0.0, B = 0.0, C = e;
+    for (int d = 0; d < 3; ++d) {
+        A += w[d] * m[d] * m[d];
+        B += w[d] * m[d] * a[d];
+        C += w[d] * a[d] * a[d];
+    }
+    const double disc = B * B - A * C, S = B * B + fabs(A * C);
+    unsigned long long bits = 0ull;
+    auto exact_row = [&]() {
+        unsigned long long r = 0ull;
+        for (int v = 0; v < n; ++v)
+            if (inside_world(K, d3{xs[v], y, z})) r |= 1ull << v;
+        return r;
+    };
+    if (!(A > 0.0) || !(fabs(disc) <= 1e300) || !(S <= 1e300)) return exact_row();
+    if (disc <= -1e-6 * S) return 0ull;  // the row misses the solid
+    if (disc < 1e-6 * S) return exact_row();
+    const double sq = sqrt(disc);
+    const double t1 = (-B - sq) / A, t2 = (-B + sq) / A;
+    const double T = fmax(fmax(fabs(t1), fabs(t2)), fmax(fabs(uy), fabs(uz)));
+    const double mg = 1e-6 * T;
+    const double in_lo = t1 + mg, in_hi = t2 - mg, out_lo = t1 - mg, out_hi = t2 + mg;
+    for (int v = 0; v < n; ++v) {
+        const double t = xs[v] - K.c[0];
+        bool in;
+        if (t > in_lo && t < in_hi)
+            in = true;
+        else if (t < out_lo || t > out_hi)
+            in = false;
+        else
+            in = inside_world(K, d3{xs[v], y, z});
+        bits |= (unsigned long long)in << v;
+    }
+    return bits;
+}
 
 // ---- signed distance in R ---------------------------------------------------
 template <class R>
