@@ -278,7 +278,7 @@ def fp32_error_statement(scene, om, n_sample=16):
             "force_torque_max_rel": float(np.max(np.abs(ft - ref["ft"]) / scale))}
 
 
-def surrogate_comparison(cloud, sc, ft0, As0, plain_ms, plain_cells_ms, steps):
+def surrogate_comparison(cloud, sc, ft0, As0, steps):
     """The same step with the learned fraction (MLP 11-32-32-1, BF16 MFMA) serving the cut cells
     of spheres and ellipsoids, at three fallback rules: the prescribed one (prediction in
     [0.02, 0.98] and a gradient angle above 30 degrees -> exact fraction), every in-band prediction
@@ -292,12 +292,16 @@ def surrogate_comparison(cloud, sc, ft0, As0, plain_ms, plain_cells_ms, steps):
     if not os.path.exists(SG.PARAMS_PATH):
         return {"skipped": "no trained parameter file"}
     p = SG.load_params()
-    rules = {"prescribed": ((0.02, 0.98), 30.0), "in_band_exact": ((0.02, 0.98), -1.0),
+    rules = {"plain": None, "prescribed": ((0.02, 0.98), 30.0), "in_band_exact": ((0.02, 0.98), -1.0),
              "all_exact": ((-1e30, 1e30), -1.0)}
     scale = np.maximum(np.abs(ft0), 1e-3 * np.abs(ft0).max())
     res = {}
-    for name, (band, angle) in rules.items():
-        cloud.set_surrogate(p, band=band, max_angle_deg=angle)
+    for name, rule in rules.items():
+        if rule is None:
+            cloud.set_surrogate(None)
+        else:
+            band, angle = rule
+            cloud.set_surrogate(p, band=band, max_angle_deg=angle)
         for _ in range(3):
             cloud.interact_device(0.0, sc.dt)
         torch.cuda.synchronize()
@@ -311,9 +315,12 @@ def surrogate_comparison(cloud, sc, ft0, As0, plain_ms, plain_cells_ms, steps):
         sg_ms = 1e3 * (time.perf_counter() - t0) / k
         ft1 = cloud.interact(0.0, sc.dt)
         As1 = cloud.fields()["As"]
-        served = cloud.surrogate_count()
+        served = cloud.surrogate_count() if rule is not None else 0
         cloud.set_surrogate(None)
         dAs = float(np.abs(As1.astype(np.float64) - As0).max())
+        if rule is None:
+            plain_ms, plain_cells_ms = sg_ms, ph[5] / k
+            continue
         res[name] = {"band": list(band) if band[0] > -1e29 else "all", "max_angle_deg": angle,
                      "served_cut_cells": int(served), "surrogate_ms_per_step": sg_ms, "speedup": plain_ms / sg_ms,
                      "k_cells_ms": ph[5] / k, "max_abs_dAs": dAs,
@@ -865,7 +872,7 @@ def main():
     per_kernel["k_cells"]["limit"] = "VALU issue (about two thirds of the SIMD issue rate), not HBM"
     line["roofline"]["per_kernel"] = per_kernel
     if args.precision == "fp32" and not args.no_surrogate:
-        line["surrogate"] = surrogate_comparison(cloud, sc, ft, f["As"], ms_per_step, phases[5], args.steps)
+        line["surrogate"] = surrogate_comparison(cloud, sc, ft, f["As"], args.steps)
     if args.precision == "fp32" and not args.no_fp64:
         # the FP64 point of the same scene (FP64 fractions and fields)
         with SolidCloud(sc.mesh, two_d=sc.two_d, rho_f=sc.rho_f, precision="fp64") as c64:
