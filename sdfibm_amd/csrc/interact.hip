@@ -486,8 +486,10 @@ constexpr int kStageYZ = 256;  // tile coordinates staged in LDS up to this many
 // dispatcher balances the uneven tiles).  MAXR bounds the vertex rows
 // (ny+1)(nz+1) of the tiles of the queue.  No shared work counter: that would
 // cost a device-scope atomic (slow across the XCDs) per solid.
+// Register budget: 64 VGPRs in FP32 (eight waves per SIMD; the kernel waits on dependent
+// loads more than it computes), 80 in FP64 (no spills).
 template <class R, int MK, int MAXR, int Q, int NT>
-__global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK* __restrict__ Ks,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 6))) k_classify(MeshDev m, int ns, const SolidK* __restrict__ Ks,
                                                      const TileInfo* __restrict__ Ts, int* __restrict__ qmeta,
                                                      const int* __restrict__ qlist, R* __restrict__ codes,
                                                      int* __restrict__ cand, long long code_cap,
@@ -538,12 +540,10 @@ __global__ void __launch_bounds__(NT) k_classify(MeshDev m, int ns, const SolidK
     //    Quadric solids on a staged rectilinear tile: a row per thread, the
     //    interval between the roots of the row's quadratic (row_inside_bits).
     if (staged && row_quadric(K.shape)) {
-        d3 col[3];
-        local_columns(K, col);
         for (int r = tid; r < VR; r += NT) {
             int vj, vk;
             divmod_small(r, VY, rVY, vk, vj);
-            vin[r] = row_inside_bits(K, col, sx, VX, sy[vj], sz[vk]);
+            vin[r] = row_inside_bits(K, sx, VX, sy[vj], sz[vk]);
         }
     } else {
         const int g = 64 / VX;

@@ -94,7 +94,10 @@ struct SolidK {
     double v[3];    // velocity
     double w[3];    // angular velocity
     double rt[9];   // R(q)^T row-major: local = rt (p - c), for the signed distance
+    double qc[9];   // columns of rotate_to_local's linear map (the inside test's rotation), column-major
 };
+
+__host__ __device__ inline d3 rotate_to_local(const SolidK& K, d3 u);
 
 // csg_table: the CSG shape constants (device or host copy, whichever side
 // evaluates K); a CSG solid's prm[0] is its index there.
@@ -169,6 +172,12 @@ __host__ __device__ inline SolidK make_solid_k(const sdfibm_solid& s, const CsgK
         K.rt[6] = 2 * (x * z + w * y);
         K.rt[7] = 2 * (y * z - w * x);
         K.rt[8] = 1 - 2 * (x * x + y * y);
+    }
+    for (int a = 0; a < 3; ++a) {
+        const d3 col = rotate_to_local(K, d3{a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0});
+        K.qc[3 * a + 0] = col.x;
+        K.qc[3 * a + 1] = col.y;
+        K.qc[3 * a + 2] = col.z;
     }
     return K;
 }
@@ -285,85 +294,71 @@ __host__ __device__ inline bool inside_world(const SolidK& K, d3 p) { return ins
 // after the 1e-3 relative separation the tangency test leaves) is far inside
 // the margin.
 
-// Columns of rotate_to_local's linear map (applied to the unit vectors).
-__host__ __device__ inline void local_columns(const SolidK& K, d3* col) {
-    col[0] = rotate_to_local(K, d3{1.0, 0.0, 0.0});
-    col[1] = rotate_to_local(K, d3{0.0, 1.0, 0.0});
-    col[2] = rotate_to_local(K, d3{0.0, 0.0, 1.0});
-}
-
 __host__ __device__ inline bool row_quadric(int shape) {
     return shape == SDFIBM_SHAPE_SPHERE || shape == SDFIBM_SHAPE_ELLIPSOID || shape == SDFIBM_SHAPE_CIRCLE ||
            shape == SDFIBM_SHAPE_ELLIPSE;
 }
 
 // Inside bits (bit v for vertex v) of the row of n <= 64 vertices at x = xs[0..n), y, z.
-__host__ __device__ inline unsigned long long row_inside_bits(const SolidK& K, const d3* col, const double* xs,
-                                                              int n, double y, double z) {
-    const double uy = y - K.c[1], uz = z - K.c[2];
-    // local point l(t) = m t + b
-    const double m[3] = {col[0].x, col[0].y, col[0].z};
-    const double b[3] = {col[1].x * uy + col[2].x * uz, col[1].y * uy + col[2].y * uz, col[1].z * uy + col[2].z * uz};
-    // f(t) = sum_d w_d (a_d + m_d t)^2 + e, per shape (inside_local's predicates)
-    double w[3] = {1.0, 1.0, 1.0}, a[3] = {b[0], b[1], b[2]}, e = 0.0;
+// The predicate reads f = sum_d w_d (off_d + l_d)^2 + e < 0 with the local point
+// l = m t + c1 (y - c.y) + c2 (z - c.z), (m, c1, c2) = K.qc.  The exact tests of
+// the undecided vertices run after the interval pass.
+__host__ __device__ inline unsigned long long row_inside_bits(const SolidK& K, const double* xs, int n, double y,
+                                                              double z) {
+    double w[3] = {1.0, 1.0, 1.0}, off[3] = {K.com[0], K.com[1], K.com[2]}, e;
     switch (K.shape) {
-    case SDFIBM_SHAPE_SPHERE:
-        for (int d = 0; d < 3; ++d) a[d] = K.com[d] + b[d];
-        e = -K.k[1];
-        break;
+    case SDFIBM_SHAPE_SPHERE: e = -K.k[1]; break;
     case SDFIBM_SHAPE_CIRCLE:  // q = com + (l.x, l.y, 0)
-        a[0] = K.com[0] + b[0];
-        a[1] = K.com[1] + b[1];
         w[2] = 0.0;
         e = K.com[2] * K.com[2] - K.k[1];
         break;
     case SDFIBM_SHAPE_ELLIPSOID:
-        w[0] = K.k[0];
-        w[1] = K.k[1];
-        w[2] = K.k[2];
+        for (int d = 0; d < 3; ++d) {
+            w[d] = K.k[d];
+            off[d] = 0.0;
+        }
         e = -1.0;
         break;
     default:  // ellipse: q = com + l, the z term absent
-        a[0] = K.com[0] + b[0];
-        a[1] = K.com[1] + b[1];
         w[0] = K.k[0];
         w[1] = K.k[1];
         w[2] = 0.0;
         e = -1.0;
         break;
     }
+    const double uy = y - K.c[1], uz = z - K.c[2];
     double A = 0.0, B = 0.0, C = e;
     for (int d = 0; d < 3; ++d) {
-        A += w[d] * m[d] * m[d];
-        B += w[d] * m[d] * a[d];
-        C += w[d] * a[d] * a[d];
+        const double md = K.qc[d], a = off[d] + (K.qc[3 + d] * uy + K.qc[6 + d] * uz);
+        A += w[d] * md * md;
+        B += w[d] * md * a;
+        C += w[d] * a * a;
     }
     const double disc = B * B - A * C, S = B * B + fabs(A * C);
-    unsigned long long bits = 0ull;
-    auto exact_row = [&]() {
-        unsigned long long r = 0ull;
-        for (int v = 0; v < n; ++v)
-            if (inside_world(K, d3{xs[v], y, z})) r |= 1ull << v;
-        return r;
-    };
-    if (!(A > 0.0) || !(fabs(disc) <= 1e300) || !(S <= 1e300)) return exact_row();
-    if (disc <= -1e-6 * S) return 0ull;  // the row misses the solid
-    if (disc < 1e-6 * S) return exact_row();
-    const double sq = sqrt(disc);
-    const double t1 = (-B - sq) / A, t2 = (-B + sq) / A;
-    const double T = fmax(fmax(fabs(t1), fabs(t2)), fmax(fabs(uy), fabs(uz)));
-    const double mg = 1e-6 * T;
-    const double in_lo = t1 + mg, in_hi = t2 - mg, out_lo = t1 - mg, out_hi = t2 + mg;
-    for (int v = 0; v < n; ++v) {
-        const double t = xs[v] - K.c[0];
-        bool in;
-        if (t > in_lo && t < in_hi)
-            in = true;
-        else if (t < out_lo || t > out_hi)
-            in = false;
-        else
-            in = inside_world(K, d3{xs[v], y, z});
-        bits |= (unsigned long long)in << v;
+    const unsigned long long all = n >= 64 ? ~0ull : (1ull << n) - 1ull;
+    unsigned long long bits = 0ull, undecided = all;
+    if (A > 0.0 && fabs(disc) <= 1e300 && S <= 1e300) {
+        if (disc <= -1e-6 * S) return 0ull;  // the row misses the solid
+        if (disc >= 1e-6 * S) {
+            const double sq = sqrt(disc);
+            const double t1 = (-B - sq) / A, t2 = (-B + sq) / A;
+            const double T = fmax(fmax(fabs(t1), fabs(t2)), fmax(fabs(uy), fabs(uz)));
+            const double mg = 1e-6 * T;
+            const double in_lo = t1 + mg, in_hi = t2 - mg, out_lo = t1 - mg, out_hi = t2 + mg;
+            undecided = 0ull;
+            for (int v = 0; v < n; ++v) {
+                const double t = xs[v] - K.c[0];
+                const bool in = t > in_lo && t < in_hi;
+                const bool out = t < out_lo || t > out_hi;
+                bits |= (unsigned long long)in << v;
+                undecided |= (unsigned long long)(!in && !out) << v;
+            }
+        }
+    }
+    while (undecided) {
+        const int v = __builtin_ctzll(undecided);
+        undecided &= undecided - 1ull;
+        if (inside_world(K, d3{xs[v], y, z})) bits |= 1ull << v;
     }
     return bits;
 }

@@ -33,8 +33,6 @@ int main(int argc, char** argv) {
         const int axis_aligned = (r % 7 == 0);
         for (int d = 0; d < 4; ++d) s.q[d] = axis_aligned ? (d == 0 ? 1.0 : 0.0) : q[d] / std::sqrt(n2);
         const SolidK K = make_solid_k(s);
-        d3 col[3];
-        local_columns(K, col);
         // the row: a uniform x grid of n vertices around the solid, y/z near the solid
         const int n = 1 + (int)(63 * U(g));
         double xs[64];
@@ -53,7 +51,7 @@ int main(int argc, char** argv) {
             xs[v] = lo;
             for (int u = v + 1; u < n; ++u) xs[u] = xs[u - 1] + h;
         }
-        const unsigned long long got = row_inside_bits(K, col, xs, n, y, z);
+        const unsigned long long got = row_inside_bits(K, xs, n, y, z);
         unsigned long long want = 0;
         for (int v = 0; v < n; ++v) if (inside_world(K, d3{xs[v], y, z})) want |= 1ull << v;
         vert += n;
