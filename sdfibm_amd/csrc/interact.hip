@@ -1782,6 +1782,15 @@ struct sdfibm_ctx {
     long long* g_vcnt = nullptr;
     long long* g_voff = nullptr;
     long long* g_prange = nullptr;
+    long long* g_ccoff = nullptr;  // per solid: its first visit chunk (ns + 1)
+    long long* g_pnch = nullptr;   // per solid: its pair chunks
+    long long* g_pcoff = nullptr;  // per solid: its first pair chunk (ns + 1)
+    int* g_csol = nullptr;         // visit chunk -> solid
+    long long* g_ccnt_v = nullptr; // visit chunk -> its cells
+    long long* g_cvoff = nullptr;  // visit chunk -> its first visit
+    int* g_psol = nullptr;         // pair chunk -> solid
+    double* g_part = nullptr;      // pair chunk -> six partial sums and two counts
+    long long g_csol_cap = 0, g_ccnt_cap = 0, g_cvoff_cap = 0, g_psol_cap = 0, g_part_cap = 0;
     int* g_vcell = nullptr;
     int* g_vsolid = nullptr;
     unsigned char* g_act = nullptr;
@@ -2203,29 +2212,23 @@ int setup_mesh(sdfibm_ctx* c, const sdfibm_mesh_desc* d) {
             c->box_hi[a] = P[3 * pe + a];
         }
         double jit = 0.0;
-        std::vector<double> soa(NP);
-        double* dp[3];
-        for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < 3; ++a)
             for (long long k = 0; k <= nz; ++k)
                 for (long long j = 0; j <= ny; ++j)
                     for (long long i = 0; i <= nx; ++i) {
-                        const long long q = pidx(i, j, k);
-                        const double v = P[3 * q + a];
+                        const double v = P[3 * pidx(i, j, k) + a];
                         if (!std::isfinite(v)) return fail(c, SDFIBM_ERR_MESH, "non-finite point");
                         const long long idx = a == 0 ? i : (a == 1 ? j : k);
                         jit = std::max(jit, std::fabs((v - o[a]) / h[a] - (double)idx));
-                        soa[q] = v;
                     }
-            if ((rc = dalloc(c, (void**)&dp[a], NP * 8))) return rc;
-            HIPCHK(c, hipMemcpy(dp[a], soa.data(), NP * 8, hipMemcpyHostToDevice));
-        }
+        double* dp = nullptr;  // AoS, as given
+        if ((rc = dalloc(c, (void**)&dp, NP * 24))) return rc;
+        HIPCHK(c, hipMemcpy(dp, P, NP * 24, hipMemcpyHostToDevice));
         if (jit > 0.45)
             return fail(c, SDFIBM_ERR_MESH,
                         "structured mesh: points deviate more than 0.45 cells from a uniform grid (" +
                             std::to_string(jit) + ")");
-        m.px = dp[0];
-        m.py = dp[1];
-        m.pz = dp[2];
+        m.pxyz = dp;
         std::vector<double> cs(N);
         double* dc[3];
         for (int a = 0; a < 3; ++a) {
@@ -2697,15 +2700,9 @@ int poly_setup(sdfibm_ctx* c, const sdfibm_polymesh_desc* d) {
             return fail(c, SDFIBM_ERR_HIP, "mesh upload failed");
         return SDFIBM_OK;
     };
-    std::vector<double> soa(nP);
-    double* pxyz[3] = {nullptr, nullptr, nullptr};
-    for (int e = 0; e < 3; ++e) {
-        for (long long q = 0; q < nP; ++q) soa[q] = d->points[3 * q + e];
-        if ((rc = up(soa.data(), nP * 8, (void**)&pxyz[e]))) return rc;
-    }
-    P.px = pxyz[0];
-    P.py = pxyz[1];
-    P.pz = pxyz[2];
+    void* pts = nullptr;  // AoS: a point's three coordinates share a cache line (scattered gathers)
+    if ((rc = up(d->points, nP * 24, &pts))) return rc;
+    P.pxyz = (const double*)pts;
     std::vector<int> fo32(nF + 1), fp32(d->face_points, d->face_points + fo[nF]);
     for (long long f = 0; f <= nF; ++f) fo32[f] = (int)fo[f];
     void* p;
@@ -2778,7 +2775,8 @@ int poly_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
     if (n > c->solid_cap) {
         for (void** b : {(void**)&c->d_solids, (void**)&c->d_K, (void**)&c->d_ft, (void**)&c->d_ncand,
                          (void**)&c->g_box, (void**)&c->g_seed, (void**)&c->g_first, (void**)&c->g_vcnt,
-                         (void**)&c->g_voff, (void**)&c->g_prange}) {
+                         (void**)&c->g_voff, (void**)&c->g_prange, (void**)&c->g_ccoff, (void**)&c->g_pnch,
+                         (void**)&c->g_pcoff}) {
             void* q = *b;
             dfree(c, q);
             *b = nullptr;
@@ -2795,6 +2793,9 @@ int poly_set_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         if ((rc = dalloc(c, (void**)&c->g_vcnt, cap * 8))) return rc;
         if ((rc = dalloc(c, (void**)&c->g_voff, (cap + 1) * 8))) return rc;
         if ((rc = dalloc(c, (void**)&c->g_prange, cap * 16))) return rc;
+        if ((rc = dalloc(c, (void**)&c->g_ccoff, (cap + 1) * 8))) return rc;
+        if ((rc = dalloc(c, (void**)&c->g_pnch, cap * 8))) return rc;
+        if ((rc = dalloc(c, (void**)&c->g_pcoff, (cap + 1) * 8))) return rc;
         c->solid_cap = cap;
     }
     if (n > 0) {
@@ -2821,9 +2822,24 @@ int poly_step(sdfibm_ctx* c, double dt) {
         if ((rc = poly_grow(c, c->g_tmp, c->g_tmpcap, std::max<long long>(ns, nC) / kScanTile + 2, 8))) return rc;
         k_poly_setup<<<grid_for(ns, 4), 256, 0, st>>>(c->d_solids, ns, c->pm, c->d_K, c->g_box, c->g_seed, c->g_vcnt,
                                                       c->d_ft, c->g_first, c->d_csg, c->poly_reach);
-        scan<long long>(c->g_vcnt, ns, c->g_voff, c->g_tmp, st);
+        // the solids' bins in chunks (a wave each): counted, scanned, then listed
+        scan<long long>(c->g_vcnt, ns, c->g_ccoff, c->g_tmp, st);
+        long long nchunk = 0;
+        HIPCHK(c, hipMemcpyAsync(&nchunk, c->g_ccoff + ns, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        if ((rc = poly_grow(c, c->g_csol, c->g_csol_cap, nchunk + 1, 4))) return rc;
+        if ((rc = poly_grow(c, c->g_ccnt_v, c->g_ccnt_cap, nchunk + 1, 8))) return rc;
+        if ((rc = poly_grow(c, c->g_cvoff, c->g_cvoff_cap, nchunk + 1, 8))) return rc;
+        if ((rc = poly_grow(c, c->g_tmp, c->g_tmpcap, nchunk / kScanTile + 2, 8))) return rc;
+        k_poly_chunk_map<<<grid_for(ns, 4), 256, 0, st>>>(ns, c->g_ccoff, c->g_csol);
+        if (nchunk > 0)
+            k_poly_visit<true><<<grid_for(nchunk, 4), 256, 0, st>>>(nchunk, c->pm, c->d_K, c->g_box, c->g_csol,
+                                                                    c->g_ccoff, c->g_ccnt_v, nullptr, nullptr, nullptr,
+                                                                    c->poly_reach);
+        scan<long long>(c->g_ccnt_v, nchunk, c->g_cvoff, c->g_tmp, st);
+        k_poly_solid_voff<<<grid_for(ns + 1, 256), 256, 0, st>>>(ns, c->g_ccoff, c->g_cvoff, c->g_voff);
         long long nv = 0;
-        HIPCHK(c, hipMemcpyAsync(&nv, c->g_voff + ns, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipMemcpyAsync(&nv, c->g_cvoff + nchunk, 8, hipMemcpyDeviceToHost, st));
         HIPCHK(c, hipStreamSynchronize(st));
         if (nv > INT_MAX) return fail(c, SDFIBM_ERR_CAPACITY, "more than 2^31-1 (solid, cell) visits");
         if (nv + 1 > c->g_vcap || !c->g_vcell) {  // the visit arrays, one capacity
@@ -2843,8 +2859,10 @@ int poly_step(sdfibm_ctx* c, double dt) {
             c->g_vcap = want;
         }
         if ((rc = poly_grow(c, c->g_tmp, c->g_tmpcap, nv / kScanTile + 2, 8))) return rc;
-        k_poly_visit<<<grid_for(ns, 4), 256, 0, st>>>(ns, c->pm, c->d_K, c->g_box, c->g_voff, c->g_vcell,
-                                                      c->g_vsolid, c->poly_reach);
+        if (nchunk > 0)
+            k_poly_visit<false><<<grid_for(nchunk, 4), 256, 0, st>>>(nchunk, c->pm, c->d_K, c->g_box, c->g_csol,
+                                                                     c->g_ccoff, nullptr, c->g_cvoff, c->g_vcell,
+                                                                     c->g_vsolid, c->poly_reach);
         if (nv > 0)
             k_poly_test<<<G, 256, 0, st>>>(c->pm, c->d_K, nv, c->g_vcell, c->g_vsolid, c->g_act, c->g_all, c->g_first);
         scan<unsigned char>(c->g_act, nv, c->g_vpos, c->g_tmp, st);
@@ -2891,8 +2909,18 @@ int poly_step(sdfibm_ctx* c, double dt) {
                                                c->g_seed, c->g_first, c->g_hk, c->g_hv, bits, (const R*)c->U[0],
                                                (const R*)c->U[1], (const R*)c->U[2], (R*)c->g_pcode, c->g_pf, dt);
         }
-        k_poly_reduce<R><<<grid_for(ns, 4), 256, 0, st>>>(ns, c->g_voff, c->g_vpos, (const R*)c->g_pcode, c->g_pf,
-                                                          c->rho_f, c->d_ft, c->d_ncand, c->g_prange);
+        // per solid: its pair range in chunks (a wave each: partial sums), then the chunks' sums
+        // in chunk order
+        k_poly_pair_chunks<<<grid_for(ns, 256), 256, 0, st>>>(ns, c->g_voff, c->g_vpos, c->g_prange, c->g_pnch);
+        if ((rc = poly_grow(c, c->g_tmp, c->g_tmpcap, ns / kScanTile + 2, 8))) return rc;
+        scan<long long>(c->g_pnch, ns, c->g_pcoff, c->g_tmp, st);
+        const long long maxch = ns + np / kPolyPairChunk + 1;  // >= sum of ceil(pairs / chunk)
+        if ((rc = poly_grow(c, c->g_psol, c->g_psol_cap, maxch, 4))) return rc;
+        if ((rc = poly_grow(c, c->g_part, c->g_part_cap, 8 * maxch, 8))) return rc;
+        k_poly_chunk_map<<<grid_for(ns, 4), 256, 0, st>>>(ns, c->g_pcoff, c->g_psol);
+        k_poly_pair_part<R><<<grid_for(maxch, 4), 256, 0, st>>>(maxch, c->g_psol, c->g_pcoff, c->g_prange,
+                                                                (const R*)c->g_pcode, c->g_pf, c->g_part, ns);
+        k_poly_reduce<<<grid_for(ns, 4), 256, 0, st>>>(ns, c->g_pcoff, c->g_part, c->rho_f, c->d_ft, c->d_ncand);
     }
     c->g_np = np;
     (void)hipEventRecord(c->ev[1], st);

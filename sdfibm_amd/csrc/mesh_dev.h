@@ -15,9 +15,7 @@ struct MeshDev {
     const double* ys;
     const double* zs;
     // structured: SoA points, SoA cell centres, volumes, AoS face centres/areas per family
-    const double* px;
-    const double* py;
-    const double* pz;
+    const double* pxyz;  // structured: points AoS [(nx+1)(ny+1)(nz+1)*3]
     const double* ccx;
     const double* ccy;
     const double* ccz;
@@ -64,7 +62,8 @@ __device__ inline d3 vertex_pos(const MeshDev& m, int i, int j, int k) {
         return {m.xs[i], m.ys[j], m.zs[k]};
     } else {
         const long long p = (long long)i + (long long)(m.nx + 1) * ((long long)j + (long long)(m.ny + 1) * k);
-        return {m.px[p], m.py[p], m.pz[p]};
+        const double* q = m.pxyz + 3 * p;  // AoS: one cache line for a point's coordinates
+        return {q[0], q[1], q[2]};
     }
 }
 

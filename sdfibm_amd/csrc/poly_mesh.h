@@ -13,9 +13,10 @@
 //
 // The mesh carries no (i, j, k), so the pass works on (solid, cell) pairs:
 //   k_poly_setup     per solid (a wave each): shape constants, world box, the
-//                    nearest cell (ring search over the cell bins), how many
-//                    cells its box's bins hold
-//   k_poly_visit     per solid: those cells, bin by bin (a fixed order)
+//                    nearest cell (ring search over the cell bins), the number
+//                    of chunks of 512 bins of its box's bin index box
+//   k_poly_visit     per chunk (a wave each): the cells of its bins, counted,
+//                    then listed bin by bin (a fixed order: solid, bin, cell)
 //   k_poly_test      per visited pair: strict inside test of the cell's points
 //                    (FP64) -> active / ALL_INSIDE flags; lowest active cell
 //   k_poly_compact   active pairs -> pair arrays (solid-major: a solid's pairs
@@ -26,7 +27,9 @@
 //   k_poly_force     per pair: membership in the seed's component, code (the
 //                    fraction, signed by the centre test; +inf ALL_INSIDE; NaN
 //                    not a candidate), FP64 force/torque contribution
-//   k_poly_reduce    per solid (a wave each): its pairs' contributions in pair
+//   k_poly_pair_part per chunk of 4,096 of a solid's pairs (a wave each): the
+//                    partial sums of their contributions and their counts
+//   k_poly_reduce    per solid (a wave each): its chunks' partial sums in chunk
 //                    order (x rho_f), its candidate counts
 //   k_poly_cells     per cell: the member pairs that touch it (counted, then
 //                    listed); k_poly_fields writes every cell once, its solids
@@ -37,9 +40,7 @@
 
 struct PolyDev {
     int n_cells, n_faces, n_points, two_d;
-    const double* px;
-    const double* py;
-    const double* pz;
+    const double* pxyz;  // points AoS [n_points*3]
     const int* f_off;    // faces: point lists (CSR)
     const int* f_pts;
     const double* fc;    // face centres [n_faces*3]
@@ -63,7 +64,10 @@ struct PolyDev {
 
 constexpr unsigned long long kPolyEmpty = ~0ull;
 
-__device__ inline d3 poly_point(const PolyDev& P, int p) { return {P.px[p], P.py[p], P.pz[p]}; }
+__device__ inline d3 poly_point(const PolyDev& P, int p) {
+    const double* q = P.pxyz + 3LL * p;
+    return {q[0], q[1], q[2]};
+}
 
 __device__ inline unsigned long long poly_key(int s, int c) {
     return ((unsigned long long)(unsigned)s << 32) | (unsigned)c;
@@ -174,6 +178,22 @@ __device__ inline void poly_plane_box(const PolyDev& P, const SolidK& K, double*
     }
 }
 
+// The bin index box of a solid's world box widened by `reach` bins (clamped to the
+// grid), and its number of bins (0 for an empty box).
+constexpr int kPolyChunk = 512;  // bins per visit chunk (a wave takes 64 at a time)
+__device__ inline long long poly_solid_bins(const PolyDev& P, const double* lo, const double* hi, int reach, int* b0o,
+                                            int* b1o) {
+    int b0[3], b1[3];
+    const bool any = poly_bin_box(P, lo, hi, b0, b1);
+    for (int d = 0; d < 3; ++d) {
+        b0[d] = max(0, b0[d] - reach);
+        b1[d] = min(P.bn[d] - 1, b1[d] + reach);
+        if (b0o) b0o[d] = b0[d];
+        if (b1o) b1o[d] = b1[d];
+    }
+    return any ? (long long)(b1[0] - b0[0] + 1) * (b1[1] - b0[1] + 1) * (b1[2] - b0[2] + 1) : 0;
+}
+
 // Per solid (one wave; 4 per workgroup): constants, world box (a cell can hold an
 // inside point only if its point box meets it), the nearest cell, and the number
 // of cells in the bins whose cell boxes meet the solid's box.  A bin's cells can
@@ -205,23 +225,8 @@ __global__ void __launch_bounds__(256) k_poly_setup(const sdfibm_solid* __restri
         for (int q = 0; q < 6; ++q) ft[6 * s + q] = 0.0;
         first[s] = 0x7fffffff;
     }
-    // cells in the bins whose boxes meet the solid's box
-    int b0[3], b1[3];
-    const bool any = poly_bin_box(P, lo, hi, b0, b1);
-    for (int d = 0; d < 3; ++d) {
-        b0[d] = max(0, b0[d] - reach);
-        b1[d] = min(P.bn[d] - 1, b1[d] + reach);
-    }
-    const int ex = b1[0] - b0[0] + 1, ey = b1[1] - b0[1] + 1;
-    const long long nb = any ? (long long)ex * ey * (b1[2] - b0[2] + 1) : 0;
-    long long cnt = 0;
-    for (long long t = lane; t < nb; t += 64) {
-        const int bi = b0[0] + (int)(t % ex), bj = b0[1] + (int)((t / ex) % ey), bk = b0[2] + (int)(t / ((long long)ex * ey));
-        const long long b = bi + (long long)P.bn[0] * (bj + (long long)P.bn[1] * bk);
-        if (poly_bin_hit(K, P.b_box + 6 * b, lo, hi)) cnt += P.b_off[b + 1] - P.b_off[b];
-    }
-    for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if (lane == 0) vcnt[s] = cnt;
+    // the bins of the solid's (widened) index box, in chunks of kPolyChunk
+    if (lane == 0) vcnt[s] = (poly_solid_bins(P, lo, hi, reach, nullptr, nullptr) + kPolyChunk - 1) / kPolyChunk;
 
     // nearest cell centre (meshSearch::findNearestCell): rings of bins around the centre's
     // bin until no unvisited bin can hold a nearer centre; ties (squared distances within a
@@ -297,33 +302,43 @@ __global__ void __launch_bounds__(256) k_poly_setup(const sdfibm_solid* __restri
     if (lane == 0) seed[s] = bc == 0x7fffffff ? -1 : bc;
 }
 
-// The visited cells of each solid (one wave per solid), bin by bin in bin index
-// order, each bin's cells ascending.
-__global__ void __launch_bounds__(256) k_poly_visit(int ns, PolyDev P, const SolidK* __restrict__ Ks,
-                                                    const double* __restrict__ box,
-                                                    const long long* __restrict__ voff, int* __restrict__ vcell,
-                                                    int* __restrict__ vsolid, int reach) {
+// Chunk -> solid map: chunk coff[s] + k of solid s, k < coff[s + 1] - coff[s]
+// (one wave per solid).
+__global__ void __launch_bounds__(256) k_poly_chunk_map(int ns, const long long* __restrict__ coff,
+                                                        int* __restrict__ csol) {
     const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (s >= ns) return;
-    const SolidK K = Ks[s];
+    for (long long c = coff[s] + lane; c < coff[s + 1]; c += 64) csol[c] = s;
+}
+
+// The visits of one chunk of kPolyChunk bins of a solid's index box (one wave
+// per chunk): its bins in bin index order, 64 at a time, one per lane; the
+// cells of the bins whose boxes can hold an inside point, each bin's cells
+// ascending.  count: the number of cells only (into cnt[chunk]); else the
+// cells and solids written from voff[chunk].
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_poly_visit(long long nchunk, PolyDev P, const SolidK* __restrict__ Ks,
+                                                    const double* __restrict__ box, const int* __restrict__ csol,
+                                                    const long long* __restrict__ coff, long long* __restrict__ cnt,
+                                                    const long long* __restrict__ voff, int* __restrict__ vcell,
+                                                    int* __restrict__ vsolid, int reach) {
+    const long long ch = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (ch >= nchunk) return;
+    const int s = csol[ch];
+    const SolidK& K = Ks[s];
     const double* lo = box + 6 * s;
     const double* hi = lo + 3;
     int b0[3], b1[3];
-    const bool any = poly_bin_box(P, lo, hi, b0, b1);
-    for (int d = 0; d < 3; ++d) {
-        b0[d] = max(0, b0[d] - reach);
-        b1[d] = min(P.bn[d] - 1, b1[d] + reach);
-    }
+    const long long nb = poly_solid_bins(P, lo, hi, reach, b0, b1);
     const int ex = b1[0] - b0[0] + 1, ey = b1[1] - b0[1] + 1;
-    const long long nb = any ? (long long)ex * ey * (b1[2] - b0[2] + 1) : 0;
-    long long o = voff[s];
-    // 64 bins at a time, one per lane: their cell counts scanned over the wave, then each lane
-    // copies its bin's cells (bin order and ascending cells within a bin: a fixed order)
-    for (long long t0 = 0; t0 < nb; t0 += 64) {
+    const long long t_lo = (ch - coff[s]) * kPolyChunk, t_hi = min(nb, t_lo + kPolyChunk);
+    long long o = COUNT ? 0 : voff[ch];
+    for (long long t0 = t_lo; t0 < t_hi; t0 += 64) {
         const long long t = t0 + lane;
         int q0 = 0, n = 0;
-        if (t < nb) {
+        if (t < t_hi) {
             const int bi = b0[0] + (int)(t % ex), bj = b0[1] + (int)((t / ex) % ey), bk = b0[2] + (int)(t / ((long long)ex * ey));
             const long long b = bi + (long long)P.bn[0] * (bj + (long long)P.bn[1] * bk);
             if (poly_bin_hit(K, P.b_box + 6 * b, lo, hi)) {
@@ -336,13 +351,23 @@ __global__ void __launch_bounds__(256) k_poly_visit(int ns, PolyDev P, const Sol
             const int y = __shfl_up(incl, off, 64);
             if (lane >= off) incl += y;
         }
-        const long long at = o + incl - n;
-        for (int q = 0; q < n; ++q) {
-            vcell[at + q] = P.b_cells[q0 + q];
-            vsolid[at + q] = s;
+        if (!COUNT) {
+            const long long at = o + incl - n;
+            for (int q = 0; q < n; ++q) {
+                vcell[at + q] = P.b_cells[q0 + q];
+                vsolid[at + q] = s;
+            }
         }
         o += __shfl(incl, 63, 64);
     }
+    if (COUNT && lane == 0) cnt[ch] = o;
+}
+
+// A solid's first visit: the visit offset of its first chunk (s = ns: the total).
+__global__ void k_poly_solid_voff(int ns, const long long* __restrict__ coff, const long long* __restrict__ cvoff,
+                                  long long* __restrict__ voff) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= ns) voff[s] = cvoff[coff[s]];
 }
 
 // Strict inside test of each visited cell's points (FP64, the reference's
@@ -548,20 +573,35 @@ __global__ void __launch_bounds__(256) k_poly_force(PolyDev P, const SolidK* __r
     }
 }
 
-// Per solid (one wave): force and torque = its pairs' contributions (lanes
-// strided over its pair range, then a fixed shuffle tree) x rho_f; its candidate
-// counts (ALL_INSIDE, cut).
-template <class R>
-__global__ void __launch_bounds__(256) k_poly_reduce(int ns, const long long* __restrict__ voff,
-                                                     const long long* __restrict__ vpos,
-                                                     const R* __restrict__ pcode, const double* __restrict__ pf,
-                                                     double rho_f, double* __restrict__ ft, int2* __restrict__ ncand,
-                                                     long long* __restrict__ prange) {
-    const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+// Pair chunks: a solid's pairs p0 <= p < p1 (p0 = the active scan at its first
+// visit, p1 at the next solid's) in chunks of kPolyPairChunk; per solid its
+// pair range and its number of chunks.
+constexpr int kPolyPairChunk = 4096;
+__global__ void k_poly_pair_chunks(int ns, const long long* __restrict__ voff, const long long* __restrict__ vpos,
+                                   long long* __restrict__ prange, long long* __restrict__ nch) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    // the solid's pairs: the active scan at its first visit and at the next solid's
     const long long p0 = vpos[voff[s]], p1 = vpos[voff[s + 1]];
+    prange[2 * s] = p0;
+    prange[2 * s + 1] = p1;
+    nch[s] = (p1 - p0 + kPolyPairChunk - 1) / kPolyPairChunk;
+}
+
+// One wave per pair chunk (chunks beyond the total exit): the chunk's
+// contributions (lanes strided over its pairs, then a fixed shuffle tree) and
+// its candidate counts, into part[8 * chunk] (six sums, ALL_INSIDE, cut).
+template <class R>
+__global__ void __launch_bounds__(256) k_poly_pair_part(long long maxch, const int* __restrict__ csol,
+                                                        const long long* __restrict__ coff,
+                                                        const long long* __restrict__ prange,
+                                                        const R* __restrict__ pcode, const double* __restrict__ pf,
+                                                        double* __restrict__ part, int ns) {
+    const long long ch = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (ch >= maxch || ch >= coff[ns]) return;
+    const int s = csol[ch];
+    const long long p0 = prange[2 * s] + (ch - coff[s]) * kPolyPairChunk;
+    const long long p1 = min(prange[2 * s + 1], p0 + kPolyPairChunk);
     double a[6] = {0, 0, 0, 0, 0, 0};
     int nall = 0, ncut = 0;
     for (long long p = p0 + lane; p < p1; p += 64) {
@@ -577,10 +617,29 @@ __global__ void __launch_bounds__(256) k_poly_reduce(int ns, const long long* __
         ncut += __shfl_xor(ncut, off, 64);
     }
     if (lane == 0) {
+        for (int t = 0; t < 6; ++t) part[8 * ch + t] = a[t];
+        part[8 * ch + 6] = (double)nall;
+        part[8 * ch + 7] = (double)ncut;
+    }
+}
+
+// Per solid (one wave): force and torque = the sum of its chunks' partial sums
+// (lanes strided over the chunks in order, then a fixed shuffle tree) x rho_f;
+// its candidate counts (ALL_INSIDE, cut).
+__global__ void __launch_bounds__(256) k_poly_reduce(int ns, const long long* __restrict__ coff,
+                                                     const double* __restrict__ part, double rho_f,
+                                                     double* __restrict__ ft, int2* __restrict__ ncand) {
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= ns) return;
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (long long ch = coff[s] + lane; ch < coff[s + 1]; ch += 64)
+        for (int t = 0; t < 8; ++t) a[t] += part[8 * ch + t];
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int t = 0; t < 8; ++t) a[t] += __shfl_xor(a[t], off, 64);
+    if (lane == 0) {
         for (int t = 0; t < 6; ++t) ft[6 * s + t] = a[t] * rho_f;
-        ncand[s] = make_int2(nall, ncut);
-        prange[2 * s] = p0;
-        prange[2 * s + 1] = p1;
+        ncand[s] = make_int2((int)a[6], (int)a[7]);
     }
 }
 
