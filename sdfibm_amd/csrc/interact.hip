@@ -2905,7 +2905,7 @@ int poly_step(sdfibm_ctx* c, double dt) {
                                               c->g_psolid, c->g_pall, c->g_label, c->g_hk, c->g_hv, bits);
             k_poly_hook<<<G, 256, 0, st>>>(c->pm, np, c->g_pcell, c->g_psolid, c->g_label, c->g_hk, c->g_hv, bits);
             k_poly_compress<<<G, 256, 0, st>>>(np, c->g_label);
-            k_poly_force<R><<<G, 256, 0, st>>>(c->pm, c->d_K, np, c->g_pcell, c->g_psolid, c->g_pall, c->g_label,
+            k_poly_force<R><<<4 * G, 64, 0, st>>>(c->pm, c->d_K, np, c->g_pcell, c->g_psolid, c->g_pall, c->g_label,
                                                c->g_seed, c->g_first, c->g_hk, c->g_hv, bits, (const R*)c->U[0],
                                                (const R*)c->U[1], (const R*)c->U[2], (R*)c->g_pcode, c->g_pf, dt);
         }

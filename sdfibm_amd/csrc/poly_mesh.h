@@ -379,10 +379,21 @@ __global__ void __launch_bounds__(256) k_poly_test(PolyDev P, const SolidK* __re
                                                    int* __restrict__ first) {
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (long long)gridDim.x * blockDim.x) {
         const int c = vcell[v], s = vsolid[v];
-        const SolidK& K = Ks[s];
+        const SolidK K = Ks[s];  // the fields the test reads, once per visit
         int nin = 0;
         const int p0 = P.c_poff[c], p1 = P.c_poff[c + 1];
-        for (int t = p0; t < p1; ++t) nin += inside_world(K, poly_point(P, P.c_pts[t])) ? 1 : 0;
+        if (p1 - p0 == 8) {  // hexahedra: the eight labels, then the eight points, in flight together
+            int id[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) id[q] = P.c_pts[p0 + q];
+            d3 pt[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) pt[q] = poly_point(P, id[q]);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) nin += inside_world(K, pt[q]) ? 1 : 0;
+        } else {
+            for (int t = p0; t < p1; ++t) nin += inside_world(K, poly_point(P, P.c_pts[t])) ? 1 : 0;
+        }
         act[v] = nin > 0 ? 1 : 0;
         all[v] = nin == p1 - p0 ? 1 : 0;
         if (nin > 0 && c < __hip_atomic_load(&first[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
@@ -435,8 +446,54 @@ __global__ void __launch_bounds__(256) k_poly_hook(PolyDev P, long long np, cons
                                                    const int* __restrict__ psolid, int* __restrict__ L,
                                                    const unsigned long long* __restrict__ hk, const int* __restrict__ hv,
                                                    int bits) {
+    auto unite = [&](int x, int y) {
+        int a = poly_find(L, x), b = poly_find(L, y);
+        while (a != b) {
+            const int hi = a > b ? a : b, lo = a > b ? b : a;
+            const int old = atomicMin(&L[hi], lo);
+            if (old == hi) break;  // hi was a root and now points at lo
+            a = poly_find(L, old);
+            b = poly_find(L, lo);
+        }
+    };
+    const unsigned long long mask = (1ull << bits) - 1ull;
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (long long)gridDim.x * blockDim.x) {
         const int c = pcell[p], s = psolid[p];
+        const int n0 = P.c_noff[c], nn = P.c_noff[c + 1] - n0;
+        if (nn <= 8) {
+            // up to eight neighbours: their labels, then the first probes of their hash slots,
+            // each in flight together; a probe that neither hits nor finds an empty slot goes on
+            int d[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) d[t] = t < nn ? P.c_nbs[n0 + t] : -1;
+            unsigned long long key[8], h[8], k0[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                key[t] = poly_key(s, d[t] > c ? d[t] : 0);
+                h[t] = poly_hash(key[t], bits);
+                k0[t] = d[t] > c ? hk[h[t]] : kPolyEmpty;
+            }
+            int q[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) q[t] = k0[t] == key[t] && d[t] > c ? hv[h[t]] : -1;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                if (d[t] <= c || k0[t] == kPolyEmpty) continue;
+                if (k0[t] != key[t]) {  // a collision: the rest of the probe sequence
+                    q[t] = -1;
+                    for (unsigned long long hh = (h[t] + 1) & mask;; hh = (hh + 1) & mask) {
+                        const unsigned long long k = hk[hh];
+                        if (k == key[t]) {
+                            q[t] = hv[hh];
+                            break;
+                        }
+                        if (k == kPolyEmpty) break;
+                    }
+                }
+                if (q[t] >= 0) unite((int)p, q[t]);
+            }
+            continue;
+        }
         for (int t = P.c_noff[c]; t < P.c_noff[c + 1]; ++t) {
             const int d = P.c_nbs[t];
             if (d < c) continue;
@@ -523,12 +580,97 @@ __device__ inline R poly_fraction(const PolyDev& P, const SolidK& K, int c) {
     return vol / R(P.vol[c]);
 }
 
+// poly_fraction for a cell of eight points (hexahedra, whatever its faces), the
+// same arithmetic on values computed once: the eight points' coordinates
+// relative to the first one and their signed distances, kept in this thread's
+// LDS slots (element q at [q * 64 + lane]); a face point's slot is found by
+// binary search (cellPoints are ascending labels).  The labels and points are
+// loaded together, not one dependent load after another.
+template <class R>
+__device__ inline R poly_fraction8(const PolyDev& P, const SolidK& K, int c, R* qx, R* qy, R* qz, R* qf, int* ql,
+                                   int lane) {
+    const int* cp = P.c_pts + P.c_poff[c];
+    int lab[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lab[q] = cp[q];
+    d3 pt[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pt[q] = poly_point(P, lab[q]);
+    const d3 w0 = pt[0];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int o = q * 64 + lane;
+        ql[o] = lab[q];
+        qx[o] = R(pt[q].x - w0.x);
+        qy[o] = R(pt[q].y - w0.y);
+        qz[o] = R(pt[q].z - w0.z);
+        qf[o] = sdf_fast<R>(K, pt[q]);
+    }
+    auto rel = [&](int q) { return r3<R>{qx[q * 64 + lane], qy[q * 64 + lane], qz[q * 64 + lane]}; };
+    auto slot = [&](int label) {  // its position among the eight ascending labels
+        int q = 0;
+#pragma unroll
+        for (int st = 4; st >= 1; st >>= 1)
+            if (ql[(q + st) * 64 + lane] <= label) q += st;
+        return q;
+    };
+    // the cell apex over cellPoints: A the first point, B the first later one across the surface
+    const R fA = qf[lane];
+    int b = 7;
+#pragma unroll
+    for (int q = 1; q < 8; ++q)
+        if (b == 7 && fA * qf[q * 64 + lane] <= 0) b = q;
+    r3<R> ap = apex_point(rel(0), fA, rel(b), qf[b * 64 + lane]);
+    if (P.two_d) ap.z = R(0.0 - w0.z);
+    R vol = R(0);
+    for (int t = P.c_foff[c]; t < P.c_foff[c + 1]; ++t) {
+        const int f = P.c_faces[t];
+        const int* fp = P.f_pts + P.f_off[f];
+        const int nf = P.f_off[f + 1] - P.f_off[f];
+        int ss = 0;
+        for (int u = 0; u < nf; ++u) ss += qf[slot(fp[u]) * 64 + lane] > 0 ? 1 : -1;
+        R eps;
+        if (ss == nf) {
+            eps = R(0);
+        } else if (ss == -nf) {
+            eps = R(1);
+        } else {
+            // the face apex over the face's own point order
+            const int s0 = slot(fp[0]);
+            const R gA = qf[s0 * 64 + lane];
+            int sb = s0;
+            for (int u = 1; u < nf; ++u) {
+                sb = slot(fp[u]);
+                if (gA * qf[sb * 64 + lane] <= 0) break;
+            }
+            const r3<R> fa = apex_point(rel(s0), gA, rel(sb), qf[sb * 64 + lane]);
+            R area = R(0);
+            int so = s0;
+            for (int u = 0; u < nf; ++u) {
+                const int sa = slot(fp[(u + 1) % nf]);
+                const r3<R> O = rel(so), A = rel(sa);
+                area += fabs(R(0.5) * rmag(rcross(sub(A, O), sub(fa, O)))) *
+                        line_fraction<R>(qf[so * 64 + lane], qf[sa * 64 + lane]);
+                so = sa;
+            }
+            const r3<R> S{R(P.sf[3 * f]), R(P.sf[3 * f + 1]), R(P.sf[3 * f + 2])};
+            eps = area / rmag(S);
+        }
+        if (eps != R(0)) {
+            const r3<R> fcr{R(P.fc[3 * f] - w0.x), R(P.fc[3 * f + 1] - w0.y), R(P.fc[3 * f + 2] - w0.z)};
+            const r3<R> S{R(P.sf[3 * f]), R(P.sf[3 * f + 1]), R(P.sf[3 * f + 2])};
+            vol += (R(1) / R(3)) * eps * fabs(rdot(sub(ap, fcr), S));
+        }
+    }
+    return vol / R(P.vol[c]);
+}
+
 // Per pair: member of its solid's seed component?  Then its code (+inf
 // ALL_INSIDE; the fraction, negative when the centre is outside) and its force
 // and torque contribution (FP64: f = alpha (uf - us), f V / dt, (cc - c) x f V / dt,
 // solidcloud.cpp:384-422); else NaN and zeros.
 template <class R>
-__global__ void __launch_bounds__(256) k_poly_force(PolyDev P, const SolidK* __restrict__ Ks, long long np,
+__global__ void __launch_bounds__(64) k_poly_force(PolyDev P, const SolidK* __restrict__ Ks, long long np,
                                                     const int* __restrict__ pcell,
                                                     const int* __restrict__ psolid, const unsigned char* __restrict__ pall,
                                                     const int* __restrict__ L, const int* __restrict__ seed,
@@ -536,6 +678,9 @@ __global__ void __launch_bounds__(256) k_poly_force(PolyDev P, const SolidK* __r
                                                     const int* __restrict__ hv, int bits, const R* __restrict__ Ux,
                                                     const R* __restrict__ Uy, const R* __restrict__ Uz,
                                                     R* __restrict__ pcode, double* __restrict__ pf, double dt) {
+    __shared__ R qx[8 * 64], qy[8 * 64], qz[8 * 64], qf[8 * 64];  // hexahedra: per thread its cell's points
+    __shared__ int ql[8 * 64];
+    const int lane = threadIdx.x;
     const double dtINV = 1.0 / dt;
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (long long)gridDim.x * blockDim.x) {
         const int c = pcell[p], s = psolid[p];
@@ -553,7 +698,8 @@ __global__ void __launch_bounds__(256) k_poly_force(PolyDev P, const SolidK* __r
         if (pall[p]) {
             pcode[p] = R(__builtin_huge_val());
         } else {
-            const R a = poly_fraction<R>(P, K, c);
+            const R a = P.c_poff[c + 1] - P.c_poff[c] == 8 ? poly_fraction8<R>(P, K, c, qx, qy, qz, qf, ql, lane)
+                                                            : poly_fraction<R>(P, K, c);
             const bool cin = inside_world(K, d3{P.cc[3 * c], P.cc[3 * c + 1], P.cc[3 * c + 2]});
             pcode[p] = cin ? a : -a;
             alpha = (double)a;
