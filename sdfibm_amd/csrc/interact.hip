@@ -158,31 +158,54 @@ __host__ __device__ inline int tile_queue(int cls) {
     return cls == TC_SMALL ? 0 : (cls == TC_MEDIUM ? 1 : (cls == TC_BIG ? 4 : -1));
 }
 
+// The tiles' code and list offsets: one atomic per wave on the step's running
+// total (a bump allocation; the layout does not depend on the order, every tile
+// is addressed through its own offset), the wave's tiles in lane order.
 template <int MK>
 __global__ void __launch_bounds__(64) k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, SolidK* __restrict__ K,
-                        TileInfo* __restrict__ T, long long* __restrict__ tile_cells, double* __restrict__ ft,
-                        int* __restrict__ qmeta, int* __restrict__ qlist, const unsigned char* __restrict__ unb,
-                        int* __restrict__ err, const CsgK* __restrict__ csg, int big_on) {
+                        TileInfo* __restrict__ T, long long* __restrict__ total, long long code_cap,
+                        double* __restrict__ ft, int* __restrict__ qmeta, int* __restrict__ qlist,
+                        const unsigned char* __restrict__ unb, int* __restrict__ err, const CsgK* __restrict__ csg,
+                        int big_on) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
-    const SolidK k = make_solid_k(in[s], csg);
-    K[s] = k;
-    TileInfo t = make_tile<MK>(m, k);
-    // solids set_solids put on the unbounded path stay there (their tile box is the cull)
-    if (unb[s] && t.cls != TC_PLANE) t.cls = t.cls == TC_EMPTY ? TC_EMPTY : TC_HUGE;
-    // structured meshes: the seed search (a box of cell centres) is spread over a wave by k_seed
-    t.seed = t.cls == TC_EMPTY ? -1 : (MK == MK_RECT ? nearest_cell<MK>(m, k.c) : -2);
-    // a tile beyond 2^22 vertices, or a big tile when set_solids found none possible (the big-tile
-    // pass is not launched): interact classifies the solids again
-    if (t.cls == TC_TOOBIG || (t.cls == TC_BIG && !big_on)) atomicOr(err, EF_TILE);
-    tile_cells[s] = tile_is_finite(t.cls) ? (long long)t.n[0] * t.n[1] * t.n[2] : 0;
+    const int lane = threadIdx.x & 63;
+    const bool live = s < ns;  // the whole wave takes part in the offsets' scan
+    TileInfo t{};
+    t.cls = TC_EMPTY;
+    if (live) {
+        const SolidK k = make_solid_k(in[s], csg);
+        K[s] = k;
+        t = make_tile<MK>(m, k);
+        // solids set_solids put on the unbounded path stay there (their tile box is the cull)
+        if (unb[s] && t.cls != TC_PLANE) t.cls = t.cls == TC_EMPTY ? TC_EMPTY : TC_HUGE;
+        // structured meshes: the seed search (a box of cell centres) is spread over a wave by k_seed
+        t.seed = t.cls == TC_EMPTY ? -1 : (MK == MK_RECT ? nearest_cell<MK>(m, k.c) : -2);
+        // a tile beyond 2^22 vertices, or a big tile when set_solids found none possible (the
+        // big-tile pass is not launched): interact classifies the solids again
+        if (t.cls == TC_TOOBIG || (t.cls == TC_BIG && !big_on)) atomicOr(err, EF_TILE);
+        for (int q = 0; q < 6; ++q) ft[6 * s + q] = 0.0;  // solids that touch no cell exert nothing
+    }
+    const long long tc = live && tile_is_finite(t.cls) ? (long long)t.n[0] * t.n[1] * t.n[2] : 0;
+    long long x = tc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const long long y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    const long long wtot = __shfl(x, 63, 64);
+    long long base = 0;
+    if (lane == 0 && wtot > 0) {
+        base = (long long)atomicAdd((unsigned long long*)total, (unsigned long long)wtot);
+        if (base + wtot > code_cap) atomicOr(err, EF_CODECAP);
+    }
+    base = __shfl(base, 0, 64);
+    if (!live) return;
+    t.off = base + x - tc;
     T[s] = t;
-    for (int q = 0; q < 6; ++q) ft[6 * s + q] = 0.0;  // solids that touch no cell exert nothing
     // queue the solid: one atomic per wave and queue (the lanes of a wave are
     // consecutive solids, queued in index order)
     int q = tile_queue(t.cls);
     if (q == 0 && (t.n[1] + 1) * (t.n[2] + 1) <= kRowsTiny) q = 3;  // tiny tiles: a wave each
-    const int lane = threadIdx.x & 63;
     for (int qq = 0; qq < kQueues; ++qq) {
         const unsigned long long b = __ballot(q == qq);
         if (!b) continue;
@@ -333,81 +356,18 @@ void scan(const T* in, long long n, long long* out, long long* tmp, hipStream_t 
     k_scan_tiles<T><<<nb, 256, 0, st>>>(in, n, tmp, nb, out);
 }
 
-__global__ void k_tile_offsets(TileInfo* __restrict__ T, const long long* __restrict__ off, int ns,
-                               long long cap, int* __restrict__ err) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0 && off[ns] > cap) atomicOr(err, EF_CODECAP);
-    if (s >= ns) return;
-    T[s].off = off[s];
-}
-
-// Up to kScanOne solids: the tile sizes' exclusive scan and the tile offsets in
-// one workgroup of 1024 threads.  With R = ceil(ns / 4096) rounds, wave w takes
-// values [256 R w, 256 R (w + 1)): first their total (the 16 wave totals go
-// through LDS), then R rounds of 256 consecutive values, 4 per lane (coalesced
-// loads): sums within the lane, a shuffle scan of the lane sums, a carry between
-// rounds.  off[ns] = total; the other entries of off are not written.
-constexpr int kScanOne = 16384;
-
-__global__ void __launch_bounds__(1024) k_tile_offsets_one(const long long* __restrict__ cells, int ns,
-                                                           TileInfo* __restrict__ T, long long* __restrict__ off,
-                                                           long long cap, int* __restrict__ err) {
-    __shared__ long long wsum[16];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int rounds = (ns + 4095) / 4096;  // <= 4
-    const int wbase = wv * 256 * rounds;
-    long long sum = 0;
-    for (int q = 0; q < rounds; ++q) {
-        const int e = wbase + 256 * q + 4 * lane;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) sum += e + u < ns ? cells[e + u] : 0;
-    }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
-    if (lane == 0) wsum[wv] = sum;
-    __syncthreads();
-    long long carry = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < 16; ++w) {
-        const long long ws = wsum[w];
-        if (w < wv) carry += ws;
-        tot += ws;
-    }
-    for (int q = 0; q < rounds && wbase + 256 * q < ns; ++q) {  // wave-uniform bound
-        const int e = wbase + 256 * q + 4 * lane;
-        long long v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = e + u < ns ? cells[e + u] : 0;
-        const long long own = v[0] + v[1] + v[2] + v[3];
-        long long x = own;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const long long y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
-        }
-        long long run = carry + x - own;  // exclusive prefix of the lane's first value
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (e + u < ns) T[e + u].off = run;
-            run += v[u];
-        }
-        carry += __shfl(x, 63, 64);
-    }
-    if (t == 0) {
-        off[ns] = tot;
-        if (tot > cap) atomicOr(err, EF_CODECAP);
-    }
-}
 
 // Per-step resets: error flags, queue counters, brick counts.
 __global__ void k_flags_out(const int* __restrict__ err, int* __restrict__ host_word) {
     if (threadIdx.x == 0) *(volatile int*)host_word = *err;
 }
 
-__global__ void k_reset(int* __restrict__ err, int* __restrict__ qmeta, int* __restrict__ bcount, long long nbricks) {
+__global__ void k_reset(int* __restrict__ err, int* __restrict__ qmeta, int* __restrict__ bcount, long long nbricks,
+                        long long* __restrict__ total) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *err = 0;
     if (i < 8) qmeta[i] = 0;
+    if (i == 0) *total = 0;  // the tiles' running total (k_setup)
     if (i < nbricks) bcount[i] = 0;
 }
 
@@ -1711,8 +1671,7 @@ struct sdfibm_ctx {
     sdfibm_solid* d_solids = nullptr;
     SolidK* d_K = nullptr;
     TileInfo* d_T = nullptr;
-    long long* d_tile_cells = nullptr;
-    long long* d_tile_off = nullptr;
+    long long* d_tile_total = nullptr;  // the step's tile cells (k_setup's running total)
     long long solid_cap = 0;
     void* d_codes = nullptr;
     int* d_cand = nullptr;  // candidate lists, same offsets as the codes
@@ -1992,18 +1951,13 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     const int ns = (int)c->ns;
     hipStream_t st = c->stream;
     if (!graph) (void)hipEventRecord(c->ev[0], st);
-    k_reset<<<grid_for(std::max(c->nbricks, 8LL), 256), 256, 0, st>>>(c->err, c->qmeta, c->bcount, c->nbricks);
+    k_reset<<<grid_for(std::max(c->nbricks, 8LL), 256), 256, 0, st>>>(c->err, c->qmeta, c->bcount, c->nbricks,
+                                                                     c->d_tile_total);
     if (ns > 0) {
-        k_setup<MK><<<grid_for(ns, 64), 64, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_cells,
-                                                       c->d_ft, c->qmeta, c->qlist, c->d_unb, c->err, c->d_csg,
-                                                       c->big_on ? 1 : 0);
+        k_setup<MK><<<grid_for(ns, 64), 64, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_total,
+                                                       c->code_cap, c->d_ft, c->qmeta, c->qlist, c->d_unb, c->err,
+                                                       c->d_csg, c->big_on ? 1 : 0);
         if (MK != MK_RECT) k_seed<<<grid_for(ns, 4), 256, 0, st>>>(c->m, ns, c->d_K, c->d_T);
-        if (ns <= kScanOne) {
-            k_tile_offsets_one<<<1, 1024, 0, st>>>(c->d_tile_cells, ns, c->d_T, c->d_tile_off, c->code_cap, c->err);
-        } else {
-            scan<long long>(c->d_tile_cells, ns, c->d_tile_off, c->scan_tmp, st);
-            k_tile_offsets<<<grid_for(ns, 128), 128, 0, st>>>(c->d_T, c->d_tile_off, ns, c->code_cap, c->err);
-        }
     }
     if (ns > 0) {
         const unsigned gb = grid_for(64LL * ns, 256);
@@ -3057,6 +3011,7 @@ int create_end(sdfibm_ctx* c) {
     if ((rc = dalloc(c, (void**)&c->Ct, c->N * 4))) return rc;
     if ((rc = dalloc(c, (void**)&c->err, 64))) return rc;
     if ((rc = dalloc(c, (void**)&c->qmeta, 64))) return rc;
+    if ((rc = dalloc(c, (void**)&c->d_tile_total, 64))) return rc;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
         c->n_cu = prop.multiProcessorCount;
@@ -3371,16 +3326,12 @@ int classify_solids(sdfibm_ctx* c, const sdfibm_solid* solids, int64_t n) {
         dfree(c, c->d_solids);
         dfree(c, c->d_K);
         dfree(c, c->d_T);
-        dfree(c, c->d_tile_cells);
-        dfree(c, c->d_tile_off);
         dfree(c, c->d_ft);
         c->solid_cap = 0;
         const long long cap = std::max<long long>(n, 16);
         if ((rc = dalloc(c, (void**)&c->d_solids, cap * sizeof(sdfibm_solid)))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_K, cap * sizeof(SolidK)))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_T, cap * sizeof(TileInfo)))) return rc;
-        if ((rc = dalloc(c, (void**)&c->d_tile_cells, cap * 8))) return rc;
-        if ((rc = dalloc(c, (void**)&c->d_tile_off, (cap + 1) * 8))) return rc;
         if ((rc = dalloc(c, (void**)&c->d_ft, cap * 6 * 8))) return rc;
         dfree(c, c->d_ncand);
         if ((rc = dalloc(c, (void**)&c->d_ncand, cap * sizeof(int2)))) return rc;
@@ -3548,7 +3499,7 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
         // (or the big-tile pass's buffers): grow them to the sizes this step needs (with
         // headroom) and run the step again
         long long need = 0;
-        HIPCHK(c, hipMemcpy(&need, c->d_tile_off + c->ns, 8, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(&need, c->d_tile_total, 8, hipMemcpyDeviceToHost));
         if (need > c->code_cap) {
             const long long want = need + need / 4 + 1024;
             graph_reset(c);
@@ -3792,7 +3743,7 @@ int sdfibm_run_steps(sdfibm_ctx* c, double time, double dt, int64_t n_steps, con
     }
     if (n_steps == 0) return SDFIBM_OK;
     long long need = 0;
-    if (ns > 0) HIPCHK(c, hipMemcpy(&need, c->d_tile_off + ns, 8, hipMemcpyDeviceToHost));
+    if (ns > 0) HIPCHK(c, hipMemcpy(&need, c->d_tile_total, 8, hipMemcpyDeviceToHost));
     const long long want = need + need / 2 + 4096;  // headroom: tiles grow as the solids move and turn
     if (want > c->code_cap) {
         graph_reset(c);
