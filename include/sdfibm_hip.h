@@ -171,7 +171,7 @@ int sdfibm_create(const sdfibm_mesh_desc* mesh, const sdfibm_config* cfg, sdfibm
 /* The same for a general polyhedral mesh (the constructor's MeshInfo binding,
  * meshinfo.h:22-29, for any OpenFOAM mesh).  Every call below works on such a
  * context except sdfibm_run_steps, sdfibm_interact_steps, sdfibm_autotune and the
- * surrogate (SDFIBM_ERR_STATE): its step reads two counts back to size its
+ * surrogate (SDFIBM_ERR_STATE): its step reads three counts back to size its
  * buffers, so it is not captured as a graph.  sdfibm_last_timing: [0] the
  * (solid, cell) pairs, their labels and force/torque, [1] the per-cell lists,
  * [2] the field pass, [3] total. */
