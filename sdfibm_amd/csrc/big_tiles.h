@@ -169,6 +169,20 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
     }
 }
 
+// A CSG solid's shape constants copied to LDS for one workgroup iteration (the
+// whole workgroup; its previous readers are done first): read from there, they
+// are not loaded again after each of the loop's stores, which the compiler
+// cannot tell apart from the global table.
+__device__ inline void big_csg_to_lds(SolidK& K, CsgK& sh, int tid, int nt) {
+    if (K.shape != SDFIBM_SHAPE_CSG) return;  // uniform
+    __syncthreads();
+    const unsigned* src = reinterpret_cast<const unsigned*>(K.csg);
+    unsigned* dst = reinterpret_cast<unsigned*>(&sh);
+    for (int t = tid; t < (int)(sizeof(CsgK) / 4); t += nt) dst[t] = src[t];
+    __syncthreads();
+    K.csg = &sh;
+}
+
 // Strict inside test at every big-tile vertex (FP64, the reference's operation
 // order, as k_classify): blocks of kBigWPB bit words of one solid per workgroup
 // iteration (the solid found once), a wave per word (one ballot).
@@ -177,6 +191,7 @@ __global__ void __launch_bounds__(256) k_big_vertices(MeshDev m, const SolidK* _
                                                       const TileInfo* __restrict__ Ts, const BigMeta* __restrict__ meta,
                                                       const long long* __restrict__ tot,
                                                       unsigned long long* __restrict__ vbits) {
+    __shared__ CsgK sh_csg;
     const int nbig = (int)tot[0];
     const long long nvb = tot[8];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -184,7 +199,8 @@ __global__ void __launch_bounds__(256) k_big_vertices(MeshDev m, const SolidK* _
         const int b = big_find(nbig, blk, [&](int x) { return (long long)meta[x].vblk0; });  // uniform
         const BigMeta M = meta[b];
         const TileInfo T = Ts[M.solid];
-        const SolidK K = Ks[M.solid];
+        SolidK K = Ks[M.solid];
+        big_csg_to_lds(K, sh_csg, threadIdx.x, 256);
         const int VX = T.n[0] + 1, VY = T.n[1] + 1;
         const long long V = (long long)VX * VY * (T.n[2] + 1), words = (V + 63) / 64;
         const float rvx = 1.0f / (float)VX, rvy = 1.0f / (float)VY;
@@ -604,6 +620,7 @@ __global__ void __launch_bounds__(NT) k_big_force(MeshDev m, const SolidK* __res
                                                   R* __restrict__ codes, long long code_cap, double dt,
                                                   double* __restrict__ part, int chunk_entries) {
     __shared__ double red[6 * 16];
+    __shared__ CsgK sh_csg;
     const int nbig = (int)tot[0];
     const long long nch = tot[4];
     const double dtINV = 1.0 / dt;
@@ -613,7 +630,8 @@ __global__ void __launch_bounds__(NT) k_big_force(MeshDev m, const SolidK* __res
         const int s = meta[b].solid;
         const TileInfo T = Ts[s];
         if (T.off + (long long)T.n[0] * T.n[1] * T.n[2] > code_cap) continue;  // flagged; uniform per workgroup
-        const SolidK K = Ks[s];
+        SolidK K = Ks[s];
+        big_csg_to_lds(K, sh_csg, tid, NT);
         const int2 cnt = ncand[s];
         const int nin = cnt.x, ntot = cnt.x + cnt.y;
         const int e0 = (int)(ch - meta[b].chunk0) * chunk_entries, e1 = min(e0 + chunk_entries, ntot);
