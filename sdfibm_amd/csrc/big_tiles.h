@@ -38,7 +38,8 @@
 //   k_big_write     candidate lists (stable compaction of each block's cells)
 //   k_big_chunks    chunks of candidates (the chunk size is a run-time choice), their offsets
 //   k_big_force     fractions, codes and forcing per chunk; FP64 partial sums
-//   k_big_ft        partials summed in chunk order (x rho_f) -> force/torque
+//   k_big_ft        partials summed in a fixed order, a wave per solid (x rho_f)
+//                   -> force/torque
 #pragma once
 
 constexpr int kBigPer = 16;                  // big solids per thread of the one-workgroup scans
@@ -94,6 +95,73 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
     if (nbig > kBigMaxSolids) {
         if (tid == 0) atomicOr(err, EF_CODECAP);
         nbig = 0;
+    }
+    auto totals = [&](long long words, long long cells, long long blocks, long long vblocks, long long ublocks) {
+        // the block counts live in a buffer of words / 16 + kBigMaxSolids entries (a tile has at
+        // least as many vertices as cells, so blocks <= words * 64 / kBigCPB + big solids)
+        const bool fits = words + 1 <= word_cap && blocks <= word_cap / 16 + kBigMaxSolids;  // + the guard word
+        if (!fits) atomicOr(err, EF_CODECAP);
+        tot[0] = fits ? nbig : 0;
+        tot[1] = fits ? words * 64 : 0;
+        tot[2] = fits ? cells : 0;
+        tot[3] = words;
+        tot[5] = words;
+        tot[7] = fits ? blocks : 0;
+        tot[8] = fits ? vblocks : 0;
+        tot[9] = fits ? ublocks : 0;
+    };
+    if (nbig <= 1024) {  // a solid per thread: wave scans by shuffles, the 16 wave totals through LDS
+        const int lane = tid & 63, wv = tid >> 6;
+        long long v[5] = {0, 0, 0, 0, 0}, un = 0;
+        int solid = 0;
+        if (tid < nbig) {
+            solid = qlist[4LL * ns + tid];
+            const TileInfo T = Ts[solid];
+            const long long V = (long long)(T.n[0] + 1) * (T.n[1] + 1) * (T.n[2] + 1);
+            v[0] = (V + 63) / 64;
+            v[1] = (long long)T.n[0] * T.n[1] * T.n[2];
+            un = (long long)T.n[1] * T.n[2] * ((T.n[0] + 63) >> 6);
+            v[2] = (v[1] + kBigCPB - 1) / kBigCPB;
+            v[3] = (v[0] + kBigWPB - 1) / kBigWPB;
+            v[4] = (un + kBigUPB - 1) / kBigUPB;
+        }
+        long long x[5];
+        for (int r = 0; r < 5; ++r) {
+            x[r] = v[r];
+            for (int d = 1; d < 64; d <<= 1) {
+                const long long y = __shfl_up(x[r], d, 64);
+                if (lane >= d) x[r] += y;
+            }
+            if (lane == 63) part[r][wv] = x[r];
+        }
+        __syncthreads();
+        long long o[5], all[5];
+        for (int r = 0; r < 5; ++r) {
+            o[r] = x[r] - v[r];
+            all[r] = 0;
+            for (int w = 0; w < 16; ++w) {
+                const long long t = part[r][w];
+                if (w < wv) o[r] += t;
+                all[r] += t;
+            }
+        }
+        if (tid < nbig) {
+            BigMeta M;
+            M.solid = solid;
+            M.first = 0x7fffffff;
+            M.woff = o[0];
+            M.vbase = o[0] * 64;
+            M.cbase = o[1];
+            M.chunk0 = 0;
+            M.nchunk = 0;
+            M.blk0 = (int)o[2];
+            M.nblk = (int)v[2];
+            M.vblk0 = (int)o[3];
+            M.ublk0 = (int)o[4];
+            meta[tid] = M;
+        }
+        if (tid == 0) totals(all[0], all[1], all[2], all[3], all[4]);
+        return;
     }
     long long w[kBigPer], c[kBigPer], un[kBigPer];
     long long sw = 0, sc = 0, sk = 0, sv = 0, su = 0;
@@ -151,22 +219,7 @@ __global__ void __launch_bounds__(1024) k_big_layout(int ns, const TileInfo* __r
         ov += (w[u] + kBigWPB - 1) / kBigWPB;
         ou += (un[u] + kBigUPB - 1) / kBigUPB;
     }
-    if (tid == 1023) {
-        const long long words = part[0][1023], cells = part[1][1023], blocks = part[2][1023], vblocks = part[3][1023],
-                        ublocks = part[4][1023];
-        // the block counts live in a buffer of words / 16 + kBigMaxSolids entries (a tile has at
-        // least as many vertices as cells, so blocks <= words * 64 / kBigCPB + big solids)
-        const bool fits = words + 1 <= word_cap && blocks <= word_cap / 16 + kBigMaxSolids;  // + the guard word
-        if (!fits) atomicOr(err, EF_CODECAP);
-        tot[0] = fits ? nbig : 0;
-        tot[1] = fits ? words * 64 : 0;
-        tot[2] = fits ? cells : 0;
-        tot[3] = words;
-        tot[5] = words;
-        tot[7] = fits ? blocks : 0;
-        tot[8] = fits ? vblocks : 0;
-        tot[9] = fits ? ublocks : 0;
-    }
+    if (tid == 1023) totals(part[0][1023], part[1][1023], part[2][1023], part[3][1023], part[4][1023]);
 }
 
 // A CSG solid's shape constants copied to LDS for one workgroup iteration (the
@@ -654,18 +707,25 @@ __global__ void __launch_bounds__(NT) k_big_force(MeshDev m, const SolidK* __res
     }
 }
 
-// Force and torque of the big solids: chunk partials in chunk order, x rho_f.
+// Force and torque of the big solids: chunk partials in a fixed order (lanes
+// strided over the chunks, then a shuffle tree), x rho_f.
 __global__ void __launch_bounds__(256) k_big_ft(const BigMeta* __restrict__ meta, const long long* __restrict__ tot,
                                                 const double* __restrict__ part, double rho_f,
                                                 double* __restrict__ ft) {
     const int nbig = (int)tot[0];
     const bool ok = tot[4] > 0 || tot[6] == 0;  // chunk partials exist (or there are no candidates)
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nbig; b += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    // a wave per solid: lanes strided over its chunks in order, then a fixed shuffle tree
+    for (int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); b < nbig; b += nw) {
         const BigMeta M = meta[b];
         double a[6] = {0, 0, 0, 0, 0, 0};
         if (ok)
-            for (int c = 0; c < M.nchunk; ++c)
+            for (int c = lane; c < M.nchunk; c += 64)
                 for (int t = 0; t < 6; ++t) a[t] += part[6LL * (M.chunk0 + c) + t];
-        for (int t = 0; t < 6; ++t) ft[6 * M.solid + t] = a[t] * rho_f;
+        for (int off = 32; off >= 1; off >>= 1)
+            for (int t = 0; t < 6; ++t) a[t] += __shfl_xor(a[t], off, 64);
+        if (lane == 0)
+            for (int t = 0; t < 6; ++t) ft[6 * M.solid + t] = a[t] * rho_f;
     }
 }

@@ -1940,7 +1940,7 @@ void launch_big(sdfibm_ctx* c, double dt, hipStream_t st) {
                                                      c->d_ncand, (const R*)c->U[0], (const R*)c->U[1],
                                                      (const R*)c->U[2], codes, c->code_cap, dt, c->d_big_part,
                                                      c->big_chunk);
-    k_big_ft<<<4, 256, 0, st>>>(c->d_big_meta, c->d_big_tot, c->d_big_part, c->rho_f, c->d_ft);
+    k_big_ft<<<64, 256, 0, st>>>(c->d_big_meta, c->d_big_tot, c->d_big_part, c->rho_f, c->d_ft);
 }
 
 // graph: the launches are being captured into a HIP graph -- no timing events
