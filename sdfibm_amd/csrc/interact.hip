@@ -719,7 +719,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
 }
 
 // The forcing of one candidate cell (solidcloud.cpp:384-422): f = alpha (uf - us),
-// force += f V / dt, torque += (cc - c) x f V / dt, in FP64.
+// force += f V / dt, torque += (cc - c) x f V / dt, summed in FP64 (each cell's
+// terms in FP32 when the fields are FP32).
 template <class R, int MK>
 __device__ inline void add_force(const MeshDev& m, const SolidK& K, int gi, int gj, int gk, double alpha,
                                  const R* __restrict__ Ux, const R* __restrict__ Uy, const R* __restrict__ Uz,
@@ -728,6 +729,23 @@ __device__ inline void add_force(const MeshDev& m, const SolidK& K, int gi, int 
     const d3 cc = cell_centre<MK>(m, gi, gj, gk);
     const double V = cell_volume<MK>(m, gi, gj, gk);
     const d3 rv = {cc.x - K.c[0], cc.y - K.c[1], cc.z - K.c[2]};
+    if constexpr (sizeof(R) == sizeof(float)) {
+        // FP32 fields: the cell's force and torque in FP32 (the field's own precision), summed in FP64
+        const float rx = (float)rv.x, ry = (float)rv.y, rz = (float)rv.z;
+        const float wx = (float)K.w[0], wy = (float)K.w[1], wz = (float)K.w[2];
+        const float usx = (float)K.v[0] + (wy * rz - wz * ry), usy = (float)K.v[1] + (wz * rx - wx * rz),
+                    usz = (float)K.v[2] + (wx * ry - wy * rx);
+        const float a = (float)alpha;
+        const float fx = a * (Ux[c] - usx), fy = a * (Uy[c] - usy), fz = a * (Uz[c] - usz);
+        const float sc = (float)(V * dtINV);
+        acc[0] += (double)(fx * sc);
+        acc[1] += (double)(fy * sc);
+        acc[2] += (double)(fz * sc);
+        acc[3] += (double)((ry * fz - rz * fy) * sc);
+        acc[4] += (double)((rz * fx - rx * fz) * sc);
+        acc[5] += (double)((rx * fy - ry * fx) * sc);
+        return;
+    }
     const d3 us = d3{K.v[0], K.v[1], K.v[2]} + cross(d3{K.w[0], K.w[1], K.w[2]}, rv);
     const d3 uf{(double)Ux[c], (double)Uy[c], (double)Uz[c]};
     const d3 f = alpha * (uf - us);
