@@ -38,6 +38,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <mutex>
 #include <vector>
 
@@ -1127,16 +1128,23 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
     const int* bl = blist + b * kMaxPerBrick;
     const int spec = tid < kSpec ? bl[tid] : 0;
     const int cnt = min(bcount[b], kMaxPerBrick);  // k_bin flagged an overflow
-    // the cells' velocity, centre and volume (used if a listed solid or a plane takes them)
-    d3 uf[CPT], cc[CPT];
+    // the cells' velocity, centre and volume (used if a listed solid or a plane takes them);
+    // a cell's terms in the fields' precision A (FP32 fields: FP32 terms and sums of the few
+    // solids sharing the cell; the planes' force and torque partials stay FP64)
+    using A = typename std::conditional<sizeof(R) == sizeof(float), float, double>::type;
+    A ufx[CPT], ufy[CPT], ufz[CPT];
+    d3 cc[CPT];
     double V[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
-        uf[q] = cc[q] = d3{0, 0, 0};
+        ufx[q] = ufy[q] = ufz[q] = A(0);
+        cc[q] = d3{0, 0, 0};
         V[q] = 0.0;
         if (valid[q]) {  // issued with the list loads, before the count is known
             const long long c = cell_id(m, i, j, kq[q]);
-            uf[q] = {(double)Ux[c], (double)Uy[c], (double)Uz[c]};
+            ufx[q] = A(Ux[c]);
+            ufy[q] = A(Uy[c]);
+            ufz[q] = A(Uz[c]);
             cc[q] = cell_centre<MK>(m, i, j, kq[q]);
             V[q] = cell_volume<MK>(m, i, j, kq[q]);
         }
@@ -1162,33 +1170,37 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
     if constexpr (HASP) brick_vertex_box<MK, BX, BY, BZ>(m, brick_box, b, bx, by, bz, bb);
 
     const double dtINV = 1.0 / dt;
-    double sA[CPT], sT[CPT], sF[CPT][3];
+    const A dti = A(dtINV);
+    A sA[CPT], sT[CPT], sF[CPT][3];
     int ct[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
-        sA[q] = sT[q] = sF[q][0] = sF[q][1] = sF[q][2] = 0.0;
+        sA[q] = sT[q] = sF[q][0] = sF[q][1] = sF[q][2] = A(0);
         ct[q] = 0;
     }
     // contribute: a solid's share in cell q; a plane also adds its force and torque to pa
     auto contribute = [&](int q, const double* cvw, double alpha, int ctype, double* pa) {
-        const d3 r = {cc[q].x - cvw[0], cc[q].y - cvw[1], cc[q].z - cvw[2]};
-        const d3 us = d3{cvw[3], cvw[4], cvw[5]} + cross(d3{cvw[6], cvw[7], cvw[8]}, r);
-        const d3 f = alpha * (uf[q] - us);
-        sA[q] += alpha;
-        sT[q] += alpha;
-        sF[q][0] += f.x * dtINV;
-        sF[q][1] += f.y * dtINV;
-        sF[q][2] += f.z * dtINV;
+        const A rx = A(cc[q].x - cvw[0]), ry = A(cc[q].y - cvw[1]), rz = A(cc[q].z - cvw[2]);
+        const A wx = A(cvw[6]), wy = A(cvw[7]), wz = A(cvw[8]);
+        const A usx = A(cvw[3]) + (wy * rz - wz * ry), usy = A(cvw[4]) + (wz * rx - wx * rz),
+                usz = A(cvw[5]) + (wx * ry - wy * rx);
+        const A a = A(alpha);
+        const A fx = a * (ufx[q] - usx), fy = a * (ufy[q] - usy), fz = a * (ufz[q] - usz);
+        sA[q] += a;
+        sT[q] += a;
+        sF[q][0] += fx * dti;
+        sF[q][1] += fy * dti;
+        sF[q][2] += fz * dti;
         ct[q] = ctype;
         if (HASP && pa != nullptr) {
-            const d3 tq = cross(r, f);
-            double* a = pa;
-            a[0] += (f.x * V[q]) * dtINV;
-            a[1] += (f.y * V[q]) * dtINV;
-            a[2] += (f.z * V[q]) * dtINV;
-            a[3] += (tq.x * V[q]) * dtINV;
-            a[4] += (tq.y * V[q]) * dtINV;
-            a[5] += (tq.z * V[q]) * dtINV;
+            const A tx = ry * fz - rz * fy, ty = rz * fx - rx * fz, tz = rx * fy - ry * fx;
+            double* p = pa;
+            p[0] += ((double)fx * V[q]) * dtINV;
+            p[1] += ((double)fy * V[q]) * dtINV;
+            p[2] += ((double)fz * V[q]) * dtINV;
+            p[3] += ((double)tx * V[q]) * dtINV;
+            p[4] += ((double)ty * V[q]) * dtINV;
+            p[5] += ((double)tz * V[q]) * dtINV;
         }
     };
 
@@ -1292,7 +1304,7 @@ __global__ void __launch_bounds__(BX* BY* BZ / CPT)
         if (!valid[q]) continue;
         const long long c = cell_id(m, i, j, kq[q]);
         // streaming stores: the fields are not read again in this step
-        __builtin_nontemporal_store(R(sA[q] < 1.0 ? sA[q] : 1.0), &As[c]);
+        __builtin_nontemporal_store(R(sA[q] < A(1) ? sA[q] : A(1)), &As[c]);
         __builtin_nontemporal_store(R(sF[q][0]), &Fx[c]);
         __builtin_nontemporal_store(R(sF[q][1]), &Fy[c]);
         __builtin_nontemporal_store(R(sF[q][2]), &Fz[c]);
