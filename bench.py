@@ -436,7 +436,7 @@ def bench_ellipsoids_loop(args):
                 "candidate_pairs": int(pairs_poly), "structured_candidate_pairs": int(pairs_init),
                 "max_abs_dAs_vs_structured": d_as, "Ct_cells_differing_from_structured": ct_diff,
                 "structured_interact_ms": float(ph[3]), "setup_s": setup_s,
-                "note": "interact of the initial state, eager (two counts read back per step)"}
+                "note": "interact of the initial state, eager (three counts read back per step)"}
     rb = 4 if args.precision == "fp32" else 8
     N = sc.mesh.n_cells
     kernels = {"k_fields": {"ms": ph[2], "bytes": (5 * rb + 4) * N + 3 * rb * nc},
