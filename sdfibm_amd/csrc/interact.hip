@@ -768,7 +768,7 @@ __device__ inline void cell_force(const MeshDev& m, const SolidK& K, int gi, int
                                   R* code, double dtINV, double* acc) {
     double alpha = 1.0;
     if (cut) {
-        const R a = cell_fraction<R, MK>(m, K, gi, gj, gk);
+        const R a = cell_fraction<R, MK, false>(m, K, gi, gj, gk);  // finite solids only
         const bool cin = inside_world(K, cell_centre<MK>(m, gi, gj, gk));
         *code = cin ? a : -a;
         alpha = (double)a;
@@ -783,8 +783,10 @@ __device__ inline void cell_force(const MeshDev& m, const SolidK& K, int gi, int
 // sg_hi] while the centre gradient and the corner-difference gradient differ by
 // more than the angle whose cosine is sg_cos; then, and for every other cell,
 // the exact box fraction.
+// The plain FP32 rectilinear pass fits 96 VGPRs without spills: five waves per SIMD.
 template <class R, int MK, bool SG, int NT>
-__global__ void __launch_bounds__(NT) k_cells(MeshDev m, int ns, const SolidK* __restrict__ Ks,
+__global__ void __launch_bounds__(NT)
+    __attribute__((amdgpu_waves_per_eu(!SG && MK == MK_RECT && sizeof(R) == 4 ? 5 : 1))) k_cells(MeshDev m, int ns, const SolidK* __restrict__ Ks,
                                                   const TileInfo* __restrict__ Ts, int* __restrict__ qmeta,
                                                   const int* __restrict__ qlist, const int* __restrict__ cand,
                                                   const int2* __restrict__ ncand, const R* __restrict__ Ux,
@@ -847,6 +849,7 @@ __global__ void __launch_bounds__(NT) k_cells(MeshDev m, int ns, const SolidK* _
             float ihx = 0.f, ihy = 0.f, ihz = 0.f;
             if (cut) {
                 box_corner_phi(m, K, gi, gj, gk, h, lc);
+                if (K.shape == SDFIBM_SHAPE_SPHERE) sphere_corner_phi(m, K, gi, gj, gk, h);  // as cell_fraction
                 ihx = (float)(1.0 / (m.xs[gi + 1] - m.xs[gi]));
                 ihy = (float)(1.0 / (m.ys[gj + 1] - m.ys[gj]));
                 ihz = (float)(1.0 / (m.zs[gk + 1] - m.zs[gk]));

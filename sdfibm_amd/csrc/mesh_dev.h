@@ -187,46 +187,97 @@ __device__ inline R fraction_of_cell(const MeshDev& m, const SolidK& K, int i, i
 }
 
 // Corner signed distances of box cell (i,j,k) of a rectilinear mesh in FP32
-// (the local coordinates of the corners share the products of sdf_fast: the
-// same values, bit for bit) and the local coordinates of the cell centre.
+// and the local coordinates of the cell centre.  Corner 0's local coordinates
+// rt (p - c) in FP64, the others by FP64 steps along the cell's edges (the
+// rotated edge vectors) in a Gray-code walk 0 1 3 2 6 7 5 4, one step per
+// corner: rounding of order 1e-16 of |p - c|, far below the FP32 distances,
+// with 12 FP64 values live instead of the 18 products of all corners.
 __device__ inline void box_corner_phi(const MeshDev& m, const SolidK& K, int i, int j, int k, Phi8<float>& h, d3& lc) {
-    const double dx[2] = {m.xs[i] - K.c[0], m.xs[i + 1] - K.c[0]};
-    const double dy[2] = {m.ys[j] - K.c[1], m.ys[j + 1] - K.c[1]};
-    const double dz[2] = {m.zs[k] - K.c[2], m.zs[k + 1] - K.c[2]};
-    double px[3][2], py[3][2], pz[3][2];
+    const double dx = m.xs[i] - K.c[0], dy = m.ys[j] - K.c[1], dz = m.zs[k] - K.c[2];
+    const double hx = m.xs[i + 1] - m.xs[i], hy = m.ys[j + 1] - m.ys[j], hz = m.zs[k + 1] - m.zs[k];
+    d3 l{K.rt[0] * dx + K.rt[1] * dy + K.rt[2] * dz, K.rt[3] * dx + K.rt[4] * dy + K.rt[5] * dz,
+         K.rt[6] * dx + K.rt[7] * dy + K.rt[8] * dz};
+    const d3 ex{K.rt[0] * hx, K.rt[3] * hx, K.rt[6] * hx}, ey{K.rt[1] * hy, K.rt[4] * hy, K.rt[7] * hy},
+        ez{K.rt[2] * hz, K.rt[5] * hz, K.rt[8] * hz};
+    const d3 l0 = l;
+    h.f0 = sdf_local_mixed(K, l);
+    l = l + ex;
+    h.f1 = sdf_local_mixed(K, l);
+    l = l + ey;
+    h.f3 = sdf_local_mixed(K, l);
+    l = l - ex;
+    h.f2 = sdf_local_mixed(K, l);
+    l = l + ez;
+    h.f6 = sdf_local_mixed(K, l);
+    l = l + ex;
+    h.f7 = sdf_local_mixed(K, l);
+    lc = 0.5 * (l0 + l);
+    l = l - ey;
+    h.f5 = sdf_local_mixed(K, l);
+    l = l - ex;
+    h.f4 = sdf_local_mixed(K, l);
+}
+
+// Corner signed distances of box cell (i,j,k) for a sphere in FP32: its
+// distance is invariant under the solid's rotation, |rt (p - c) + com| = |p - cw|
+// with cw = c - R com its world centre (K.k[2..4]), so the 8 squared distances
+// are sums of per-axis squares (6 products, 16 sums).
+__device__ inline void sphere_corner_phi(const MeshDev& m, const SolidK& K, int i, int j, int k, Phi8<float>& h) {
+    double sx[2], sy[2], sz[2];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            px[r][b] = K.rt[3 * r + 0] * dx[b];
-            py[r][b] = K.rt[3 * r + 1] * dy[b];
-            pz[r][b] = K.rt[3 * r + 2] * dz[b];
-        }
-    auto loc = [&](int bx, int by, int bz) {
-        return d3{px[0][bx] + py[0][by] + pz[0][bz], px[1][bx] + py[1][by] + pz[1][bz],
-                  px[2][bx] + py[2][by] + pz[2][bz]};
+    for (int b = 0; b < 2; ++b) {
+        const double dx = m.xs[i + b] - K.k[2], dy = m.ys[j + b] - K.k[3], dz = m.zs[k + b] - K.k[4];
+        sx[b] = dx * dx;
+        sy[b] = dy * dy;
+        sz[b] = dz * dz;
+    }
+    const double r = K.k[0], r2 = K.k[1];
+    auto f = [&](int bx, int by, int bz) { return filter_sd<float>(sphere_sd_mixed(sx[bx] + sy[by] + sz[bz], r, r2)); };
+    h.f0 = f(0, 0, 0);
+    h.f1 = f(1, 0, 0);
+    h.f2 = f(0, 1, 0);
+    h.f3 = f(1, 1, 0);
+    h.f4 = f(0, 0, 1);
+    h.f5 = f(1, 0, 1);
+    h.f6 = f(0, 1, 1);
+    h.f7 = f(1, 1, 1);
+}
+
+// Corner signed distances of box cell (i,j,k) for a plane in FP32: the local y of
+// each corner by the quaternion transform, as sdf_fast forms it.  A plane's
+// distance is not filtered, so for a wall lying in a mesh face the rounding-level
+// distances of that face's vertices decide the cut; they keep the reference's
+// operation order.
+__device__ inline void plane_corner_phi(const MeshDev& m, const SolidK& K, int i, int j, int k, Phi8<float>& h) {
+    auto f = [&](int bx, int by, int bz) {
+        return (float)world_to_local(K, d3{m.xs[i + bx], m.ys[j + by], m.zs[k + bz]}).y;
     };
-    const d3 l0 = loc(0, 0, 0), l7 = loc(1, 1, 1);
-    h.f0 = sdf_local_mixed(K, l0);
-    h.f1 = sdf_local_mixed(K, loc(1, 0, 0));
-    h.f2 = sdf_local_mixed(K, loc(0, 1, 0));
-    h.f3 = sdf_local_mixed(K, loc(1, 1, 0));
-    h.f4 = sdf_local_mixed(K, loc(0, 0, 1));
-    h.f5 = sdf_local_mixed(K, loc(1, 0, 1));
-    h.f6 = sdf_local_mixed(K, loc(0, 1, 1));
-    h.f7 = sdf_local_mixed(K, l7);
-    lc = 0.5 * (l0 + l7);
+    h.f0 = f(0, 0, 0);
+    h.f1 = f(1, 0, 0);
+    h.f2 = f(0, 1, 0);
+    h.f3 = f(1, 1, 0);
+    h.f4 = f(0, 0, 1);
+    h.f5 = f(1, 0, 1);
+    h.f6 = f(0, 1, 1);
+    h.f7 = f(1, 1, 1);
 }
 
 // Volume fraction of cell (i,j,k) inside solid K as the cut-cell kernels form
-// it: in FP32 on a rectilinear mesh the box form; otherwise the general hex
-// form (FP64 keeps the reference's operation order).
-template <class R, int MK>
+// it: in FP32 on a rectilinear mesh the box form (spheres without the
+// rotation, planes by the quaternion transform); otherwise the general hex
+// form (FP64 keeps the reference's operation order).  PL = false for the
+// kernels of finite solids, which never see a plane.
+template <class R, int MK, bool PL = true>
 __device__ inline R cell_fraction(const MeshDev& m, const SolidK& K, int i, int j, int k) {
     if constexpr (MK == MK_RECT && sizeof(R) == sizeof(float)) {
         Phi8<float> h;
         d3 lc;
-        box_corner_phi(m, K, i, j, k, h, lc);
+        if (K.shape == SDFIBM_SHAPE_SPHERE)
+            sphere_corner_phi(m, K, i, j, k, h);
+        else if (PL && K.shape == SDFIBM_SHAPE_PLANE)
+            plane_corner_phi(m, K, i, j, k, h);
+        else
+            box_corner_phi(m, K, i, j, k, h, lc);
         const double z0 = m.zs[k], z1 = m.zs[k + 1];
         const float az = (float)((0.0 - z0) / (z1 - z0));
         return box_fraction<float>(h, i == 0, j == 0, k == 0, m.two_d != 0, az);

@@ -116,7 +116,7 @@ __host__ __device__ inline SolidK make_solid_k(const sdfibm_solid& s, const CsgK
     case SDFIBM_SHAPE_SPHERE:
     case SDFIBM_SHAPE_CIRCLE:
         K.k[0] = p[0];
-        K.k[1] = p[0] * p[0];
+        K.k[1] = p[0] * p[0];  // spheres: k[2..4] = the world centre (set below, after rt)
         K.com[0] = p[1];
         K.com[1] = p[2];
         K.com[2] = p[3];
@@ -173,6 +173,9 @@ __host__ __device__ inline SolidK make_solid_k(const sdfibm_solid& s, const CsgK
         K.rt[7] = 2 * (y * z - w * x);
         K.rt[8] = 1 - 2 * (x * x + y * y);
     }
+    if (s.shape == SDFIBM_SHAPE_SPHERE)  // rt (p - c) + com = 0  <=>  p = c - R com
+        for (int d = 0; d < 3; ++d)
+            K.k[2 + d] = K.c[d] - (K.rt[d] * K.com[0] + K.rt[3 + d] * K.com[1] + K.rt[6 + d] * K.com[2]);
     for (int a = 0; a < 3; ++a) {
         const d3 col = rotate_to_local(K, d3{a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0});
         K.qc[3 * a + 0] = col.x;
@@ -478,12 +481,16 @@ __host__ __device__ inline R sdf_local(const SolidK& K, R px, R py, R pz) {
 // divides of well-conditioned quantities run in FP32, on the hardware's 1-ulp
 // square root, reciprocal and reciprocal square root.  Same function as the
 // reference formula up to rounding.
+// A sphere's distance from its squared centre distance d2 (radius r, r2 = r^2):
+// (d2 - r2) / (sqrt(d2) + r), the difference formed in FP64.
+__device__ inline float sphere_sd_mixed(double d2, double r, double r2) {
+    const float num = (float)(d2 - r2);
+    return num * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf((float)d2) + (float)r);
+}
+
 __device__ inline float sdf_local_mixed(const SolidK& K, d3 p) {
     const double cx = K.com[0], cy = K.com[1], cz = K.com[2];
-    auto sph = [](double d2, double r, double r2) -> float {
-        const float num = (float)(d2 - r2);
-        return num * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf((float)d2) + (float)r);
-    };
+    auto sph = [](double d2, double r, double r2) -> float { return sphere_sd_mixed(d2, r, r2); };
     auto rect = [](double x, double y, double ra, double rb) -> float {
         const float dx = (float)(fabs(x) - ra), dy = (float)(fabs(y) - rb);
         const float dxp = fmaxf(0.f, dx), dyp = fmaxf(0.f, dy);
