@@ -126,22 +126,16 @@ __device__ inline R face_fraction(const Hex8<R>& h, R sf_mag, R l01, R l12, R l2
     return area / sf_mag;
 }
 
-// Face data of one hex cell: centres (relative to vertex 0) and area vectors in
-// the order -x, +x, -y, +y, -z, +z, and whether each minus face is on the
-// boundary (its vertex list then starts/winds differently: an internal face
-// takes its owner's -- the lower-index neighbour's -- listing).
-template <class R>
-struct HexFaces {
-    r3<R> fc[6];
-    r3<R> sf[6];
-    bool bnd[3];
-};
-
 // Solid volume fraction of a hex cell (geometrictools.cpp:47-72): sum over faces
 // of (1/3) eps_f |(apex - fc).Sf|, divided by V.  two_d: the apex z is forced to
-// apex_z_rel (world z = 0, geometrictools.cpp:57-58).
-template <class R>
-__device__ inline R hex_fraction(const Hex8<R>& h, const HexFaces<R>& F, R V, bool two_d, R apex_z_rel) {
+// apex_z_rel (world z = 0, geometrictools.cpp:57-58).  face(slot, fc, sf) gives
+// the centre (relative to vertex 0) and area vector of face slot in the order
+// -x, +x, -y, +y, -z, +z; bnd[a]: the minus face along axis a is on the boundary
+// (its vertex list then starts/winds differently: an internal face takes its
+// owner's -- the lower-index neighbour's -- listing).  Each face is fetched,
+// used and dropped in turn, so no more than one face's geometry is live.
+template <class R, class FaceAt>
+__device__ inline R hex_fraction(const Hex8<R>& h, FaceAt face, const bool* bnd, R V, bool two_d, R apex_z_rel) {
     // cell apex: A = vertex 0, B = first later vertex with a sign change, else vertex 7
     // B is blended as sum_t w_t P_t with exactly one weight 1 and the rest 0, which is
     // exact and keeps the vertices in registers (a select chain is lowered to an
@@ -173,20 +167,27 @@ __device__ inline R hex_fraction(const Hex8<R>& h, const HexFaces<R>& F, R V, bo
             y46 = line_fraction<R>(h.f4, h.f6), y57 = line_fraction<R>(h.f5, h.f7);
     const R z04 = line_fraction<R>(h.f0, h.f4), z15 = line_fraction<R>(h.f1, h.f5),
             z26 = line_fraction<R>(h.f2, h.f6), z37 = line_fraction<R>(h.f3, h.f7);
-    R eps[6];
-    eps[0] = F.bnd[0] ? face_fraction<R, 0, 4, 6, 2>(h, rmag(F.sf[0]), z04, y46, z26, y02)
-                      : face_fraction<R, 0, 2, 6, 4>(h, rmag(F.sf[0]), y02, z26, y46, z04);
-    eps[1] = face_fraction<R, 1, 3, 7, 5>(h, rmag(F.sf[1]), y13, z37, y57, z15);
-    eps[2] = F.bnd[1] ? face_fraction<R, 0, 1, 5, 4>(h, rmag(F.sf[2]), x01, z15, x45, z04)
-                      : face_fraction<R, 0, 4, 5, 1>(h, rmag(F.sf[2]), z04, x45, z15, x01);
-    eps[3] = face_fraction<R, 2, 6, 7, 3>(h, rmag(F.sf[3]), z26, x67, z37, x23);
-    eps[4] = F.bnd[2] ? face_fraction<R, 0, 2, 3, 1>(h, rmag(F.sf[4]), y02, x23, y13, x01)
-                      : face_fraction<R, 0, 1, 3, 2>(h, rmag(F.sf[4]), x01, y13, x23, y02);
-    eps[5] = face_fraction<R, 4, 5, 7, 6>(h, rmag(F.sf[5]), x45, y57, x67, y46);
     R vol = R(0);
-#pragma unroll
-    for (int f = 0; f < 6; ++f)
-        if (eps[f] != R(0)) vol += (R(1) / R(3)) * eps[f] * fabs(rdot(sub(ap, F.fc[f]), F.sf[f]));
+    // one face: its fraction eps (sf_mag = |Sf|) and, where eps != 0, its pyramid term
+    auto add = [&](const r3<R>& fc, const r3<R>& sf, R eps) {
+        if (eps != R(0)) vol += (R(1) / R(3)) * eps * fabs(rdot(sub(ap, fc), sf));
+    };
+    r3<R> fc, sf;
+    face(0, fc, sf);
+    add(fc, sf, bnd[0] ? face_fraction<R, 0, 4, 6, 2>(h, rmag(sf), z04, y46, z26, y02)
+                       : face_fraction<R, 0, 2, 6, 4>(h, rmag(sf), y02, z26, y46, z04));
+    face(1, fc, sf);
+    add(fc, sf, face_fraction<R, 1, 3, 7, 5>(h, rmag(sf), y13, z37, y57, z15));
+    face(2, fc, sf);
+    add(fc, sf, bnd[1] ? face_fraction<R, 0, 1, 5, 4>(h, rmag(sf), x01, z15, x45, z04)
+                       : face_fraction<R, 0, 4, 5, 1>(h, rmag(sf), z04, x45, z15, x01));
+    face(3, fc, sf);
+    add(fc, sf, face_fraction<R, 2, 6, 7, 3>(h, rmag(sf), z26, x67, z37, x23));
+    face(4, fc, sf);
+    add(fc, sf, bnd[2] ? face_fraction<R, 0, 2, 3, 1>(h, rmag(sf), y02, x23, y13, x01)
+                       : face_fraction<R, 0, 1, 3, 2>(h, rmag(sf), x01, y13, x23, y02));
+    face(5, fc, sf);
+    add(fc, sf, face_fraction<R, 4, 5, 7, 6>(h, rmag(sf), x45, y57, x67, y46));
     return vol / V;
 }
 

@@ -93,11 +93,12 @@ __device__ inline void cell_vertices(const MeshDev& m, int i, int j, int k, d3* 
     for (int v = 0; v < 8; ++v) P[v] = vertex_pos<MK>(m, i + (v & 1), j + ((v >> 1) & 1), k + ((v >> 2) & 1));
 }
 
-// Face centres (relative to v0) and area vectors of cell (i,j,k), order
+// Centre (relative to v0) and area vector of face slot of cell (i,j,k), slots
 // -x, +x, -y, +y, -z, +z.
 template <class R, int MK>
-__device__ inline void hex_faces(const MeshDev& m, int i, int j, int k, d3 v0, HexFaces<R>& F) {
-    auto put = [&](int slot, int fam, int a, int b, int c) {
+__device__ inline void hex_face(const MeshDev& m, int i, int j, int k, d3 v0, int slot, r3<R>& fco, r3<R>& sfo) {
+    const int fam = slot >> 1, a = i + (slot == 1), b = j + (slot == 3), c = k + (slot == 5);
+    {
         d3 fc, sf;
         if constexpr (MK == MK_RECT) {
             if (fam == 0) {
@@ -123,18 +124,9 @@ __device__ inline void hex_faces(const MeshDev& m, int i, int j, int k, d3 v0, H
             fc = {pc[0], pc[1], pc[2]};
             sf = {ps[0], ps[1], ps[2]};
         }
-        F.fc[slot] = {R(fc.x - v0.x), R(fc.y - v0.y), R(fc.z - v0.z)};
-        F.sf[slot] = {R(sf.x), R(sf.y), R(sf.z)};
-    };
-    put(0, 0, i, j, k);
-    put(1, 0, i + 1, j, k);
-    put(2, 1, i, j, k);
-    put(3, 1, i, j + 1, k);
-    put(4, 2, i, j, k);
-    put(5, 2, i, j, k + 1);
-    F.bnd[0] = i == 0;
-    F.bnd[1] = j == 0;
-    F.bnd[2] = k == 0;
+        fco = {R(fc.x - v0.x), R(fc.y - v0.y), R(fc.z - v0.z)};
+        sfo = {R(sf.x), R(sf.y), R(sf.z)};
+    }
 }
 
 // Signed distance at world point p for the fraction: local = R^T (p - c) with
@@ -174,10 +166,10 @@ __device__ inline R fraction_with_phi(const MeshDev& m, int i, int j, int k, Phi
     vtx(1, 0, 1, h.p5, h.f5);
     vtx(0, 1, 1, h.p6, h.f6);
     vtx(1, 1, 1, h.p7, h.f7);
-    HexFaces<R> F;
-    hex_faces<R, MK>(m, i, j, k, w0, F);
+    const bool bnd[3] = {i == 0, j == 0, k == 0};
     const R V = R(cell_volume<MK>(m, i, j, k));
-    return hex_fraction<R>(h, F, V, m.two_d != 0, R(0.0 - w0.z));
+    return hex_fraction<R>(h, [&](int slot, r3<R>& fc, r3<R>& sf) { hex_face<R, MK>(m, i, j, k, w0, slot, fc, sf); },
+                           bnd, V, m.two_d != 0, R(0.0 - w0.z));
 }
 
 // Volume fraction of cell (i,j,k) inside solid K, corner distances evaluated here.
