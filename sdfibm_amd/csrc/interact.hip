@@ -161,7 +161,18 @@ __host__ __device__ inline int tile_queue(int cls) {
 
 // The tiles' code and list offsets: one atomic per wave on the step's running
 // total (a bump allocation; the layout does not depend on the order, every tile
-// is addressed through its own offset), the wave's tiles in lane order.
+// is addressed through its own offset), the wave's tiles in lane order.  On a
+// rectilinear mesh of up to kSetupStage vertex coordinates the three coordinate
+// arrays are staged in LDS first (dynamic LDS of setup_lds_bytes): each lane's
+// tile bounds and nearest cell are searches whose probes depend on each other,
+// served from LDS instead of memory.
+constexpr int kSetupStage = 3 * 1025;
+
+__host__ inline size_t setup_lds_bytes(const MeshDev& m, int mk) {
+    const long long n = (long long)m.nx + m.ny + m.nz + 3;
+    return mk == MK_RECT && n <= kSetupStage ? (size_t)n * sizeof(double) : 0;
+}
+
 template <int MK>
 __global__ void __launch_bounds__(64) k_setup(const sdfibm_solid* __restrict__ in, int ns, MeshDev m, SolidK* __restrict__ K,
                         TileInfo* __restrict__ T, long long* __restrict__ total, long long code_cap,
@@ -171,16 +182,30 @@ __global__ void __launch_bounds__(64) k_setup(const sdfibm_solid* __restrict__ i
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool live = s < ns;  // the whole wave takes part in the offsets' scan
+    MeshDev mm = m;
+    if constexpr (MK == MK_RECT) {
+        extern __shared__ double sc[];
+        const int n0 = m.nx + 1, n1 = m.ny + 1, n2 = m.nz + 1;
+        if (n0 + n1 + n2 <= kSetupStage) {  // the same for the whole workgroup
+            for (int v = threadIdx.x; v < n0; v += blockDim.x) sc[v] = m.xs[v];
+            for (int v = threadIdx.x; v < n1; v += blockDim.x) sc[n0 + v] = m.ys[v];
+            for (int v = threadIdx.x; v < n2; v += blockDim.x) sc[n0 + n1 + v] = m.zs[v];
+            __syncthreads();
+            mm.xs = sc;
+            mm.ys = sc + n0;
+            mm.zs = sc + n0 + n1;
+        }
+    }
     TileInfo t{};
     t.cls = TC_EMPTY;
     if (live) {
         const SolidK k = make_solid_k(in[s], csg);
         K[s] = k;
-        t = make_tile<MK>(m, k);
+        t = make_tile<MK>(mm, k);
         // solids set_solids put on the unbounded path stay there (their tile box is the cull)
         if (unb[s] && t.cls != TC_PLANE) t.cls = t.cls == TC_EMPTY ? TC_EMPTY : TC_HUGE;
         // structured meshes: the seed search (a box of cell centres) is spread over a wave by k_seed
-        t.seed = t.cls == TC_EMPTY ? -1 : (MK == MK_RECT ? nearest_cell<MK>(m, k.c) : -2);
+        t.seed = t.cls == TC_EMPTY ? -1 : (MK == MK_RECT ? nearest_cell<MK>(mm, k.c) : -2);
         // a tile beyond 2^22 vertices, or a big tile when set_solids found none possible (the
         // big-tile pass is not launched): interact classifies the solids again
         if (t.cls == TC_TOOBIG || (t.cls == TC_BIG && !big_on)) atomicOr(err, EF_TILE);
@@ -194,28 +219,29 @@ __global__ void __launch_bounds__(64) k_setup(const sdfibm_solid* __restrict__ i
         if (lane >= d) x += y;
     }
     const long long wtot = __shfl(x, 63, 64);
-    long long base = 0;
-    if (lane == 0 && wtot > 0) {
-        base = (long long)atomicAdd((unsigned long long*)total, (unsigned long long)wtot);
-        if (base + wtot > code_cap) atomicOr(err, EF_CODECAP);
+    // the solid's queue (the lanes of a wave are consecutive solids, queued in index order)
+    int q = live ? tile_queue(t.cls) : -1;
+    if (q == 0 && (t.n[1] + 1) * (t.n[2] + 1) <= kRowsTiny) q = 3;  // tiny tiles: a wave each
+    unsigned long long bq = 0;  // lane 1 + qq: the wave's members of queue qq
+#pragma unroll
+    for (int qq = 0; qq < kQueues; ++qq) {
+        const unsigned long long b = __ballot(q == qq);
+        if (lane == 1 + qq) bq = b;
     }
-    base = __shfl(base, 0, 64);
+    // the wave's atomics in one round: lane 0 the tiles' running total, lane 1 + qq queue qq
+    long long got = 0;
+    if (lane == 0 && wtot > 0)
+        got = (long long)atomicAdd((unsigned long long*)total, (unsigned long long)wtot);
+    else if (lane >= 1 && lane <= kQueues && bq)
+        got = atomicAdd(&qmeta[lane - 1], __popcll(bq));
+    const long long base = __shfl(got, 0, 64);
+    const int qbase = (int)__shfl(got, q >= 0 ? 1 + q : 0, 64);
+    const unsigned long long qmask = __shfl(bq, q >= 0 ? 1 + q : 0, 64);
+    if (lane == 0 && wtot > 0 && base + wtot > code_cap) atomicOr(err, EF_CODECAP);
     if (!live) return;
     t.off = base + x - tc;
     T[s] = t;
-    // queue the solid: one atomic per wave and queue (the lanes of a wave are
-    // consecutive solids, queued in index order)
-    int q = tile_queue(t.cls);
-    if (q == 0 && (t.n[1] + 1) * (t.n[2] + 1) <= kRowsTiny) q = 3;  // tiny tiles: a wave each
-    for (int qq = 0; qq < kQueues; ++qq) {
-        const unsigned long long b = __ballot(q == qq);
-        if (!b) continue;
-        const int leader = __builtin_ctzll(b);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&qmeta[qq], __popcll(b));
-        base = __shfl(base, leader, 64);
-        if (q == qq) qlist[(long long)qq * ns + base + __popcll(b & ((1ull << lane) - 1ull))] = s;
-    }
+    if (q >= 0) qlist[(long long)q * ns + qbase + __popcll(qmask & ((1ull << lane) - 1ull))] = s;
 }
 
 // The seed cell of every non-empty solid on a structured mesh (nearest_cell's
@@ -1987,7 +2013,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     k_reset<<<grid_for(std::max(c->nbricks, 8LL), 256), 256, 0, st>>>(c->err, c->qmeta, c->bcount, c->nbricks,
                                                                      c->d_tile_total);
     if (ns > 0) {
-        k_setup<MK><<<grid_for(ns, 64), 64, 0, st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_total,
+        k_setup<MK><<<grid_for(ns, 64), 64, setup_lds_bytes(c->m, MK), st>>>(c->d_solids, ns, c->m, c->d_K, c->d_T, c->d_tile_total,
                                                        c->code_cap, c->d_ft, c->qmeta, c->qlist, c->d_unb, c->err,
                                                        c->d_csg, c->big_on ? 1 : 0);
         if (MK != MK_RECT) k_seed<<<grid_for(ns, 4), 256, 0, st>>>(c->m, ns, c->d_K, c->d_T);
