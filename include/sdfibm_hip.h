@@ -375,8 +375,17 @@ int sdfibm_pair_counts(sdfibm_ctx* ctx, int64_t* out);
  * written (<= n, at most 2).  No counterpart in the reference. */
 int sdfibm_buffer_capacity(sdfibm_ctx* ctx, int64_t* out, int n);
 
+/* Phase timing of the structured-mesh interact on (on != 0) or off (the
+ * default).  On, every eager interact records HIP events between its phases for
+ * sdfibm_last_timing; each record orders the kernels around it, which costs about
+ * 25 us per step on a 256^3 / 10,000-solid scene, so it is off unless asked for.
+ * sdfibm_autotune turns it on for its own timed steps.  No counterpart in the
+ * reference. */
+int sdfibm_set_phase_timing(sdfibm_ctx* ctx, int32_t on);
+
 /* Timing of the phases of the last interact (milliseconds, HIP events on the
- * context's stream): [0] setup+binning, [1] per-solid tiles (classification,
+ * context's stream; structured meshes: only with sdfibm_set_phase_timing on,
+ * zeros otherwise): [0] setup+binning, [1] per-solid tiles (classification,
  * cut cells and forcing, plane labelling, and the wait for the big-tile pass),
  * [2] field pass, [3] total, [4] tile classification alone, [5] cut cells +
  * forcing of the tiles of the on-chip tile pass, [6] the big-tile pass (on the

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libsdfibm_hip.so")
 EXPORTS = (
     "sdfibm_create", "sdfibm_destroy", "sdfibm_set_solids", "sdfibm_set_velocity",
     "sdfibm_interact", "sdfibm_get_fields", "sdfibm_field_pointers", "sdfibm_candidates",
-    "sdfibm_last_timing", "sdfibm_set_stream", "sdfibm_last_error", "sdfibm_status_string",
+    "sdfibm_last_timing", "sdfibm_set_phase_timing", "sdfibm_set_stream", "sdfibm_last_error", "sdfibm_status_string",
     "sdfibm_abi_version", "sdfibm_update_velocity", "sdfibm_fix_internal", "sdfibm_get_velocity",
     "sdfibm_set_dynamics", "sdfibm_evolve", "sdfibm_get_solids", "sdfibm_set_contact", "sdfibm_run_steps", "sdfibm_pair_counts",
     "sdfibm_surrogate_train", "sdfibm_surrogate_samples", "sdfibm_surrogate_eval", "sdfibm_surrogate_grad",
@@ -85,6 +85,7 @@ def load():
         "sdfibm_field_pointers": (i32, [P, P, P, P, P, P]),
         "sdfibm_candidates": (i32, [P, i64, P, P, P, i64]),
         "sdfibm_last_timing": (i32, [P, P, i32]),
+        "sdfibm_set_phase_timing": (i32, [P, i32]),
         "sdfibm_set_stream": (i32, [P, P]),
         "sdfibm_last_error": (C.c_char_p, [P]),
         "sdfibm_status_string": (C.c_char_p, [i32]),

@@ -273,10 +273,16 @@ class SolidCloud:
         _lib.load().sdfibm_buffer_capacity(self._h, _p(out), 2)
         return int(out[0]), int(out[1])
 
+    def set_phase_timing(self, on: bool = True) -> None:
+        """HIP events between the phases of each interact (off by default: each record orders
+        the kernels around it); timing() reads them."""
+        _lib.check(_lib.load().sdfibm_set_phase_timing(self._h, 1 if on else 0), self._h)
+
     def timing(self) -> np.ndarray:
         """Phase times of the last interact in ms: setup+binning, tile pass, field pass, total,
         tile classification, cut cells + forcing (+ plane labelling), big-tile pass (second
-        stream)."""
+        stream).  Structured meshes: zeros unless set_phase_timing(True); the replays'
+        device time (index 3) after run_steps and interact_steps either way."""
         ms = np.zeros(7)
         _lib.load().sdfibm_last_timing(self._h, _p(ms), 7)
         return ms

@@ -1789,6 +1789,7 @@ struct sdfibm_ctx {
     bool big_on = false;
     BigMeta* d_big_meta = nullptr;  // kBigMaxSolids
     long long* d_big_tot = nullptr;  // kBigTotN
+    bool phase_timing = false;  // HIP events between the step's phases (sdfibm_set_phase_timing)
     int lm_grid = 0;  // workgroups of the cut-cell pass's 256-thread part (solids whose tile can be medium or big)
     // general polyhedral meshes (SDFIBM_MESH_POLY): the mesh, per-solid, per-visit, per-pair
     // and per-cell buffers of the step
@@ -2007,9 +2008,12 @@ void launch_big(sdfibm_ctx* c, double dt, hipStream_t st) {
 // bitmaps of the last eager pass are used, the planes being fixed.
 template <class R, int MK>
 int launch_step(sdfibm_ctx* c, double dt, bool graph) {
+    // timing events only on request: each record orders the kernels around it (about 25 us
+    // per step on the cloud3d workload)
+    const bool ev_on = !graph && c->phase_timing;
     const int ns = (int)c->ns;
     hipStream_t st = c->stream;
-    if (!graph) (void)hipEventRecord(c->ev[0], st);
+    if (ev_on) (void)hipEventRecord(c->ev[0], st);
     k_reset<<<grid_for(std::max(c->nbricks, 8LL), 256), 256, 0, st>>>(c->err, c->qmeta, c->bcount, c->nbricks,
                                                                      c->d_tile_total);
     if (ns > 0) {
@@ -2025,7 +2029,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
         else
             k_bin<kBX3, kBY3, kBZ3><<<gb, 256, 0, st>>>(ns, c->d_T, c->nbx, c->nby, c->bcount, c->d_blist, c->err);
     }
-    if (!graph) (void)hipEventRecord(c->ev[1], st);
+    if (ev_on) (void)hipEventRecord(c->ev[1], st);
     R* codes = (R*)c->d_codes;
     if (ns > 0) {
         // the class sizes live on the device: the small-tile grid covers every solid (one
@@ -2040,9 +2044,9 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
         // the cut-cell pass needs these tiles, the field pass also the big ones
         if (hipEventRecord(c->joinc, c->aux) != hipSuccess) return -1;
         if (c->big_on) {
-            if (!graph) (void)hipEventRecord(c->ev[5], c->aux);
+            if (ev_on) (void)hipEventRecord(c->ev[5], c->aux);
             launch_big<R, MK>(c, dt, c->aux);
-            if (!graph) (void)hipEventRecord(c->ev[6], c->aux);
+            if (ev_on) (void)hipEventRecord(c->ev[6], c->aux);
         }
         if (hipEventRecord(c->join, c->aux) != hipSuccess) return -1;
         k_classify<R, MK, kRowsSmall, 0, kBlock><<<g, kBlock, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
@@ -2050,7 +2054,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
         k_classify<R, MK, kRowsTiny, 3, 64><<<g, 64, 0, st>>>(c->m, ns, c->d_K, c->d_T, c->qmeta, c->qlist, codes,
                                                               c->d_cand, c->code_cap, c->d_ncand);
         if (hipStreamWaitEvent(st, c->joinc, 0) != hipSuccess) return -1;
-        if (!graph) (void)hipEventRecord(c->ev[4], st);
+        if (ev_on) (void)hipEventRecord(c->ev[4], st);
         constexpr bool kSgOk = MK == MK_RECT && sizeof(R) == sizeof(float);
         const bool sg = kSgOk && c->d_sg != nullptr;
         // same workgroup sizes with and without the surrogate: identical reduction order when
@@ -2081,10 +2085,10 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
             }
         }
 #undef SDFIBM_CELLS
-        if (!graph) (void)hipEventRecord(c->ev[7], st);
+        if (ev_on) (void)hipEventRecord(c->ev[7], st);
     } else {
-        if (!graph) (void)hipEventRecord(c->ev[4], st);
-        if (!graph) (void)hipEventRecord(c->ev[7], st);
+        if (ev_on) (void)hipEventRecord(c->ev[4], st);
+        if (ev_on) (void)hipEventRecord(c->ev[7], st);
     }
     const int np = (int)c->h_planes.size();
     unsigned dirty = 0;
@@ -2134,7 +2138,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
                                                 (R*)c->d_pcode + (long long)p * c->N);
     }
     if (ns > 0 && hipStreamWaitEvent(st, c->join, 0) != hipSuccess) return -1;  // the big-tile pass
-    if (!graph) (void)hipEventRecord(c->ev[2], st);
+    if (ev_on) (void)hipEventRecord(c->ev[2], st);
 #define SDFIBM_FIELDS(BX_, BY_, BZ_, CPT_, HP_)                                                                 \
     k_fields<R, MK, BX_, BY_, BZ_, CPT_, HP_><<<dim3(c->nbx, c->nby, c->nbz), BX_ * BY_ * BZ_ / CPT_, 0, st>>>(     \
         c->m, c->nbx, c->nby, c->bcount, c->d_blist, c->d_K, c->d_T, codes, c->code_cap, c->d_planes, np,          \
@@ -2155,7 +2159,7 @@ int launch_step(sdfibm_ctx* c, double dt, bool graph) {
     }
 #undef SDFIBM_FIELDS
     if (np > 0) k_plane_reduce<<<np, 1024, 0, st>>>(c->d_plane_partial, c->nbricks, c->d_planes, c->rho_f, c->d_ft);
-    if (!graph) (void)hipEventRecord(c->ev[3], st);
+    if (ev_on) (void)hipEventRecord(c->ev[3], st);
     return 0;
 }
 
@@ -3583,11 +3587,13 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     if (ft && c->ns > 0) HIPCHK(c, hipMemcpyAsync(ft, c->d_ft, c->ns * 48, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms[5] = {0, 0, 0, 0, 0};
-    (void)hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]);
-    (void)hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]);
-    (void)hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]);
-    (void)hipEventElapsedTime(&ms[3], c->ev[1], c->ev[4]);
-    (void)hipEventElapsedTime(&ms[4], c->ev[4], c->ev[7]);
+    if (c->phase_timing) {
+        (void)hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]);
+        (void)hipEventElapsedTime(&ms[3], c->ev[1], c->ev[4]);
+        (void)hipEventElapsedTime(&ms[4], c->ev[4], c->ev[7]);
+    }
     c->timing[0] = ms[0];
     c->timing[1] = ms[1];
     c->timing[2] = ms[2];
@@ -3595,7 +3601,7 @@ int sdfibm_interact(sdfibm_ctx* c, double time, double dt, double* ft) {
     c->timing[4] = ms[3];
     c->timing[5] = ms[4];
     float msb = 0.f;
-    if (c->big_on) (void)hipEventElapsedTime(&msb, c->ev[5], c->ev[6]);
+    if (c->big_on && c->phase_timing) (void)hipEventElapsedTime(&msb, c->ev[5], c->ev[6]);
     c->timing[6] = msb;
     if (flags) {
         c->have_step = false;
@@ -4005,6 +4011,12 @@ int sdfibm_buffer_capacity(sdfibm_ctx* c, int64_t* out, int n) {
     return k;
 }
 
+int sdfibm_set_phase_timing(sdfibm_ctx* c, int32_t on) {
+    if (!c) return SDFIBM_ERR_ARG;
+    c->phase_timing = on != 0;
+    return SDFIBM_OK;
+}
+
 int sdfibm_last_timing(sdfibm_ctx* c, double* ms, int n) {
     if (!c || !ms || n < 0) return SDFIBM_ERR_ARG;
     const int k = n < 7 ? n : 7;
@@ -4215,11 +4227,17 @@ struct Tuning {
 // context's current variant: the median of `reps` interacts after two untimed ones.
 int time_step(sdfibm_ctx* c, double dt, int reps, int phase, double& ms) {
     std::vector<double> t;
+    const bool was = c->phase_timing;
+    c->phase_timing = true;
     for (int r = 0; r < reps + 2; ++r) {
         const int rc = sdfibm_interact(c, 0.0, dt, nullptr);
-        if (rc) return rc;
+        if (rc) {
+            c->phase_timing = was;
+            return rc;
+        }
         if (r >= 2) t.push_back(c->timing[phase]);
     }
+    c->phase_timing = was;
     std::sort(t.begin(), t.end());
     ms = t[t.size() / 2];
     return SDFIBM_OK;

@@ -133,6 +133,7 @@ def test_polydisperse_csg_full_size():
         c.set_solids(sc.solids)
         c.set_velocity(U)
         print("[fullsize] interact", flush=True)
+        c.set_phase_timing(True)
         ft = c.interact(0.0, sc.dt)
         assert c.timing()[6] > 0
         big_pairs = [int(c.candidates(s)[0][:3].sum()) for s in range(50)]

@@ -52,6 +52,7 @@ def test_variants_agree(prec):
     m, table, sol, U = _mixed_big_scene()
     with SolidCloud(m, precision=prec) as c:
         _run(c, table, sol, U)
+        c.set_phase_timing(True)
         ref_ft = c.interact(0.0, 1e-3)
         ref = c.fields()
         for w, ch, g in ((0, 1024, 4), (1, 256, 2), (0, 4096, 8), (1, 64, 1)):
@@ -100,3 +101,25 @@ def test_interact_graph_replays_equal_eager(prec):
             c.interact(0.0, 1e-3)  # eager work between the replays
         pairs = c.pair_counts()
         assert sum(pairs) > 100000
+
+
+def test_phase_timing_switch():
+    """Phase timing is off by default (no events between the kernels: zeros), on request the
+    phases are timed, and the results are the same either way."""
+    sc = scenes.small_mixed_3d(32, seed=3, n_each=3)
+    with SolidCloud(sc.mesh, rho_f=sc.rho_f, precision="fp32") as c:
+        c.set_solids(sc.solids)
+        c.set_velocity(sc.U)
+        ft0 = c.interact(0.0, sc.dt)
+        f0 = c.fields()
+        assert np.all(c.timing() == 0)
+        c.set_phase_timing(True)
+        ft1 = c.interact(0.0, sc.dt)
+        t = c.timing()
+        assert t[3] > 0 and t[0] > 0 and t[2] > 0
+        np.testing.assert_array_equal(ft1, ft0)
+        for k in ("As", "Fs", "Ts", "Ct"):
+            np.testing.assert_array_equal(c.fields()[k], f0[k])
+        c.set_phase_timing(False)
+        c.interact(0.0, sc.dt)
+        assert np.all(c.timing() == 0)
