@@ -875,7 +875,7 @@ def main():
         per_kernel[k] = {"ms": kernels[k]["ms"], "achieved": a / 1e9, "frac": a / HBM_PEAK,
                          "algorithmic_bytes": kernels[k]["bytes"],
                          "traffic": measured_traffic(k, f"cloud3d_{args.n}_{args.solids}_{args.precision}")}
-    per_kernel["k_cells"]["limit"] = "VALU issue (about two thirds of the SIMD issue rate), not HBM"
+    per_kernel["k_cells"]["limit"] = "VALU issue (about two thirds of the SIMD issue rate) and the scattered U gathers of the candidate cells (measured traffic about 3x the algorithmic bytes)"
     line["roofline"]["per_kernel"] = per_kernel
     if args.precision == "fp32" and not args.no_surrogate:
         line["surrogate"] = surrogate_comparison(cloud, sc, ft, f["As"], args.steps)
